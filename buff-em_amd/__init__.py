@@ -1,0 +1,277 @@
+"""Python host bindings of the MI355X dense VIE hot path.
+
+Thin ctypes layer over the C ABI in include/buffem_amd.h (libbuffem_amd.so,
+built by buff-em_amd/build.py).  It mirrors the reference's scattering API
+(HomerReid/buff-em, src/libs/libbuff/libbuff.h:200-283):
+
+    SWGGeometry          -> Geometry (mesh, SWG topology, device upload)
+    AssembleVIEMatrix    -> Geometry.assemble_vie
+    AssembleRHSVector    -> Geometry.assemble_rhs
+    HMatrix::LUFactorize -> lu_factor
+    HMatrix::LUSolve     -> lu_solve
+    GetPFTMatrix (EMT)   -> Geometry.emt_power
+
+Device buffers are passed as raw pointers; torch tensors are accepted for
+convenience (complex128, contiguous).  A dense N x N matrix in the library's
+column-major layout with ld = N is a torch tensor T of shape (N, N) with
+T[c, r] = M(r, c), i.e. M = T.T.
+
+There is no CPU fallback: every compute call goes to the HIP library and
+raises if it is missing or fails.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libbuffem_amd.so")
+_lib = None
+
+
+class BFAError(RuntimeError):
+    pass
+
+
+class bfa_material(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("p", ctypes.c_double * 6)]
+
+
+class bfa_planewave(ctypes.Structure):
+    _fields_ = [("E0", ctypes.c_double * 6), ("dir", ctypes.c_double * 3)]
+
+
+P = ctypes.c_void_p
+I = ctypes.c_int
+I64 = ctypes.c_int64
+D = ctypes.c_double
+
+_SIGS = {
+    "bfa_last_error": (ctypes.c_char_p, []),
+    "bfa_status_string": (ctypes.c_char_p, [I]),
+    "bfa_version": (I, []),
+    "bfa_mesh_kuhn_sphere": (I, [I, D, D, ctypes.c_uint64, P, P, P, P]),
+    "bfa_mesh_read_vmsh": (I, [ctypes.c_char_p, P, P, P, P]),
+    "bfa_geometry_create": (I, [I, P, I, P, P]),
+    "bfa_geometry_destroy": (None, [P]),
+    "bfa_geometry_info": (I, [P, P, P, P, P]),
+    "bfa_geometry_faces": (I, [P, P, P, P, P, P, P, P, P]),
+    "bfa_geometry_touch_count": (I, [P, P]),
+    "bfa_geometry_touch_pairs": (I, [P, P, P]),
+    "bfa_geometry_upload": (I, [P, P]),
+    "bfa_static_build": (I, [P, I, P]),
+    "bfa_static_download": (I, [P, P]),
+    "bfa_assemble_vie": (I, [P, D, D, P, P, I64, P]),
+    "bfa_assemble_rhs": (I, [P, D, P, I, P, I64, P]),
+    "bfa_lu_factor_z": (I, [P, I, I64, P, P, P]),
+    "bfa_lu_solve_z": (I, [P, I, I64, P, P, I, I64, P]),
+    "bfa_emt_power": (I, [P, D, P, I, P, I64, P, P]),
+    "bfa_zgemm_sub": (I, [I, I, I, P, I64, P, I64, P, I64, P]),
+    "bfa_device_malloc": (I, [P, ctypes.c_size_t]),
+    "bfa_device_free": (I, [P]),
+    "bfa_memcpy_htod": (I, [P, P, ctypes.c_size_t, P]),
+    "bfa_memcpy_dtoh": (I, [P, P, ctypes.c_size_t, P]),
+    "bfa_stream_synchronize": (I, [P]),
+}
+
+
+def lib():
+    """Load libbuffem_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BFAError(f"{LIB_PATH} missing: run python buff-em_amd/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def declared_symbols():
+    return sorted(_SIGS)
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = lib().bfa_last_error().decode(errors="replace")
+        raise BFAError(f"{what} failed ({rc}): {msg}")
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        return x.data_ptr()
+    if isinstance(x, np.ndarray):
+        return x.ctypes.data
+    raise TypeError(type(x))
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    if hasattr(stream, "cuda_stream"):
+        return stream.cuda_stream
+    return int(stream)
+
+
+def material(kind="iso", **kw):
+    """Material descriptor: iso(eps), diag(eps=(ex,ey,ez)), sincos(base, amp, imag), janus(up, down)."""
+    m = bfa_material()
+    p = [0.0] * 6
+    if kind == "iso":
+        e = complex(kw["eps"])
+        m.kind, p[0], p[1] = 0, e.real, e.imag
+    elif kind == "diag":
+        es = [complex(x) for x in kw["eps"]]
+        m.kind = 1
+        p = [es[0].real, es[0].imag, es[1].real, es[1].imag, es[2].real, es[2].imag]
+    elif kind == "sincos":
+        m.kind = 2
+        p = [kw.get("base", 4.0), kw.get("imag", 0.1), kw.get("amp", 2.0), 0.0, 0.0, 0.0]
+    elif kind == "janus":
+        up, dn = complex(kw["up"]), complex(kw["down"])
+        m.kind = 3
+        p = [up.real, up.imag, dn.real, dn.imag, 0.0, 0.0]
+    else:
+        raise ValueError(kind)
+    for i in range(6):
+        m.p[i] = p[i]
+    return m
+
+
+def planewaves(specs):
+    """specs: list of (E0 3-vector (complex), direction 3-vector)."""
+    arr = (bfa_planewave * len(specs))()
+    for j, (e0, d) in enumerate(specs):
+        e0 = [complex(x) for x in e0]
+        for c in range(3):
+            arr[j].E0[2 * c] = e0[c].real
+            arr[j].E0[2 * c + 1] = e0[c].imag
+            arr[j].dir[c] = float(d[c])
+    return arr
+
+
+def kuhn_sphere(n, radius=1.0, jitter=0.05, seed=0x5EED):
+    """Kuhn-subdivided sphere mesh: returns (V[nv,3], tets[nt,4])."""
+    L = lib()
+    nv, nt = I(), I()
+    _check(L.bfa_mesh_kuhn_sphere(n, radius, jitter, seed, ctypes.byref(nv), ctypes.byref(nt), None, None),
+           "bfa_mesh_kuhn_sphere")
+    V = np.zeros((nv.value, 3))
+    T = np.zeros((nt.value, 4), dtype=np.int32)
+    _check(L.bfa_mesh_kuhn_sphere(n, radius, jitter, seed, ctypes.byref(nv), ctypes.byref(nt), V.ctypes.data,
+                                  T.ctypes.data), "bfa_mesh_kuhn_sphere")
+    return V, T
+
+
+def read_vmsh(path):
+    L = lib()
+    nv, nt = I(), I()
+    b = os.fsencode(path)
+    _check(L.bfa_mesh_read_vmsh(b, ctypes.byref(nv), ctypes.byref(nt), None, None), "bfa_mesh_read_vmsh")
+    V = np.zeros((nv.value, 3))
+    T = np.zeros((nt.value, 4), dtype=np.int32)
+    _check(L.bfa_mesh_read_vmsh(b, ctypes.byref(nv), ctypes.byref(nt), V.ctypes.data, T.ctypes.data),
+           "bfa_mesh_read_vmsh")
+    return V, T
+
+
+class Geometry:
+    """SWGGeometry counterpart for one object (src/libs/libbuff/SWGGeometry.cc:214-383)."""
+
+    def __init__(self, V, T):
+        L = lib()
+        self.V = np.ascontiguousarray(V, dtype=np.float64)
+        self.T = np.ascontiguousarray(T, dtype=np.int32)
+        h = P()
+        _check(L.bfa_geometry_create(len(self.V), self.V.ctypes.data, len(self.T), self.T.ctypes.data,
+                                     ctypes.byref(h)), "bfa_geometry_create")
+        self.h = h
+        nv, nt, nbf, nft = I(), I(), I(), I()
+        _check(L.bfa_geometry_info(h, ctypes.byref(nv), ctypes.byref(nt), ctypes.byref(nbf), ctypes.byref(nft)),
+               "bfa_geometry_info")
+        self.nv, self.nt, self.N, self.nfaces = nv.value, nt.value, nbf.value, nft.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bfa_geometry_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def faces(self):
+        n = self.nfaces
+        out = {k: np.zeros(n, dtype=np.int32) for k in ("QP", "QM", "PT", "MT", "PI", "MI")}
+        fV = np.zeros((n, 3), dtype=np.int32)
+        area = np.zeros(n)
+        _check(lib().bfa_geometry_faces(self.h, fV.ctypes.data, out["QP"].ctypes.data, out["QM"].ctypes.data,
+                                        out["PT"].ctypes.data, out["MT"].ctypes.data, out["PI"].ctypes.data,
+                                        out["MI"].ctypes.data, area.ctypes.data), "bfa_geometry_faces")
+        out["V"] = fV
+        out["Area"] = area
+        return out
+
+    def touch_pairs(self):
+        nnz = I64()
+        _check(lib().bfa_geometry_touch_count(self.h, ctypes.byref(nnz)), "bfa_geometry_touch_count")
+        rowptr = np.zeros(self.N + 1, dtype=np.int32)
+        cols = np.zeros(nnz.value, dtype=np.int32)
+        _check(lib().bfa_geometry_touch_pairs(self.h, rowptr.ctypes.data, cols.ctypes.data),
+               "bfa_geometry_touch_pairs")
+        return rowptr, cols
+
+    def upload(self, stream=None):
+        _check(lib().bfa_geometry_upload(self.h, _stream_ptr(stream)), "bfa_geometry_upload")
+
+    def build_static(self, td_order=0, stream=None):
+        _check(lib().bfa_static_build(self.h, int(td_order), _stream_ptr(stream)), "bfa_static_build")
+
+    def static_table(self):
+        nnz = I64()
+        _check(lib().bfa_geometry_touch_count(self.h, ctypes.byref(nnz)), "bfa_geometry_touch_count")
+        t = np.zeros((nnz.value, 6))
+        _check(lib().bfa_static_download(self.h, t.ctypes.data), "bfa_static_download")
+        return t
+
+    def assemble_vie(self, omega, mat, d_M, ld=None, stream=None):
+        omega = complex(omega)
+        _check(lib().bfa_assemble_vie(self.h, omega.real, omega.imag, ctypes.byref(mat), _ptr(d_M),
+                                      int(ld or self.N), _stream_ptr(stream)), "bfa_assemble_vie")
+
+    def assemble_rhs(self, omega, pws, d_B, ldb=None, stream=None):
+        _check(lib().bfa_assemble_rhs(self.h, float(omega), pws, len(pws), _ptr(d_B), int(ldb or self.N),
+                                      _stream_ptr(stream)), "bfa_assemble_rhs")
+
+    def emt_power(self, omega, pws, d_J, ldj=None, stream=None):
+        out = np.zeros(2 * len(pws))
+        _check(lib().bfa_emt_power(self.h, float(omega), pws, len(pws), _ptr(d_J), int(ldj or self.N),
+                                   out.ctypes.data, _stream_ptr(stream)), "bfa_emt_power")
+        return out.reshape(-1, 2)
+
+
+def lu_factor(d_A, n, d_ipiv, lda=None, stream=None):
+    info = I()
+    _check(lib().bfa_lu_factor_z(_ptr(d_A), int(n), int(lda or n), _ptr(d_ipiv), ctypes.byref(info),
+                                 _stream_ptr(stream)), "bfa_lu_factor_z")
+    return info.value
+
+
+def lu_solve(d_A, n, d_ipiv, d_B, nrhs, lda=None, ldb=None, stream=None):
+    _check(lib().bfa_lu_solve_z(_ptr(d_A), int(n), int(lda or n), _ptr(d_ipiv), _ptr(d_B), int(nrhs),
+                                int(ldb or n), _stream_ptr(stream)), "bfa_lu_solve_z")
+
+
+def zgemm_sub(m, n, k, d_A, lda, d_B, ldb, d_C, ldc, stream=None):
+    """C -= A @ B on the device (column-major complex128)."""
+    _check(lib().bfa_zgemm_sub(int(m), int(n), int(k), _ptr(d_A), int(lda), _ptr(d_B), int(ldb), _ptr(d_C),
+                               int(ldc), _stream_ptr(stream)), "bfa_zgemm_sub")

@@ -1,0 +1,593 @@
+// bfa_lu.hip -- blocked complex LU with partial pivoting and triangular
+// solves on CDNA4, in complex128 and complex64.
+//
+// Replaces HMatrix::LUFactorize / LUSolve (LAPACK zgetrf/zgetrs) as called
+// by the reference driver (src/applications/buff-scatter/buff-scatter.cc:285,
+// :314) and by multi-RHS callers (src/applications/buff-neq/GetFlux.cc:265).
+//
+// Layout: column-major, interleaved complex, leading dimension ld.
+// Algorithm: right-looking blocking (outer block NB); the panel is factored
+// recursively down to width PW, where a cooperative kernel keeps the panel's
+// rows resident in LDS across all workgroups and performs one grid barrier
+// per column (pivot = first index of max |re|+|im|, the izamax rule).  The
+// trailing update C -= A*B runs on MFMA (4 real products per complex
+// product, v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).
+#include <hip/hip_runtime.h>
+#include <climits>
+#include <cmath>
+#include <vector>
+
+#include "bfa_internal.h"
+#include "bfa_lu.h"
+
+using namespace bfa;
+
+#define HIPCHECK(call)                                                              \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      return fail(e_ == hipErrorOutOfMemory ? BFA_ERR_OOM : BFA_ERR_HIP, "%s: %s (%s:%d)", \
+                  #call, hipGetErrorString(e_), __FILE__, __LINE__);                \
+  } while (0)
+
+namespace {
+
+template <typename T> struct cplx { T re, im; };
+
+template <typename T> __device__ __forceinline__ cplx<T> c_mul(cplx<T> a, cplx<T> b)
+{ return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+template <typename T> __device__ __forceinline__ cplx<T> c_sub(cplx<T> a, cplx<T> b)
+{ return {a.re - b.re, a.im - b.im}; }
+template <typename T> __device__ __forceinline__ cplx<T> c_inv(cplx<T> a)
+{ T d = a.re * a.re + a.im * a.im; return {a.re / d, -a.im / d}; }
+template <typename T> __device__ __forceinline__ T c_abs1(cplx<T> a) { return fabs(a.re) + fabs(a.im); }
+
+// ---------------------------------------------------------------------------
+// MFMA wrappers: D(16x16) += A(16x4) * B(4x16).  Lane l supplies A[l%16][l/16]
+// and B[l/16][l%16]; D element (4*(l/16)+j, l%16) is register j of lane l.
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+template <typename T> struct mf;
+template <> struct mf<double> {
+  typedef d4 acc;
+  __device__ static __forceinline__ acc op(double a, double b, acc c)
+  { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+};
+template <> struct mf<float> {
+  typedef f4 acc;
+  __device__ static __forceinline__ acc op(float a, float b, acc c)
+  { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+};
+
+// ---------------------------------------------------------------------------
+// C(m x n) -= A(m x k) * B(k x n).  Workgroup tile 64 x 64 complex, 4 waves
+// in a 2 x 2 layout of 32 x 32 each, K staged through LDS 16 at a time with
+// real and imaginary planes stored separately ([k][row] and [k][col]).
+#define GBM 64
+#define GBN 64
+#define GBK 16
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_sub(int m, int n, int k, const cplx<T> *__restrict__ A, int64_t lda,
+                                                  const cplx<T> *__restrict__ B, int64_t ldb, cplx<T> *C, int64_t ldc)
+{
+  typedef typename mf<T>::acc acc_t;
+  __shared__ T As[2][GBK][GBM];
+  __shared__ T Bs[2][GBK][GBN];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int row0 = blockIdx.x * GBM, col0 = blockIdx.y * GBN;
+  const int l16 = lane & 15, lq = lane >> 4;
+  acc_t cr[2][2], ci[2][2];
+  // load C into accumulators
+#pragma unroll
+  for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++) {
+      const int col = col0 + wc * 32 + cb * 16 + l16;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int row = row0 + wr * 32 + rb * 16 + 4 * lq + j;
+        cplx<T> v = {0, 0};
+        if (row < m && col < n) v = C[(size_t)col * ldc + row];
+        cr[rb][cb][j] = v.re; ci[rb][cb][j] = v.im;
+      }
+    }
+  for (int k0 = 0; k0 < k; k0 += GBK) {
+    // A tile: 64 rows x 16 k; thread -> row tid%64, k = tid/64 + 4i
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int kk = (tid >> 6) + 4 * i, r = tid & 63;
+      cplx<T> v = {0, 0};
+      if (row0 + r < m && k0 + kk < k) v = A[(size_t)(k0 + kk) * lda + row0 + r];
+      As[0][kk][r] = v.re; As[1][kk][r] = v.im;
+    }
+    // B tile: 16 k x 64 cols; thread -> k = tid%16, col = tid/16 + 16i
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int kk = tid & 15, c = (tid >> 4) + 16 * i;
+      cplx<T> v = {0, 0};
+      if (col0 + c < n && k0 + kk < k) v = B[(size_t)(col0 + c) * ldb + k0 + kk];
+      Bs[0][kk][c] = v.re; Bs[1][kk][c] = v.im;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GBK; kk += 4) {
+      T ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+      for (int rb = 0; rb < 2; rb++) {
+        ar[rb] = -As[0][kk + lq][wr * 32 + rb * 16 + l16];
+        ai[rb] = As[1][kk + lq][wr * 32 + rb * 16 + l16];
+      }
+#pragma unroll
+      for (int cb = 0; cb < 2; cb++) {
+        br[cb] = Bs[0][kk + lq][wc * 32 + cb * 16 + l16];
+        bi[cb] = Bs[1][kk + lq][wc * 32 + cb * 16 + l16];
+      }
+#pragma unroll
+      for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++) {
+          // Cr -= ar*br - ai*bi ; Ci -= ar*bi + ai*br  (ar holds -Re A)
+          cr[rb][cb] = mf<T>::op(ar[rb], br[cb], cr[rb][cb]);
+          cr[rb][cb] = mf<T>::op(ai[rb], bi[cb], cr[rb][cb]);
+          ci[rb][cb] = mf<T>::op(ar[rb], bi[cb], ci[rb][cb]);
+          ci[rb][cb] = mf<T>::op(-ai[rb], br[cb], ci[rb][cb]);
+        }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++) {
+      const int col = col0 + wc * 32 + cb * 16 + l16;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int row = row0 + wr * 32 + rb * 16 + 4 * lq + j;
+        if (row < m && col < n) C[(size_t)col * ldc + row] = {cr[rb][cb][j], ci[rb][cb][j]};
+      }
+    }
+}
+
+template <typename T>
+static int gemm_sub(int m, int n, int k, const cplx<T> *A, int64_t lda, const cplx<T> *B, int64_t ldb, cplx<T> *C,
+                    int64_t ldc, hipStream_t st)
+{
+  if (m <= 0 || n <= 0 || k <= 0) return BFA_OK;
+  dim3 grid((m + GBM - 1) / GBM, (n + GBN - 1) / GBN);
+  hipLaunchKernelGGL(k_gemm_sub<T>, grid, dim3(256), 0, st, m, n, k, A, lda, B, ldb, C, ldc);
+  return BFA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// B(kb x n) := L^-1 B (L unit lower, kb <= 256) or U^-1 B (U upper, non-unit).
+// One workgroup per 16 columns of B, held in LDS; L/U read through the cache.
+#define TRSM_COLS 16
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void k_trsm(int kb, int n, const cplx<T> *L, int64_t ldl, cplx<T> *B, int64_t ldb)
+{
+  __shared__ cplx<T> X[256][TRSM_COLS];
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * TRSM_COLS;
+  const int nc = min(TRSM_COLS, n - c0);
+  for (int idx = tid; idx < kb * TRSM_COLS; idx += blockDim.x) {
+    const int r = idx % kb, c = idx / kb;
+    X[r][c] = c < nc ? B[(size_t)(c0 + c) * ldb + r] : cplx<T>{0, 0};
+  }
+  __syncthreads();
+  const int c = tid % TRSM_COLS, rg = tid / TRSM_COLS, nrg = blockDim.x / TRSM_COLS;
+  if (LOWER) {
+    for (int i = 0; i < kb - 1; i++) {
+      const cplx<T> xi = X[i][c];
+      for (int r = i + 1 + rg; r < kb; r += nrg) X[r][c] = c_sub(X[r][c], c_mul(L[(size_t)i * ldl + r], xi));
+      __syncthreads();
+    }
+  } else {
+    for (int i = kb - 1; i >= 0; i--) {
+      if (rg == 0) X[i][c] = c_mul(X[i][c], c_inv(L[(size_t)i * ldl + i]));
+      __syncthreads();
+      const cplx<T> xi = X[i][c];
+      for (int r = rg; r < i; r += nrg) X[r][c] = c_sub(X[r][c], c_mul(L[(size_t)i * ldl + r], xi));
+      __syncthreads();
+    }
+  }
+  for (int idx = tid; idx < kb * TRSM_COLS; idx += blockDim.x) {
+    const int r = idx % kb, cc = idx / kb;
+    if (cc < nc) B[(size_t)(c0 + cc) * ldb + r] = X[r][cc];
+  }
+}
+
+template <typename T, bool LOWER>
+static void trsm(int kb, int n, const cplx<T> *L, int64_t ldl, cplx<T> *B, int64_t ldb, hipStream_t st)
+{
+  if (kb <= 0 || n <= 0) return;
+  hipLaunchKernelGGL((k_trsm<T, LOWER>), dim3((n + TRSM_COLS - 1) / TRSM_COLS), dim3(256), 0, st, kb, n, L, ldl, B, ldb);
+}
+
+// ---------------------------------------------------------------------------
+// Row interchanges rows k0..k0+kb-1 <-> ipiv[.] (absolute, 0-based) on the
+// columns [c0, c1) of A; one thread per column, swaps applied in order.
+template <typename T>
+__global__ void k_laswp(cplx<T> *A, int64_t lda, int c0, int c1, int k0, int kb, const int *ipiv)
+{
+  const int c = c0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= c1) return;
+  cplx<T> *col = A + (size_t)c * lda;
+  for (int i = 0; i < kb; i++) {
+    const int p = ipiv[k0 + i];
+    if (p != k0 + i) {
+      cplx<T> t = col[k0 + i];
+      col[k0 + i] = col[p];
+      col[p] = t;
+    }
+  }
+}
+
+template <typename T>
+static void laswp(cplx<T> *A, int64_t lda, int c0, int c1, int k0, int kb, const int *ipiv, hipStream_t st)
+{
+  if (c1 <= c0 || kb <= 0) return;
+  hipLaunchKernelGGL(k_laswp<T>, dim3((c1 - c0 + 127) / 128), dim3(128), 0, st, A, lda, c0, c1, k0, kb, ipiv);
+}
+
+// ---------------------------------------------------------------------------
+// Cooperative panel factorisation of an m x w panel (w <= PW).  Workgroup g
+// owns rows [g*rpw, (g+1)*rpw) in LDS.  Per column: local argmax, publish
+// (value, row, row data) and, by the owner, the current row j; grid barrier;
+// every workgroup resolves the same pivot, swaps in LDS, scales and updates.
+#define PW 32
+#define SPIN_LIMIT (1u << 28)
+
+struct PanelWork {
+  double *val;      // [2][maxwg]
+  int *row;         // [2][maxwg]
+  void *rowdata;    // [2][maxwg][PW] complex
+  void *rowj;       // [2][PW] complex
+  unsigned *bar;    // barrier counter
+  int *flags;       // [0] spin timeout, [1] first zero pivot (1-based, 0 none)
+};
+
+__device__ __forceinline__ void grid_barrier(unsigned *bar, unsigned target, int *flags)
+{
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(bar, 1u);
+    unsigned spins = 0;
+    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (++spins > SPIN_LIMIT) { atomicOr(&flags[0], 1); break; }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  __syncthreads();
+  __threadfence();
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel(cplx<T> *A, int m, int w, int64_t lda, int *ipiv, int rpw, PanelWork pw,
+                                               int piv_off)
+{
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T> *rows = (cplx<T> *)smem_raw;                 // [rpw][w]
+  __shared__ cplx<T> prow[PW], rj[PW];
+  __shared__ T sval[256];
+  __shared__ int srow[256];
+  __shared__ int piv_s, win_s;
+  __shared__ T pval_s;
+  const int tid = threadIdx.x, g = blockIdx.x, nwg = gridDim.x;
+  const int r0 = g * rpw, r1 = min(m, r0 + rpw), nr = max(0, r1 - r0);
+  cplx<T> *rdata = (cplx<T> *)pw.rowdata;
+  cplx<T> *rowjg = (cplx<T> *)pw.rowj;
+  for (int idx = tid; idx < nr * w; idx += blockDim.x) {
+    const int r = idx % nr, c = idx / nr;
+    rows[r * w + c] = A[(size_t)c * lda + r0 + r];
+  }
+  __syncthreads();
+  for (int j = 0; j < w; j++) {
+    const int buf = j & 1;
+    // local argmax over rows >= j (first index of the maximum |re|+|im|)
+    T best = -1;
+    int brow = INT_MAX;
+    for (int r = tid; r < nr; r += blockDim.x) {
+      const int gr = r0 + r;
+      if (gr < j) continue;
+      const T v = c_abs1(rows[r * w + j]);
+      if (v > best || (v == best && gr < brow)) { best = v; brow = gr; }
+    }
+    sval[tid] = best; srow[tid] = brow;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+      if (tid < off) {
+        const T v = sval[tid + off];
+        const int rr = srow[tid + off];
+        if (v > sval[tid] || (v == sval[tid] && rr < srow[tid])) { sval[tid] = v; srow[tid] = rr; }
+      }
+      __syncthreads();
+    }
+    const int lrow = srow[0];
+    if (tid == 0) { pw.val[buf * nwg + g] = (double)sval[0]; pw.row[buf * nwg + g] = lrow; }
+    if (lrow != INT_MAX && tid < w) rdata[((size_t)buf * nwg + g) * PW + tid] = rows[(lrow - r0) * w + tid];
+    if (j >= r0 && j < r1 && tid < w) rowjg[buf * PW + tid] = rows[(j - r0) * w + tid];
+    grid_barrier(pw.bar, (unsigned)((j + 1) * nwg), pw.flags);
+    // global pivot (same result in every workgroup)
+    if (tid == 0) {
+      double bv = -1.0;
+      int bw = -1, br = INT_MAX;
+      for (int q = 0; q < nwg; q++) {
+        const double v = __hip_atomic_load(&pw.val[buf * nwg + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int rr = __hip_atomic_load(&pw.row[buf * nwg + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (rr == INT_MAX) continue;
+        if (v > bv || (v == bv && rr < br)) { bv = v; bw = q; br = rr; }
+      }
+      piv_s = br; win_s = bw; pval_s = (T)bv;
+    }
+    __syncthreads();
+    const int p = piv_s, win = win_s;
+    if (tid < w) {
+      prow[tid] = rdata[((size_t)buf * nwg + win) * PW + tid];
+      rj[tid] = rowjg[buf * PW + tid];
+    }
+    __syncthreads();
+    if (g == 0 && tid == 0) {
+      ipiv[j] = p + piv_off;
+      if (pval_s == (T)0 && pw.flags[1] == 0) pw.flags[1] = piv_off + j + 1;
+    }
+    // interchange rows j and p in LDS
+    if (p != j) {
+      if (j >= r0 && j < r1 && tid < w) rows[(j - r0) * w + tid] = prow[tid];
+      if (p >= r0 && p < r1 && tid < w) rows[(p - r0) * w + tid] = rj[tid];
+    }
+    __syncthreads();
+    // scale column j below the diagonal and update the rest of the panel
+    if (pval_s != (T)0) {
+      const cplx<T> inv = c_inv(prow[j]);
+      for (int r = tid; r < nr; r += blockDim.x) {
+        const int gr = r0 + r;
+        if (gr <= j) continue;
+        const cplx<T> l = c_mul(rows[r * w + j], inv);
+        rows[r * w + j] = l;
+        for (int c = j + 1; c < w; c++) rows[r * w + c] = c_sub(rows[r * w + c], c_mul(l, prow[c]));
+      }
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < nr * w; idx += blockDim.x) {
+    const int r = idx % nr, c = idx / nr;
+    A[(size_t)c * lda + r0 + r] = rows[r * w + c];
+  }
+}
+
+__global__ void k_zero_u32(unsigned *p, int n)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+__global__ void k_add_offset(int *ipiv, int n, int off)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ipiv[i] += off;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+namespace bfa {
+
+struct LUWork {
+  int device = -1, maxwg = 0;
+  PanelWork pw{};
+  void *mem = nullptr;
+  int num_cu = 0;
+};
+static LUWork g_lu[16];
+
+static int lu_work(LUWork *&w)
+{
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 16) return fail(BFA_ERR_ARG, "device index %d unsupported", dev);
+  w = &g_lu[dev];
+  if (w->mem) return BFA_OK;
+  hipDeviceProp_t prop;
+  HIPCHECK(hipGetDeviceProperties(&prop, dev));
+  w->num_cu = prop.multiProcessorCount;
+  w->maxwg = 1024;
+  const size_t bytes = 2 * w->maxwg * 8 + 2 * w->maxwg * 4 + 2 * (size_t)w->maxwg * PW * 16 + 2 * PW * 16 + 64 + 64;
+  HIPCHECK(hipMalloc(&w->mem, bytes));
+  char *p = (char *)w->mem;
+  w->pw.val = (double *)p; p += 2 * w->maxwg * 8;
+  w->pw.row = (int *)p; p += 2 * w->maxwg * 4;
+  w->pw.rowdata = p; p += 2 * (size_t)w->maxwg * PW * 16;
+  w->pw.rowj = p; p += 2 * PW * 16;
+  w->pw.bar = (unsigned *)p; p += 64;
+  w->pw.flags = (int *)p;
+  w->device = dev;
+  return BFA_OK;
+}
+
+template <typename T>
+static int panel_base(cplx<T> *A, int m, int w, int64_t lda, int *ipiv, int piv_off, LUWork *lw, hipStream_t st)
+{
+  // workgroups: one per CU, doubled until each workgroup's rows fit 64 KB of LDS
+  const size_t lds_cap = 64 * 1024;
+  int nwg = std::min(lw->num_cu, std::max(1, (m + 15) / 16));
+  int rpw = (m + nwg - 1) / nwg;
+  while ((size_t)rpw * w * sizeof(cplx<T>) > lds_cap && nwg < 2 * lw->num_cu) {
+    nwg = std::min(2 * nwg, 2 * lw->num_cu);
+    rpw = (m + nwg - 1) / nwg;
+  }
+  nwg = (m + rpw - 1) / rpw;
+  if ((size_t)rpw * w * sizeof(cplx<T>) > lds_cap || nwg > lw->maxwg)
+    return fail(BFA_ERR_ARG, "panel of %d rows too tall for LDS-resident factorisation", m);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, lw->pw.bar, 16);
+  size_t lds = (size_t)rpw * w * sizeof(cplx<T>);
+  void *args[] = {(void *)&A, (void *)&m, (void *)&w, (void *)&lda, (void *)&ipiv, (void *)&rpw, (void *)&lw->pw,
+                  (void *)&piv_off};
+  hipError_t e = hipLaunchCooperativeKernel((const void *)k_panel<T>, dim3(nwg), dim3(256), args, (unsigned)lds, st);
+  if (e != hipSuccess) return fail(BFA_ERR_HIP, "cooperative panel launch (%d workgroups): %s", nwg, hipGetErrorString(e));
+  return BFA_OK;
+}
+
+// recursive panel: m x w, pivots written absolute (offset piv_off + local)
+template <typename T>
+static int panel_rec(cplx<T> *A, int m, int w, int64_t lda, int *ipiv, int piv_off, LUWork *lw, hipStream_t st)
+{
+  if (w <= PW) return panel_base<T>(A, m, w, lda, ipiv, piv_off, lw, st);
+  const int w1 = w / 2, w2 = w - w1;
+  int rc = panel_rec<T>(A, m, w1, lda, ipiv, piv_off, lw, st);
+  if (rc) return rc;
+  // apply left pivots to the right half (rows relative to panel top)
+  cplx<T> *Ar = A + (size_t)w1 * lda;
+  hipLaunchKernelGGL(k_laswp<T>, dim3((w2 + 127) / 128), dim3(128), 0, st, Ar - (ptrdiff_t)piv_off, lda, 0, w2, piv_off,
+                     w1, (const int *)(ipiv - piv_off));
+  trsm<T, true>(w1, w2, A, lda, Ar, lda, st);
+  gemm_sub<T>(m - w1, w2, w1, A + w1, lda, Ar, lda, Ar + w1, lda, st);
+  rc = panel_rec<T>(A + (size_t)w1 * lda + w1, m - w1, w2, lda, ipiv + w1, piv_off + w1, lw, st);
+  if (rc) return rc;
+  // apply right pivots to the left half
+  hipLaunchKernelGGL(k_laswp<T>, dim3((w1 + 127) / 128), dim3(128), 0, st, A - (ptrdiff_t)piv_off, lda, 0, w1,
+                     piv_off + w1, w2, (const int *)(ipiv - piv_off));
+  return BFA_OK;
+}
+
+template <typename T>
+int lu_factor(cplx<T> *A, int n, int64_t lda, int *ipiv, int *info, hipStream_t st, int nb)
+{
+  LUWork *lw = nullptr;
+  int rc = lu_work(lw);
+  if (rc) return rc;
+  HIPCHECK(hipMemsetAsync(lw->pw.flags, 0, 8, st));
+  for (int j = 0; j < n; j += nb) {
+    const int jb = std::min(nb, n - j);
+    cplx<T> *Ajj = A + (size_t)j * lda + j;
+    // pivots of the panel are produced relative to row j, stored absolute
+    rc = panel_rec<T>(Ajj, n - j, jb, lda, ipiv + j, j, lw, st);
+    if (rc) return rc;
+    // rows j..j+jb-1 of the columns left and right of the panel
+    laswp<T>(A, lda, 0, j, j, jb, ipiv, st);
+    laswp<T>(A, lda, j + jb, n, j, jb, ipiv, st);
+    if (j + jb < n) {
+      trsm<T, true>(jb, n - j - jb, Ajj, lda, Ajj + (size_t)jb * lda, lda, st);
+      gemm_sub<T>(n - j - jb, n - j - jb, jb, Ajj + jb, lda, Ajj + (size_t)jb * lda, lda,
+                  Ajj + (size_t)jb * lda + jb, lda, st);
+    }
+  }
+  HIPCHECK(hipGetLastError());
+  int flags[2] = {0, 0};
+  HIPCHECK(hipMemcpyAsync(flags, lw->pw.flags, 8, hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  if (flags[0]) return fail(BFA_ERR_HIP, "LU panel barrier timed out");
+  if (info) *info = flags[1];
+  return BFA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Solve: permutation from ipiv (composed on the host), row gather, blocked
+// forward (unit L) and backward (U) substitution.
+template <typename T>
+__global__ void k_gather_rows(const cplx<T> *B, int64_t ldb, cplx<T> *X, int64_t ldx, int n, int nrhs, const int *perm)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i < n && j < nrhs) X[(size_t)j * ldx + i] = B[(size_t)j * ldb + perm[i]];
+}
+
+template <typename T>
+int lu_solve(const cplx<T> *A, int n, int64_t lda, const int *ipiv, cplx<T> *B, int nrhs, int64_t ldb, hipStream_t st,
+             void *scratch)
+{
+  const int nb = 256;
+  int *perm = (int *)scratch;
+  cplx<T> *X = (cplx<T> *)((char *)scratch + (((size_t)n * 4 + 255) / 256) * 256);
+  {
+    std::vector<int> hp(n), pm(n);
+    HIPCHECK(hipMemcpyAsync(hp.data(), ipiv, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    for (int i = 0; i < n; i++) pm[i] = i;
+    for (int i = 0; i < n; i++) {
+      const int p = hp[i];
+      if (p < 0 || p >= n) return fail(BFA_ERR_ARG, "invalid pivot %d at %d", p, i);
+      if (p != i) std::swap(pm[i], pm[p]);
+    }
+    HIPCHECK(hipMemcpyAsync(perm, pm.data(), (size_t)n * 4, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipStreamSynchronize(st));
+  }
+  hipLaunchKernelGGL(k_gather_rows<T>, dim3((n + 255) / 256, nrhs), dim3(256), 0, st, B, ldb, X, (int64_t)n, n, nrhs,
+                     (const int *)perm);
+  for (int k = 0; k < n; k += nb) {
+    const int kb = std::min(nb, n - k);
+    trsm<T, true>(kb, nrhs, A + (size_t)k * lda + k, lda, X + k, n, st);
+    if (k + kb < n) gemm_sub<T>(n - k - kb, nrhs, kb, A + (size_t)k * lda + k + kb, lda, X + k, n, X + k + kb, n, st);
+  }
+  const int nblk = (n + nb - 1) / nb;
+  for (int bi = nblk - 1; bi >= 0; bi--) {
+    const int k = bi * nb, kb = std::min(nb, n - k);
+    trsm<T, false>(kb, nrhs, A + (size_t)k * lda + k, lda, X + k, n, st);
+    if (k > 0) gemm_sub<T>(k, nrhs, kb, A + (size_t)k * lda, lda, X + k, n, X, n, st);
+  }
+  // copy back
+  HIPCHECK(hipMemcpy2DAsync(B, ldb * sizeof(cplx<T>), X, n * sizeof(cplx<T>), n * sizeof(cplx<T>), nrhs,
+                            hipMemcpyDeviceToDevice, st));
+  HIPCHECK(hipGetLastError());
+  return BFA_OK;
+}
+
+size_t lu_solve_scratch_bytes(int n, int nrhs, size_t elem)
+{
+  return (((size_t)n * 4 + 255) / 256) * 256 + (size_t)n * nrhs * elem;
+}
+
+int gemm_sub_z(int m, int n, int k, const void *A, int64_t lda, const void *B, int64_t ldb, void *C, int64_t ldc,
+               hipStream_t st)
+{
+  return gemm_sub<double>(m, n, k, (const cplx<double> *)A, lda, (const cplx<double> *)B, ldb, (cplx<double> *)C, ldc,
+                          st);
+}
+int gemm_sub_c(int m, int n, int k, const void *A, int64_t lda, const void *B, int64_t ldb, void *C, int64_t ldc,
+               hipStream_t st)
+{
+  return gemm_sub<float>(m, n, k, (const cplx<float> *)A, lda, (const cplx<float> *)B, ldb, (cplx<float> *)C, ldc, st);
+}
+int lu_factor_z(void *A, int n, int64_t lda, int *ipiv, int *info, hipStream_t st, int nb)
+{ return lu_factor<double>((cplx<double> *)A, n, lda, ipiv, info, st, nb); }
+int lu_factor_c(void *A, int n, int64_t lda, int *ipiv, int *info, hipStream_t st, int nb)
+{ return lu_factor<float>((cplx<float> *)A, n, lda, ipiv, info, st, nb); }
+int lu_solve_z(const void *A, int n, int64_t lda, const int *ipiv, void *B, int nrhs, int64_t ldb, hipStream_t st, void *s)
+{ return lu_solve<double>((const cplx<double> *)A, n, lda, ipiv, (cplx<double> *)B, nrhs, ldb, st, s); }
+int lu_solve_c(const void *A, int n, int64_t lda, const int *ipiv, void *B, int nrhs, int64_t ldb, hipStream_t st, void *s)
+{ return lu_solve<float>((const cplx<float> *)A, n, lda, ipiv, (cplx<float> *)B, nrhs, ldb, st, s); }
+
+}  // namespace bfa
+
+// ---------------------------------------------------------------------------
+extern "C" int bfa_lu_factor_z(void *d_A, int n, int64_t lda, int *d_ipiv, int *info, void *stream)
+{
+  if (!d_A || !d_ipiv || n <= 0 || lda < n) return fail(BFA_ERR_ARG, "bfa_lu_factor_z: bad argument");
+  return bfa::lu_factor_z(d_A, n, lda, d_ipiv, info, (hipStream_t)stream, 256);
+}
+
+extern "C" int bfa_lu_solve_z(const void *d_A, int n, int64_t lda, const int *d_ipiv, void *d_B, int nrhs, int64_t ldb,
+                              void *stream)
+{
+  if (!d_A || !d_ipiv || !d_B || n <= 0 || nrhs <= 0 || lda < n || ldb < n)
+    return fail(BFA_ERR_ARG, "bfa_lu_solve_z: bad argument");
+  void *scratch = nullptr;
+  HIPCHECK(hipMalloc(&scratch, bfa::lu_solve_scratch_bytes(n, nrhs, 16)));
+  int rc = bfa::lu_solve_z(d_A, n, lda, d_ipiv, d_B, nrhs, ldb, (hipStream_t)stream, scratch);
+  hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+  hipFree(scratch);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(BFA_ERR_HIP, "bfa_lu_solve_z: %s", hipGetErrorString(e));
+  return BFA_OK;
+}
+
+extern "C" int bfa_zgemm_sub(int m, int n, int k, const void *d_A, int64_t lda, const void *d_B, int64_t ldb,
+                             void *d_C, int64_t ldc, void *stream)
+{
+  if (m < 0 || n < 0 || k < 0 || (m > 0 && (lda < m || ldc < m)) || (k > 0 && ldb < k))
+    return fail(BFA_ERR_ARG, "bfa_zgemm_sub: bad sizes");
+  bfa::gemm_sub_z(m, n, k, d_A, lda, d_B, ldb, d_C, ldc, (hipStream_t)stream);
+  HIPCHECK(hipGetLastError());
+  return BFA_OK;
+}

@@ -1,0 +1,138 @@
+/*
+ * buffem_amd.h -- C ABI of the MI355X-native dense VIE hot path.
+ *
+ * This is the drop-in boundary beneath the SWGGeometry / HMatrix API of
+ * HomerReid/buff-em (src/libs/libbuff/libbuff.h:200-283 and SCUFF-EM's
+ * HMatrix::LUFactorize/LUSolve as called from
+ * src/applications/buff-scatter/buff-scatter.cc:236-314).  Every entry point
+ * takes plain pointers and sizes; device buffers are raw HIP device pointers,
+ * streams are hipStream_t passed as void* (NULL = default stream).
+ *
+ * Conventions
+ *  - Dense matrices are complex128 (interleaved re,im doubles), column-major
+ *    with leading dimension ld (in elements).  Entry (r,c) is at
+ *    ((size_t)c*ld + r).  Complex64 matrices use the same layout with floats.
+ *  - Basis-function (BF) numbering follows the reference's InitFaceList
+ *    (interior faces in order of their second tetrahedron visit,
+ *    src/libs/libbuff/InitFaceList.cc:141-260).
+ *  - Pivot vectors are 0-based: row i was interchanged with row ipiv[i].
+ *  - Every function returns a status code (BFA_OK == 0); nothing calls
+ *    exit().  bfa_last_error() returns a message for the calling thread.
+ *    This replaces ErrExit (src/libs/libbuff/SWGVolume.cc:107,164-174,
+ *    src/libs/libbuff/TetCR.cc:104).
+ */
+#ifndef BUFFEM_AMD_H
+#define BUFFEM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BFA_OK             0
+#define BFA_ERR_ARG       -1   /* invalid argument / size                     */
+#define BFA_ERR_MESH      -2   /* invalid mesh (file format, topology)        */
+#define BFA_ERR_SINGULAR  -3   /* exactly zero pivot in LU                    */
+#define BFA_ERR_OOM       -4   /* device or host allocation failed            */
+#define BFA_ERR_HIP       -5   /* HIP runtime error                           */
+#define BFA_ERR_STATE     -6   /* call order (e.g. static table not built)    */
+#define BFA_ERR_IO        -7   /* file could not be opened / read             */
+
+typedef struct bfa_geometry bfa_geometry;
+
+/* Isotropic / diagonal material eps(x), frequency independent closed forms.
+ * Stands in for SVTensor::Evaluate (src/libs/libbuff/SVTensor.cc:306-373)
+ * for the material families of the benchmark configurations.
+ *   kind 0: eps = p0 + i p1
+ *   kind 1: diag(p0+i p1, p2+i p3, p4+i p5)
+ *   kind 2: eps(x) = p0 + p2 sin(pi x) cos(pi y) + i p1
+ *   kind 3: Janus, z > 0: p0 + i p1, otherwise p2 + i p3                   */
+typedef struct { int kind; double p[6]; } bfa_material;
+
+/* Plane wave E = E0 exp(i omega dir.x); E0 complex (re,im per component).
+ * Stands in for SCUFF-EM's PlaneWave (buff-scatter.cc:186).               */
+typedef struct { double E0[6]; double dir[3]; } bfa_planewave;
+
+const char *bfa_last_error(void);
+const char *bfa_status_string(int status);
+int bfa_version(void);
+
+/* ---- mesh construction (host only) ------------------------------------ */
+/* Kuhn-subdivided cube [-1,1]^3 with n^3 subcubes x 6 tetrahedra, radially
+ * mapped onto a sphere of radius R, interior nodes jittered by
+ * U(-jitter,jitter)*(2R/n) per coordinate from splitmix64(seed); tetrahedra
+ * positively oriented.  Pass V/tetVI == NULL to query nv/nt.             */
+int bfa_mesh_kuhn_sphere(int n, double radius, double jitter, uint64_t seed,
+                         int *nv, int *nt, double *V, int *tetVI);
+/* GMSH .msh/.vmsh reader with 1e-6 vertex merging
+ * (src/libs/libbuff/ReadGMSHFile.cc:66-261).  V/tetVI NULL: size query.   */
+int bfa_mesh_read_vmsh(const char *path, int *nv, int *nt, double *V, int *tetVI);
+
+/* ---- geometry: SWGGeometry/SWGVolume (SWGGeometry.cc:214-383,
+ *      SWGVolume.cc:90-181) --------------------------------------------- */
+int  bfa_geometry_create(int nv, const double *V, int nt, const int *tetVI,
+                         bfa_geometry **out);
+void bfa_geometry_destroy(bfa_geometry *g);
+int  bfa_geometry_info(const bfa_geometry *g, int *nv, int *nt, int *nbf,
+                       int *nfaces_total);
+/* face tables (any pointer may be NULL), sizes nfaces_total (x3 for fV)   */
+int  bfa_geometry_faces(const bfa_geometry *g, int *fV, int *fQP, int *fQM,
+                        int *fPT, int *fMT, int *fPI, int *fMI, double *fArea);
+/* touching BF pairs a<=b (CompareBFs > 0, SWGVolume.cc:295-344) as CSR    */
+int  bfa_geometry_touch_count(const bfa_geometry *g, int64_t *nnz);
+int  bfa_geometry_touch_pairs(const bfa_geometry *g, int *rowptr, int *cols);
+/* upload topology and cubature tables to the current HIP device           */
+int  bfa_geometry_upload(bfa_geometry *g, void *stream);
+
+/* ---- cold step: frequency-independent static table ------------------- */
+/* One record of 6 doubles per touching pair, the FIBBI data of
+ * FIBBICache::GetFIBBIData / ComputeFIBBIData (FIBBICache.cc:211-245,
+ * GMatrixElements.cc:295-375); Taylor-Duffy order td_order (<=0: default). */
+int  bfa_static_build(bfa_geometry *g, int td_order, void *stream);
+int  bfa_static_download(const bfa_geometry *g, double *table /* nnz*6 */);
+
+/* ---- warm step: VIE matrix --------------------------------------------- */
+/* SWGGeometry::AssembleVIEMatrix (VIEMatrix.cc:431-460): d_M (N x N,
+ * complex128, ld >= N) receives G + chi^-1 block, both triangles.        */
+int  bfa_assemble_vie(bfa_geometry *g, double omega_re, double omega_im,
+                      const bfa_material *mat, void *d_M, int64_t ld, void *stream);
+
+/* SWGGeometry::AssembleRHSVector (RHSVector.cc:85-124) for npw incident
+ * plane waves, one column each: d_B is N x npw complex128, ldb >= N.      */
+int  bfa_assemble_rhs(bfa_geometry *g, double omega, const bfa_planewave *pw,
+                      int npw, void *d_B, int64_t ldb, void *stream);
+
+/* ---- dense solve: HMatrix::LUFactorize / LUSolve ----------------------- */
+/* In-place partial-pivot LU of a complex128 matrix (zgetrf semantics,
+ * buff-scatter.cc:285).  d_ipiv: n ints on device.  info (host, may be
+ * NULL): 0, or k+1 if U(k,k) is exactly zero.                              */
+int  bfa_lu_factor_z(void *d_A, int n, int64_t lda, int *d_ipiv, int *info,
+                     void *stream);
+/* LUSolve(HVector*)/LUSolve(HMatrix*) (buff-scatter.cc:314,
+ * buff-neq GetFlux.cc:265): in-place multi-RHS solve of d_B (n x nrhs).   */
+int  bfa_lu_solve_z(const void *d_A, int n, int64_t lda, const int *d_ipiv,
+                    void *d_B, int nrhs, int64_t ldb, void *stream);
+
+/* Dense complex128 product C(m x n) -= A(m x k) * B(k x n), column-major;
+ * the MFMA kernel of the LU trailing update (HMatrix::Multiply-style use).  */
+int  bfa_zgemm_sub(int m, int n, int k, const void *d_A, int64_t lda, const void *d_B,
+                   int64_t ldb, void *d_C, int64_t ldc, void *stream);
+
+/* ---- power (GetPFTMatrix with EMT, GetPFT.cc:133-189, EMTPFT.cc:324-546) */
+/* out[2*j] = P_abs, out[2*j+1] = P_scat for RHS column j (host array).    */
+int  bfa_emt_power(bfa_geometry *g, double omega, const bfa_planewave *pw,
+                   int npw, const void *d_J, int64_t ldj, double *out, void *stream);
+
+/* ---- device memory helpers (for hosts without a tensor library) -------- */
+int  bfa_device_malloc(void **ptr, size_t bytes);
+int  bfa_device_free(void *ptr);
+int  bfa_memcpy_htod(void *dst, const void *src, size_t bytes, void *stream);
+int  bfa_memcpy_dtoh(void *dst, const void *src, size_t bytes, void *stream);
+int  bfa_stream_synchronize(void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BUFFEM_AMD_H */

@@ -1,0 +1,228 @@
+"""Python driver of the CPU oracle.
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg as the checker, never by the product path.
+
+Wraps liboracle.so (the C restatement of the reference algorithm, see
+buffem_oracle.h) and adds the dense solve through LAPACK (scipy's zgetrf /
+zgetrs), which is what the reference's HMatrix::LUFactorize / LUSolve call
+(src/applications/buff-scatter/buff-scatter.cc:285,314).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+P = ctypes.c_void_p
+I = ctypes.c_int
+D = ctypes.c_double
+
+
+class orc_material(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("p", ctypes.c_double * 6)]
+
+
+class orc_planewave(ctypes.Structure):
+    _fields_ = [("E0", ctypes.c_double * 6), ("dir", ctypes.c_double * 3)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        sig = {
+            "orc_geom_build": (I, [I, P, I, P, P]),
+            "orc_geom_free": (None, [P]),
+            "orc_geom_info": (I, [P, P, P, P, P]),
+            "orc_geom_faces": (None, [P, P, P, P, P, P, P, P, P, P]),
+            "orc_compare_bfs": (I, [P, I, I]),
+            "orc_compare_tets": (I, [P, I, I, P, P]),
+            "orc_tetcr": (P, [I]),
+            "orc_exprel": (None, [D, D, I, I, P]),
+            "orc_td_moments": (None, [I, P, P, P, P, I, P]),
+            "orc_fibbi": (None, [P, I, I, I, P]),
+            "orc_gme": (None, [P, I, I, D, D, P, P]),
+            "orc_touch_pairs": (I, [P, P, P]),
+            "orc_static_table": (None, [P, P, P, I, P]),
+            "orc_assemble": (None, [P, D, D, P, P, P, P, P]),
+            "orc_overlaps": (I, [P, I, D, P, P, P]),
+            "orc_rhs": (None, [P, D, P, I, P]),
+            "orc_emt_power": (None, [P, D, P, I, P, P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def material(kind="iso", **kw):
+    m = orc_material()
+    p = [0.0] * 6
+    if kind == "iso":
+        e = complex(kw["eps"])
+        m.kind, p[0], p[1] = 0, e.real, e.imag
+    elif kind == "diag":
+        es = [complex(x) for x in kw["eps"]]
+        m.kind = 1
+        p = [es[0].real, es[0].imag, es[1].real, es[1].imag, es[2].real, es[2].imag]
+    elif kind == "sincos":
+        m.kind = 2
+        p = [kw.get("base", 4.0), kw.get("imag", 0.1), kw.get("amp", 2.0), 0, 0, 0]
+    elif kind == "janus":
+        up, dn = complex(kw["up"]), complex(kw["down"])
+        m.kind = 3
+        p = [up.real, up.imag, dn.real, dn.imag, 0, 0]
+    for i in range(6):
+        m.p[i] = p[i]
+    return m
+
+
+def planewaves(specs):
+    arr = (orc_planewave * len(specs))()
+    for j, (e0, d) in enumerate(specs):
+        e0 = [complex(x) for x in e0]
+        for c in range(3):
+            arr[j].E0[2 * c] = e0[c].real
+            arr[j].E0[2 * c + 1] = e0[c].imag
+            arr[j].dir[c] = float(d[c])
+    return arr
+
+
+def tetcr(npts):
+    p = lib().orc_tetcr(npts)
+    return np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_double)), shape=(npts, 4)).copy()
+
+
+def exprel3(z):
+    out = np.zeros(2)
+    lib().orc_exprel(z.real, z.imag, 3, 3, out.ctypes.data)
+    return complex(out[0], out[1])
+
+
+def td_moments(ncv, VA, VB, cA, cB, order):
+    out = np.zeros((3, 8))
+    VA = np.ascontiguousarray(VA, float)
+    VB = np.ascontiguousarray(VB, float)
+    cA = np.ascontiguousarray(cA, float)
+    cB = np.ascontiguousarray(cB, float)
+    lib().orc_td_moments(ncv, VA.ctypes.data, VB.ctypes.data, cA.ctypes.data, cB.ctypes.data, order,
+                         out.ctypes.data)
+    return out
+
+
+class Geometry:
+    def __init__(self, V, T):
+        self.V = np.ascontiguousarray(V, float)
+        self.T = np.ascontiguousarray(T, np.int32)
+        h = P()
+        rc = lib().orc_geom_build(len(self.V), self.V.ctypes.data, len(self.T), self.T.ctypes.data,
+                                  ctypes.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"orc_geom_build failed ({rc})")
+        self.h = h
+        nv, nt, nf, nft = I(), I(), I(), I()
+        lib().orc_geom_info(h, ctypes.byref(nv), ctypes.byref(nt), ctypes.byref(nf), ctypes.byref(nft))
+        self.nv, self.nt, self.N, self.nfaces = nv.value, nt.value, nf.value, nft.value
+        self._touch = None
+
+    def __del__(self):
+        try:
+            lib().orc_geom_free(self.h)
+        except Exception:
+            pass
+
+    def faces(self):
+        n = self.nfaces
+        out = {k: np.zeros(n, dtype=np.int32) for k in ("QP", "QM", "PT", "MT", "PI", "MI")}
+        fV = np.zeros((n, 3), dtype=np.int32)
+        area = np.zeros(n)
+        tetFI = np.zeros((self.nt, 4), dtype=np.int32)
+        lib().orc_geom_faces(self.h, fV.ctypes.data, out["QP"].ctypes.data, out["QM"].ctypes.data,
+                             out["PT"].ctypes.data, out["MT"].ctypes.data, out["PI"].ctypes.data,
+                             out["MI"].ctypes.data, area.ctypes.data, tetFI.ctypes.data)
+        out["V"] = fV
+        out["Area"] = area
+        out["tetFI"] = tetFI
+        return out
+
+    def touch_pairs(self):
+        if self._touch is None:
+            rowptr = np.zeros(self.N + 1, dtype=np.int32)
+            nnz = lib().orc_touch_pairs(self.h, rowptr.ctypes.data, None)
+            cols = np.zeros(nnz, dtype=np.int32)
+            lib().orc_touch_pairs(self.h, rowptr.ctypes.data, cols.ctypes.data)
+            self._touch = (rowptr, cols)
+        return self._touch
+
+    def compare_bfs(self, a, b):
+        return lib().orc_compare_bfs(self.h, a, b)
+
+    def static_table(self, td_order=12):
+        rowptr, cols = self.touch_pairs()
+        t = np.zeros((len(cols), 6))
+        lib().orc_static_table(self.h, rowptr.ctypes.data, cols.ctypes.data, td_order, t.ctypes.data)
+        return t
+
+    def fibbi(self, a, b, td_order=12):
+        G = np.zeros(6)
+        lib().orc_fibbi(self.h, a, b, td_order, G.ctypes.data)
+        return G
+
+    def gme(self, a, b, omega, fibbi=None):
+        out = np.zeros(2)
+        omega = complex(omega)
+        fp = None if fibbi is None else np.ascontiguousarray(fibbi, float)
+        lib().orc_gme(self.h, a, b, omega.real, omega.imag, None if fp is None else fp.ctypes.data,
+                      out.ctypes.data)
+        return complex(out[0], out[1])
+
+    def overlaps(self, a, omega, mat):
+        idx = np.zeros(7, dtype=np.int32)
+        val = np.zeros((7, 2))
+        n = lib().orc_overlaps(self.h, a, float(omega), ctypes.byref(mat), idx.ctypes.data, val.ctypes.data)
+        return idx[:n], val[:n, 0] + 1j * val[:n, 1]
+
+    def assemble(self, omega, mat, table=None, td_order=12):
+        """Dense VIE matrix as a numpy array M[r, c] (complex128)."""
+        rowptr, cols = self.touch_pairs()
+        if table is None:
+            table = self.static_table(td_order)
+        table = np.ascontiguousarray(table, float)
+        N = self.N
+        Mcol = np.zeros((N, N), dtype=np.complex128)       # Mcol[c, r] = M(r, c)
+        omega = complex(omega)
+        lib().orc_assemble(self.h, omega.real, omega.imag, ctypes.byref(mat), rowptr.ctypes.data,
+                           cols.ctypes.data, table.ctypes.data, Mcol.ctypes.data)
+        return Mcol.T.copy()
+
+    def rhs(self, omega, pws):
+        N = self.N
+        B = np.zeros((len(pws), N), dtype=np.complex128)
+        lib().orc_rhs(self.h, float(omega), pws, len(pws), B.ctypes.data)
+        return B.T.copy()
+
+    def emt_power(self, omega, pws, J):
+        Jc = np.ascontiguousarray(np.asarray(J).T, dtype=np.complex128)   # columns contiguous
+        out = np.zeros(2 * len(pws))
+        lib().orc_emt_power(self.h, float(omega), pws, len(pws), Jc.ctypes.data, out.ctypes.data)
+        return out.reshape(-1, 2)
+
+
+def lu_solve(M, B):
+    """LAPACK zgetrf + zgetrs (the reference's HMatrix::LUFactorize/LUSolve)."""
+    import scipy.linalg as sl
+    lu, piv = sl.lu_factor(M, check_finite=False)
+    return sl.lu_solve((lu, piv), B, check_finite=False)
