@@ -67,6 +67,9 @@ _SIGS = {
     "bfa_lu_solve_z": (I, [P, I, I64, P, P, I, I64, P]),
     "bfa_emt_power": (I, [P, D, P, I, P, I64, P, P]),
     "bfa_zgemm_sub": (I, [I, I, I, P, I64, P, I64, P, I64, P]),
+    "bfa_cgemm_sub": (I, [I, I, I, P, I64, P, I64, P, I64, P]),
+    "bfa_profile_enable": (I, [I]),
+    "bfa_profile_read": (I, [I, P, P, P]),
     "bfa_device_malloc": (I, [P, ctypes.c_size_t]),
     "bfa_device_free": (I, [P]),
     "bfa_memcpy_htod": (I, [P, P, ctypes.c_size_t, P]),
@@ -275,3 +278,21 @@ def zgemm_sub(m, n, k, d_A, lda, d_B, ldb, d_C, ldc, stream=None):
     """C -= A @ B on the device (column-major complex128)."""
     _check(lib().bfa_zgemm_sub(int(m), int(n), int(k), _ptr(d_A), int(lda), _ptr(d_B), int(ldb), _ptr(d_C),
                                int(ldc), _stream_ptr(stream)), "bfa_zgemm_sub")
+
+
+def cgemm_sub(m, n, k, d_A, lda, d_B, ldb, d_C, ldc, stream=None):
+    """C -= A @ B on the device (column-major complex64)."""
+    _check(lib().bfa_cgemm_sub(int(m), int(n), int(k), _ptr(d_A), int(lda), _ptr(d_B), int(ldb), _ptr(d_C),
+                               int(ldc), _stream_ptr(stream)), "bfa_cgemm_sub")
+
+
+def profile_enable(on=True):
+    _check(lib().bfa_profile_enable(1 if on else 0), "bfa_profile_enable")
+
+
+def profile_read(which=0):
+    """(kernel ms, algorithmic flops, launches) of the GEMM kernel since the last read."""
+    ms, fl, nl = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+    _check(lib().bfa_profile_read(int(which), ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(nl)),
+           "bfa_profile_read")
+    return ms.value, fl.value, nl.value

@@ -44,7 +44,9 @@ template <typename T> __device__ __forceinline__ T c_abs1(cplx<T> a) { return fa
 
 // ---------------------------------------------------------------------------
 // MFMA wrappers: D(16x16) += A(16x4) * B(4x16).  Lane l supplies A[l%16][l/16]
-// and B[l/16][l%16]; D element (4*(l/16)+j, l%16) is register j of lane l.
+// and B[l/16][l%16]; register j of lane l holds D(row(j, l/16), l%16), where
+// the f64 instruction interleaves rows (4j + l/16) -- measured on gfx950 --
+// and the f32 one keeps them blocked (4(l/16) + j).
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 template <typename T> struct mf;
@@ -52,11 +54,13 @@ template <> struct mf<double> {
   typedef d4 acc;
   __device__ static __forceinline__ acc op(double a, double b, acc c)
   { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static __forceinline__ int row(int j, int lq) { return 4 * j + lq; }
 };
 template <> struct mf<float> {
   typedef f4 acc;
   __device__ static __forceinline__ acc op(float a, float b, acc c)
   { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static __forceinline__ int row(int j, int lq) { return 4 * lq + j; }
 };
 
 // ---------------------------------------------------------------------------
@@ -87,7 +91,7 @@ __global__ __launch_bounds__(256) void k_gemm_sub(int m, int n, int k, const cpl
       const int col = col0 + wc * 32 + cb * 16 + l16;
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        const int row = row0 + wr * 32 + rb * 16 + 4 * lq + j;
+        const int row = row0 + wr * 32 + rb * 16 + mf<T>::row(j, lq);
         cplx<T> v = {0, 0};
         if (row < m && col < n) v = C[(size_t)col * ldc + row];
         cr[rb][cb][j] = v.re; ci[rb][cb][j] = v.im;
@@ -144,10 +148,33 @@ __global__ __launch_bounds__(256) void k_gemm_sub(int m, int n, int k, const cpl
       const int col = col0 + wc * 32 + cb * 16 + l16;
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        const int row = row0 + wr * 32 + rb * 16 + 4 * lq + j;
+        const int row = row0 + wr * 32 + rb * 16 + mf<T>::row(j, lq);
         if (row < m && col < n) C[(size_t)col * ldc + row] = {cr[rb][cb][j], ci[rb][cb][j]};
       }
     }
+}
+
+// Optional instrumentation: HIP events around every GEMM launch, so a caller
+// can read the kernel's total time and algorithmic flop count.
+struct GemmProfile {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> rec[2];
+  double flops[2] = {0, 0};
+  size_t used = 0;
+};
+static GemmProfile g_prof;
+
+static void prof_begin(hipStream_t st, hipEvent_t &a)
+{
+  if (g_prof.used + 2 > g_prof.pool.size()) {
+    for (int i = 0; i < 256; i++) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) == hipSuccess) g_prof.pool.push_back(e);
+    }
+  }
+  a = g_prof.pool[g_prof.used++];
+  (void)hipEventRecord(a, st);
 }
 
 template <typename T>
@@ -156,7 +183,15 @@ static int gemm_sub(int m, int n, int k, const cplx<T> *A, int64_t lda, const cp
 {
   if (m <= 0 || n <= 0 || k <= 0) return BFA_OK;
   dim3 grid((m + GBM - 1) / GBM, (n + GBN - 1) / GBN);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const int p = sizeof(T) == 8 ? 0 : 1;
+  if (g_prof.on) prof_begin(st, e0);
   hipLaunchKernelGGL(k_gemm_sub<T>, grid, dim3(256), 0, st, m, n, k, A, lda, B, ldb, C, ldc);
+  if (g_prof.on) {
+    prof_begin(st, e1);
+    g_prof.rec[p].push_back({e0, e1});
+    g_prof.flops[p] += 8.0 * m * (double)n * k;
+  }
   return BFA_OK;
 }
 
@@ -589,5 +624,40 @@ extern "C" int bfa_zgemm_sub(int m, int n, int k, const void *d_A, int64_t lda, 
     return fail(BFA_ERR_ARG, "bfa_zgemm_sub: bad sizes");
   bfa::gemm_sub_z(m, n, k, d_A, lda, d_B, ldb, d_C, ldc, (hipStream_t)stream);
   HIPCHECK(hipGetLastError());
+  return BFA_OK;
+}
+
+extern "C" int bfa_cgemm_sub(int m, int n, int k, const void *d_A, int64_t lda, const void *d_B, int64_t ldb,
+                             void *d_C, int64_t ldc, void *stream)
+{
+  if (m < 0 || n < 0 || k < 0 || (m > 0 && (lda < m || ldc < m)) || (k > 0 && ldb < k))
+    return fail(BFA_ERR_ARG, "bfa_cgemm_sub: bad sizes");
+  bfa::gemm_sub_c(m, n, k, d_A, lda, d_B, ldb, d_C, ldc, (hipStream_t)stream);
+  HIPCHECK(hipGetLastError());
+  return BFA_OK;
+}
+
+extern "C" int bfa_profile_enable(int on)
+{
+  g_prof.on = on != 0;
+  return BFA_OK;
+}
+
+extern "C" int bfa_profile_read(int which, double *ms, double *flops, int *launches)
+{
+  if (which < 0 || which > 1) return fail(BFA_ERR_ARG, "bfa_profile_read: which must be 0 (z) or 1 (c)");
+  double t = 0.0;
+  for (auto &pr : g_prof.rec[which]) {
+    HIPCHECK(hipEventSynchronize(pr.second));
+    float x = 0.0f;
+    HIPCHECK(hipEventElapsedTime(&x, pr.first, pr.second));
+    t += x;
+  }
+  if (ms) *ms = t;
+  if (flops) *flops = g_prof.flops[which];
+  if (launches) *launches = (int)g_prof.rec[which].size();
+  g_prof.rec[which].clear();
+  g_prof.flops[which] = 0.0;
+  if (g_prof.rec[0].empty() && g_prof.rec[1].empty()) g_prof.used = 0;
   return BFA_OK;
 }

@@ -119,11 +119,22 @@ int  bfa_lu_solve_z(const void *d_A, int n, int64_t lda, const int *d_ipiv,
  * the MFMA kernel of the LU trailing update (HMatrix::Multiply-style use).  */
 int  bfa_zgemm_sub(int m, int n, int k, const void *d_A, int64_t lda, const void *d_B,
                    int64_t ldb, void *d_C, int64_t ldc, void *stream);
+/* Same for complex64 (the FP32 MFMA kernel of the precision form).        */
+int  bfa_cgemm_sub(int m, int n, int k, const void *d_A, int64_t lda, const void *d_B,
+                   int64_t ldb, void *d_C, int64_t ldc, void *stream);
 
 /* ---- power (GetPFTMatrix with EMT, GetPFT.cc:133-189, EMTPFT.cc:324-546) */
 /* out[2*j] = P_abs, out[2*j+1] = P_scat for RHS column j (host array).    */
 int  bfa_emt_power(bfa_geometry *g, double omega, const bfa_planewave *pw,
                    int npw, const void *d_J, int64_t ldj, double *out, void *stream);
+
+/* ---- instrumentation (no reference counterpart) ------------------------ */
+/* When enabled, every trailing-update GEMM launch is bracketed by HIP
+ * events; bfa_profile_read returns the summed kernel time (ms), the summed
+ * algorithmic flops (8mnk per complex product) and the launch count for
+ * which = 0 (complex128 kernel) or 1 (complex64 kernel), then resets.      */
+int  bfa_profile_enable(int on);
+int  bfa_profile_read(int which, double *ms, double *flops, int *launches);
 
 /* ---- device memory helpers (for hosts without a tensor library) -------- */
 int  bfa_device_malloc(void **ptr, size_t bytes);

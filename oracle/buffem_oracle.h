@@ -87,6 +87,8 @@ void orc_static_table(const orc_geom *g, const int *rowptr, const int *cols,
 void orc_assemble(const orc_geom *g, double omr, double omi, const orc_material *mat,
                   const int *rowptr, const int *cols, const double *table,
                   double *M);
+void orc_assemble_rows(const orc_geom *g, double omr, double omi, int nrows, const int *rows,
+                       const int *rowptr, const int *cols, const double *table, double *out);
 /* only the Gram / chi^-1 entries of row a (VIEMatrix.cc:200-316)         */
 int  orc_overlaps(const orc_geom *g, int a, double omega, const orc_material *mat,
                   int idx[7], double val[7][2]);

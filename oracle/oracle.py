@@ -56,6 +56,7 @@ def lib():
             "orc_touch_pairs": (I, [P, P, P]),
             "orc_static_table": (None, [P, P, P, I, P]),
             "orc_assemble": (None, [P, D, D, P, P, P, P, P]),
+            "orc_assemble_rows": (None, [P, D, D, I, P, P, P, P, P]),
             "orc_overlaps": (I, [P, I, D, P, P, P]),
             "orc_rhs": (None, [P, D, P, I, P]),
             "orc_emt_power": (None, [P, D, P, I, P, P]),
@@ -207,6 +208,17 @@ class Geometry:
         lib().orc_assemble(self.h, omega.real, omega.imag, ctypes.byref(mat), rowptr.ctypes.data,
                            cols.ctypes.data, table.ctypes.data, Mcol.ctypes.data)
         return Mcol.T.copy()
+
+    def assemble_rows(self, omega, rows, table=None):
+        """Upper-triangle entries (b >= a) of the given rows, zero static records if table is None."""
+        rowptr, cols = self.touch_pairs()
+        rows = np.ascontiguousarray(rows, np.int32)
+        out = np.zeros((len(rows), self.N), dtype=np.complex128)
+        omega = complex(omega)
+        tp = None if table is None else np.ascontiguousarray(table, float)
+        lib().orc_assemble_rows(self.h, omega.real, omega.imag, len(rows), rows.ctypes.data, rowptr.ctypes.data,
+                                cols.ctypes.data, None if tp is None else tp.ctypes.data, out.ctypes.data)
+        return out
 
     def rhs(self, omega, pws):
         N = self.N

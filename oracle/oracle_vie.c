@@ -557,6 +557,35 @@ void orc_assemble(const orc_geom *g, double omr, double omi, const orc_material 
     for (int r = c + 1; r < N; r++) Z[(size_t)c * N + r] = Z[(size_t)r * N + c];
 }
 
+/* Warm assembly of the upper-triangle part (b >= a) of selected rows, as the
+ * row loop of AssembleGBlock (VIEMatrix.cc:401-416) does; used to time a
+ * bounded sample.  table may be NULL (static records taken as zero: the
+ * warm cost does not depend on their values).  out: [nrows][N] complex,
+ * entries b < a left untouched.                                          */
+void orc_assemble_rows(const orc_geom *g, double omr, double omi, int nrows, const int *rows,
+                       const int *rowptr, const int *cols, const double *table, double *out)
+{
+  int N = g->nf_int;
+  cplx *Z = (cplx *)out;
+  const double zero6[6] = {0, 0, 0, 0, 0, 0};
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int i = 0; i < nrows; i++) {
+    int a = rows[i];
+    int j = rowptr[a];
+    for (int b = a; b < N; b++) {
+      const double *fib = NULL;
+      if (j < rowptr[a + 1] && cols[j] == b) { fib = table ? table + 6 * (size_t)j : zero6; j++; }
+      double v[2];
+      if (fib) orc_gme(g, a, b, omr, omi, fib, v);
+      else {
+        cplx z = bfbf_gme(g, a, b, K_HELM, omr + I * omi);
+        v[0] = creal(z); v[1] = cimag(z);
+      }
+      Z[(size_t)i * N + b] = v[0] + I * v[1];
+    }
+  }
+}
+
 /* ---------------- incident field, RHS, EMT ------------------------------ */
 static void pw_field(const orc_planewave *pw, double k, const double *x, cplx E[3])
 {

@@ -77,6 +77,25 @@ def test_zgemm(bfa, torch_cuda):
         assert err < 1e-13
 
 
+def test_cgemm(bfa, torch_cuda):
+    torch = torch_cuda
+    rng = np.random.default_rng(3)
+    for (m, n, k) in [(16, 16, 4), (70, 33, 21), (200, 130, 256)]:
+        print(f"[cgemm] {m}x{n}x{k}", flush=True)
+        A = (rng.standard_normal((m, k)) + 1j * rng.standard_normal((m, k))).astype(np.complex64)
+        B = (rng.standard_normal((k, n)) + 1j * rng.standard_normal((k, n))).astype(np.complex64)
+        C = (rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))).astype(np.complex64)
+        dA = torch.from_numpy(np.ascontiguousarray(A.T)).cuda()
+        dB = torch.from_numpy(np.ascontiguousarray(B.T)).cuda()
+        dC = torch.from_numpy(np.ascontiguousarray(C.T)).cuda()
+        bfa.cgemm_sub(m, n, k, dA, m, dB, k, dC, m)
+        torch.cuda.synchronize()
+        ref = C.astype(np.complex128) - A.astype(np.complex128) @ B.astype(np.complex128)
+        err = np.abs(colmajor(dC) - ref).max() / np.abs(ref).max()
+        print("cgemm rel err", err)
+        assert err < 1e-5
+
+
 def test_lu_random(bfa, torch_cuda):
     torch = torch_cuda
     rng = np.random.default_rng(1)
