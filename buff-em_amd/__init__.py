@@ -68,6 +68,10 @@ _SIGS = {
     "bfa_emt_power": (I, [P, D, P, I, P, I64, P, P]),
     "bfa_zgemm_sub": (I, [I, I, I, P, I64, P, I64, P, I64, P]),
     "bfa_cgemm_sub": (I, [I, I, I, P, I64, P, I64, P, I64, P]),
+    "bfa_convert_z2c": (I, [P, I, I, I64, P, I64, P]),
+    "bfa_lu_factor_c": (I, [P, I, I64, P, P, P]),
+    "bfa_lu_solve_c": (I, [P, I, I64, P, P, I, I64, P]),
+    "bfa_solve_refine": (I, [P, I64, P, I64, P, I, P, I, I64, D, I, P, P, P]),
     "bfa_profile_enable": (I, [I]),
     "bfa_profile_read": (I, [I, P, P, P]),
     "bfa_device_malloc": (I, [P, ctypes.c_size_t]),
@@ -296,3 +300,30 @@ def profile_read(which=0):
     _check(lib().bfa_profile_read(int(which), ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(nl)),
            "bfa_profile_read")
     return ms.value, fl.value, nl.value
+
+
+def convert_z2c(d_A64, m, n, d_A32, lda=None, lda32=None, stream=None):
+    _check(lib().bfa_convert_z2c(_ptr(d_A64), int(m), int(n), int(lda or m), _ptr(d_A32), int(lda32 or m),
+                                 _stream_ptr(stream)), "bfa_convert_z2c")
+
+
+def lu_factor_c(d_A, n, d_ipiv, lda=None, stream=None):
+    info = I()
+    _check(lib().bfa_lu_factor_c(_ptr(d_A), int(n), int(lda or n), _ptr(d_ipiv), ctypes.byref(info),
+                                 _stream_ptr(stream)), "bfa_lu_factor_c")
+    return info.value
+
+
+def lu_solve_c(d_A, n, d_ipiv, d_B, nrhs, lda=None, ldb=None, stream=None):
+    _check(lib().bfa_lu_solve_c(_ptr(d_A), int(n), int(lda or n), _ptr(d_ipiv), _ptr(d_B), int(nrhs),
+                                int(ldb or n), _stream_ptr(stream)), "bfa_lu_solve_c")
+
+
+def solve_refine(d_M64, d_LU32, d_ipiv, n, d_B, nrhs, tol=1e-10, maxit=10, ldm=None, ldlu=None, ldb=None,
+                 stream=None):
+    """Mixed-precision solve; returns (iterations, final relative residual)."""
+    it, res = I(), D()
+    _check(lib().bfa_solve_refine(_ptr(d_M64), int(ldm or n), _ptr(d_LU32), int(ldlu or n), _ptr(d_ipiv), int(n),
+                                  _ptr(d_B), int(nrhs), int(ldb or n), float(tol), int(maxit), ctypes.byref(it),
+                                  ctypes.byref(res), _stream_ptr(stream)), "bfa_solve_refine")
+    return it.value, res.value

@@ -14,6 +14,7 @@
 // product, v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).
 #include <hip/hip_runtime.h>
 #include <climits>
+#include <cstdlib>
 #include <cmath>
 #include <vector>
 
@@ -154,6 +155,110 @@ __global__ __launch_bounds__(256) void k_gemm_sub(int m, int n, int k, const cpl
     }
 }
 
+// Large-tile variant: workgroup tile 128 x 128 complex, 8 waves (2 x 4) of
+// 64 x 32 each, K staged 16 at a time through double-buffered LDS with the
+// next stage prefetched into registers; workgroups are renumbered so that
+// each XCD (dispatch round-robin over 8) sweeps a contiguous band of tiles.
+#define G2M 128
+#define G2N 128
+#define G2K 16
+template <typename T>
+__global__ __launch_bounds__(512) void k_gemm_sub2(int m, int n, int k, const cplx<T> *__restrict__ A, int64_t lda,
+                                                   const cplx<T> *__restrict__ B, int64_t ldb, cplx<T> *C, int64_t ldc,
+                                                   int tiles_m, int tiles_n)
+{
+  typedef typename mf<T>::acc acc_t;
+  __shared__ T As[2][2][G2K][G2M];
+  __shared__ T Bs[2][2][G2K][G2N];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 2, wc = wave & 3;
+  const int l16 = lane & 15, lq = lane >> 4;
+  // XCD-aware renumbering of the linear workgroup id (bijective when the
+  // tile count is a multiple of the 8 XCDs, identity otherwise)
+  const int ntile = tiles_m * tiles_n;
+  const int bid = blockIdx.x;
+  const int t = (ntile % 8 == 0) ? (bid % 8) * (ntile / 8) + bid / 8 : bid;
+  const int tm = t % tiles_m, tn = t / tiles_m;
+  const int row0 = tm * G2M, col0 = tn * G2N;
+  acc_t cr[4][2], ci[4][2];
+#pragma unroll
+  for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++) {
+      const int col = col0 + wc * 32 + cb * 16 + l16;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int row = row0 + wr * 64 + rb * 16 + mf<T>::row(j, lq);
+        cplx<T> v = {0, 0};
+        if (row < m && col < n) v = C[(size_t)col * ldc + row];
+        cr[rb][cb][j] = v.re; ci[rb][cb][j] = v.im;
+      }
+    }
+  cplx<T> pa[4], pb[4];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int r = tid & 127, kk = (tid >> 7) + 4 * i;
+      pa[i] = (row0 + r < m && k0 + kk < k) ? A[(size_t)(k0 + kk) * lda + row0 + r] : cplx<T>{0, 0};
+      const int kb = tid & 15, c = (tid >> 4) + 32 * i;
+      pb[i] = (col0 + c < n && k0 + kb < k) ? B[(size_t)(col0 + c) * ldb + k0 + kb] : cplx<T>{0, 0};
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int r = tid & 127, kk = (tid >> 7) + 4 * i;
+      As[buf][0][kk][r] = pa[i].re; As[buf][1][kk][r] = pa[i].im;
+      const int kb = tid & 15, c = (tid >> 4) + 32 * i;
+      Bs[buf][0][kb][c] = pb[i].re; Bs[buf][1][kb][c] = pb[i].im;
+    }
+  };
+  const int nst = (k + G2K - 1) / G2K;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int st = 0; st < nst; st++) {
+    const int buf = st & 1;
+    if (st + 1 < nst) gload((st + 1) * G2K);
+#pragma unroll
+    for (int kk = 0; kk < G2K; kk += 4) {
+      T ar[4], ai[4], br[2], bi[2];
+#pragma unroll
+      for (int rb = 0; rb < 4; rb++) {
+        ar[rb] = -As[buf][0][kk + lq][wr * 64 + rb * 16 + l16];
+        ai[rb] = As[buf][1][kk + lq][wr * 64 + rb * 16 + l16];
+      }
+#pragma unroll
+      for (int cb = 0; cb < 2; cb++) {
+        br[cb] = Bs[buf][0][kk + lq][wc * 32 + cb * 16 + l16];
+        bi[cb] = Bs[buf][1][kk + lq][wc * 32 + cb * 16 + l16];
+      }
+#pragma unroll
+      for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++) {
+          cr[rb][cb] = mf<T>::op(ar[rb], br[cb], cr[rb][cb]);
+          cr[rb][cb] = mf<T>::op(ai[rb], bi[cb], cr[rb][cb]);
+          ci[rb][cb] = mf<T>::op(ar[rb], bi[cb], ci[rb][cb]);
+          ci[rb][cb] = mf<T>::op(-ai[rb], br[cb], ci[rb][cb]);
+        }
+    }
+    if (st + 1 < nst) sstore(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+    for (int cb = 0; cb < 2; cb++) {
+      const int col = col0 + wc * 32 + cb * 16 + l16;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int row = row0 + wr * 64 + rb * 16 + mf<T>::row(j, lq);
+        if (row < m && col < n) C[(size_t)col * ldc + row] = {cr[rb][cb][j], ci[rb][cb][j]};
+      }
+    }
+}
+
 // Optional instrumentation: HIP events around every GEMM launch, so a caller
 // can read the kernel's total time and algorithmic flop count.
 struct GemmProfile {
@@ -186,7 +291,12 @@ static int gemm_sub(int m, int n, int k, const cplx<T> *A, int64_t lda, const cp
   hipEvent_t e0 = nullptr, e1 = nullptr;
   const int p = sizeof(T) == 8 ? 0 : 1;
   if (g_prof.on) prof_begin(st, e0);
-  hipLaunchKernelGGL(k_gemm_sub<T>, grid, dim3(256), 0, st, m, n, k, A, lda, B, ldb, C, ldc);
+  if (m >= 256 && n >= 256) {
+    const int tm = (m + G2M - 1) / G2M, tn = (n + G2N - 1) / G2N;
+    hipLaunchKernelGGL(k_gemm_sub2<T>, dim3(tm * tn), dim3(512), 0, st, m, n, k, A, lda, B, ldb, C, ldc, tm, tn);
+  } else {
+    hipLaunchKernelGGL(k_gemm_sub<T>, grid, dim3(256), 0, st, m, n, k, A, lda, B, ldb, C, ldc);
+  }
   if (g_prof.on) {
     prof_begin(st, e1);
     g_prof.rec[p].push_back({e0, e1});
@@ -196,13 +306,17 @@ static int gemm_sub(int m, int n, int k, const cplx<T> *A, int64_t lda, const cp
 }
 
 // ---------------------------------------------------------------------------
-// B(kb x n) := L^-1 B (L unit lower, kb <= 256) or U^-1 B (U upper, non-unit).
-// One workgroup per 16 columns of B, held in LDS; L/U read through the cache.
-#define TRSM_COLS 16
+// Diagonal-block solve, kb <= TRSM_KB: B(kb x n) := L^-1 B (L unit lower) or
+// U^-1 B (U upper, non-unit).  One workgroup per TRSM_COLS columns of B held
+// in LDS.  Larger triangles are blocked (trsm() below): the off-diagonal
+// updates run on the MFMA GEMM.
+#define TRSM_KB 32
+#define TRSM_COLS 64
 template <typename T, bool LOWER>
 __global__ __launch_bounds__(256) void k_trsm(int kb, int n, const cplx<T> *L, int64_t ldl, cplx<T> *B, int64_t ldb)
 {
-  __shared__ cplx<T> X[256][TRSM_COLS];
+  __shared__ cplx<T> X[TRSM_KB][TRSM_COLS];
+  __shared__ cplx<T> Ls[TRSM_KB][TRSM_KB];
   const int tid = threadIdx.x;
   const int c0 = blockIdx.x * TRSM_COLS;
   const int nc = min(TRSM_COLS, n - c0);
@@ -210,20 +324,24 @@ __global__ __launch_bounds__(256) void k_trsm(int kb, int n, const cplx<T> *L, i
     const int r = idx % kb, c = idx / kb;
     X[r][c] = c < nc ? B[(size_t)(c0 + c) * ldb + r] : cplx<T>{0, 0};
   }
+  for (int idx = tid; idx < kb * kb; idx += blockDim.x) {
+    const int r = idx % kb, c = idx / kb;
+    Ls[r][c] = L[(size_t)c * ldl + r];
+  }
   __syncthreads();
   const int c = tid % TRSM_COLS, rg = tid / TRSM_COLS, nrg = blockDim.x / TRSM_COLS;
   if (LOWER) {
     for (int i = 0; i < kb - 1; i++) {
       const cplx<T> xi = X[i][c];
-      for (int r = i + 1 + rg; r < kb; r += nrg) X[r][c] = c_sub(X[r][c], c_mul(L[(size_t)i * ldl + r], xi));
+      for (int r = i + 1 + rg; r < kb; r += nrg) X[r][c] = c_sub(X[r][c], c_mul(Ls[r][i], xi));
       __syncthreads();
     }
   } else {
     for (int i = kb - 1; i >= 0; i--) {
-      if (rg == 0) X[i][c] = c_mul(X[i][c], c_inv(L[(size_t)i * ldl + i]));
+      if (rg == 0) X[i][c] = c_mul(X[i][c], c_inv(Ls[i][i]));
       __syncthreads();
       const cplx<T> xi = X[i][c];
-      for (int r = rg; r < i; r += nrg) X[r][c] = c_sub(X[r][c], c_mul(L[(size_t)i * ldl + r], xi));
+      for (int r = rg; r < i; r += nrg) X[r][c] = c_sub(X[r][c], c_mul(Ls[r][i], xi));
       __syncthreads();
     }
   }
@@ -234,10 +352,36 @@ __global__ __launch_bounds__(256) void k_trsm(int kb, int n, const cplx<T> *L, i
 }
 
 template <typename T, bool LOWER>
+static void trsm_diag(int kb, int n, const cplx<T> *L, int64_t ldl, cplx<T> *B, int64_t ldb, hipStream_t st)
+{
+  if (kb <= 0 || n <= 0) return;
+  hipLaunchKernelGGL((k_trsm<T, LOWER>), dim3((n + TRSM_COLS - 1) / TRSM_COLS), dim3(256), 0, st, kb, n, L, ldl, B,
+                     ldb);
+}
+
+template <typename T> static int gemm_sub(int m, int n, int k, const cplx<T> *A, int64_t lda, const cplx<T> *B,
+                                          int64_t ldb, cplx<T> *C, int64_t ldc, hipStream_t st);
+
+// B(kb x n) := L^-1 B or U^-1 B for any kb, blocked by TRSM_KB
+template <typename T, bool LOWER>
 static void trsm(int kb, int n, const cplx<T> *L, int64_t ldl, cplx<T> *B, int64_t ldb, hipStream_t st)
 {
   if (kb <= 0 || n <= 0) return;
-  hipLaunchKernelGGL((k_trsm<T, LOWER>), dim3((n + TRSM_COLS - 1) / TRSM_COLS), dim3(256), 0, st, kb, n, L, ldl, B, ldb);
+  if (LOWER) {
+    for (int i0 = 0; i0 < kb; i0 += TRSM_KB) {
+      const int ib = std::min(TRSM_KB, kb - i0);
+      trsm_diag<T, true>(ib, n, L + (size_t)i0 * ldl + i0, ldl, B + i0, ldb, st);
+      if (i0 + ib < kb)
+        gemm_sub<T>(kb - i0 - ib, n, ib, L + (size_t)i0 * ldl + i0 + ib, ldl, B + i0, ldb, B + i0 + ib, ldb, st);
+    }
+  } else {
+    const int nb = (kb + TRSM_KB - 1) / TRSM_KB;
+    for (int bi = nb - 1; bi >= 0; bi--) {
+      const int i0 = bi * TRSM_KB, ib = std::min(TRSM_KB, kb - i0);
+      trsm_diag<T, false>(ib, n, L + (size_t)i0 * ldl + i0, ldl, B + i0, ldb, st);
+      if (i0 > 0) gemm_sub<T>(i0, n, ib, L + (size_t)i0 * ldl, ldl, B + i0, ldb, B, ldb, st);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -283,20 +427,22 @@ struct PanelWork {
   int *flags;       // [0] spin timeout, [1] first zero pivot (1-based, 0 none)
 };
 
+// Grid barrier: every thread's stores are drained by the workgroup barrier,
+// one thread per workgroup publishes them device-wide (release fence) and
+// arrives; it then polls with relaxed agent-scope loads and acquires.
 __device__ __forceinline__ void grid_barrier(unsigned *bar, unsigned target, int *flags)
 {
-  __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) {
-    atomicAdd(bar, 1u);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     unsigned spins = 0;
-    while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       if (++spins > SPIN_LIMIT) { atomicOr(&flags[0], 1); break; }
-      __builtin_amdgcn_s_sleep(1);
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
-  __threadfence();
 }
 
 template <typename T>
@@ -304,10 +450,11 @@ __global__ __launch_bounds__(256) void k_panel(cplx<T> *A, int m, int w, int64_t
                                                int piv_off)
 {
   extern __shared__ unsigned char smem_raw[];
-  cplx<T> *rows = (cplx<T> *)smem_raw;                 // [rpw][w]
+  cplx<T> *rows = (cplx<T> *)smem_raw;                 // column-major [w][rpw]
+#define RW(r, c) rows[(size_t)(c) * rpw + (r)]
   __shared__ cplx<T> prow[PW], rj[PW];
   __shared__ T sval[256];
-  __shared__ int srow[256];
+  __shared__ int srow[256], swin[256];
   __shared__ int piv_s, win_s;
   __shared__ T pval_s;
   const int tid = threadIdx.x, g = blockIdx.x, nwg = gridDim.x;
@@ -316,7 +463,7 @@ __global__ __launch_bounds__(256) void k_panel(cplx<T> *A, int m, int w, int64_t
   cplx<T> *rowjg = (cplx<T> *)pw.rowj;
   for (int idx = tid; idx < nr * w; idx += blockDim.x) {
     const int r = idx % nr, c = idx / nr;
-    rows[r * w + c] = A[(size_t)c * lda + r0 + r];
+    RW(r, c) = A[(size_t)c * lda + r0 + r];
   }
   __syncthreads();
   for (int j = 0; j < w; j++) {
@@ -327,7 +474,7 @@ __global__ __launch_bounds__(256) void k_panel(cplx<T> *A, int m, int w, int64_t
     for (int r = tid; r < nr; r += blockDim.x) {
       const int gr = r0 + r;
       if (gr < j) continue;
-      const T v = c_abs1(rows[r * w + j]);
+      const T v = c_abs1(RW(r, j));
       if (v > best || (v == best && gr < brow)) { best = v; brow = gr; }
     }
     sval[tid] = best; srow[tid] = brow;
@@ -342,20 +489,33 @@ __global__ __launch_bounds__(256) void k_panel(cplx<T> *A, int m, int w, int64_t
     }
     const int lrow = srow[0];
     if (tid == 0) { pw.val[buf * nwg + g] = (double)sval[0]; pw.row[buf * nwg + g] = lrow; }
-    if (lrow != INT_MAX && tid < w) rdata[((size_t)buf * nwg + g) * PW + tid] = rows[(lrow - r0) * w + tid];
-    if (j >= r0 && j < r1 && tid < w) rowjg[buf * PW + tid] = rows[(j - r0) * w + tid];
+    if (lrow != INT_MAX && tid < w) rdata[((size_t)buf * nwg + g) * PW + tid] = RW(lrow - r0, tid);
+    if (j >= r0 && j < r1 && tid < w) rowjg[buf * PW + tid] = RW(j - r0, tid);
     grid_barrier(pw.bar, (unsigned)((j + 1) * nwg), pw.flags);
-    // global pivot (same result in every workgroup)
-    if (tid == 0) {
-      double bv = -1.0;
-      int bw = -1, br = INT_MAX;
-      for (int q = 0; q < nwg; q++) {
-        const double v = __hip_atomic_load(&pw.val[buf * nwg + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const int rr = __hip_atomic_load(&pw.row[buf * nwg + q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // global pivot (same result in every workgroup): parallel scan of the
+    // workgroup slots, first index of the maximum
+    {
+      T bv = -1;
+      int br = INT_MAX, bw = -1;
+      for (int q = tid; q < nwg; q += blockDim.x) {
+        const int rr = pw.row[buf * nwg + q];
         if (rr == INT_MAX) continue;
-        if (v > bv || (v == bv && rr < br)) { bv = v; bw = q; br = rr; }
+        const T v = (T)pw.val[buf * nwg + q];
+        if (v > bv || (v == bv && rr < br)) { bv = v; br = rr; bw = q; }
       }
-      piv_s = br; win_s = bw; pval_s = (T)bv;
+      sval[tid] = bv; srow[tid] = br; swin[tid] = bw;
+      __syncthreads();
+      for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if (tid < off) {
+          const T v = sval[tid + off];
+          const int rr = srow[tid + off];
+          if (v > sval[tid] || (v == sval[tid] && rr < srow[tid])) {
+            sval[tid] = v; srow[tid] = rr; swin[tid] = swin[tid + off];
+          }
+        }
+        __syncthreads();
+      }
+      if (tid == 0) { piv_s = srow[0]; win_s = swin[0]; pval_s = sval[0]; }
     }
     __syncthreads();
     const int p = piv_s, win = win_s;
@@ -370,8 +530,8 @@ __global__ __launch_bounds__(256) void k_panel(cplx<T> *A, int m, int w, int64_t
     }
     // interchange rows j and p in LDS
     if (p != j) {
-      if (j >= r0 && j < r1 && tid < w) rows[(j - r0) * w + tid] = prow[tid];
-      if (p >= r0 && p < r1 && tid < w) rows[(p - r0) * w + tid] = rj[tid];
+      if (j >= r0 && j < r1 && tid < w) RW(j - r0, tid) = prow[tid];
+      if (p >= r0 && p < r1 && tid < w) RW(p - r0, tid) = rj[tid];
     }
     __syncthreads();
     // scale column j below the diagonal and update the rest of the panel
@@ -380,17 +540,18 @@ __global__ __launch_bounds__(256) void k_panel(cplx<T> *A, int m, int w, int64_t
       for (int r = tid; r < nr; r += blockDim.x) {
         const int gr = r0 + r;
         if (gr <= j) continue;
-        const cplx<T> l = c_mul(rows[r * w + j], inv);
-        rows[r * w + j] = l;
-        for (int c = j + 1; c < w; c++) rows[r * w + c] = c_sub(rows[r * w + c], c_mul(l, prow[c]));
+        const cplx<T> l = c_mul(RW(r, j), inv);
+        RW(r, j) = l;
+        for (int c = j + 1; c < w; c++) RW(r, c) = c_sub(RW(r, c), c_mul(l, prow[c]));
       }
     }
     __syncthreads();
   }
   for (int idx = tid; idx < nr * w; idx += blockDim.x) {
     const int r = idx % nr, c = idx / nr;
-    A[(size_t)c * lda + r0 + r] = rows[r * w + c];
+    A[(size_t)c * lda + r0 + r] = RW(r, c);
   }
+#undef RW
 }
 
 __global__ void k_zero_u32(unsigned *p, int n)
@@ -445,16 +606,27 @@ static int lu_work(LUWork *&w)
 template <typename T>
 static int panel_base(cplx<T> *A, int m, int w, int64_t lda, int *ipiv, int piv_off, LUWork *lw, hipStream_t st)
 {
-  // workgroups: one per CU, doubled until each workgroup's rows fit 64 KB of LDS
-  const size_t lds_cap = 64 * 1024;
-  int nwg = std::min(lw->num_cu, std::max(1, (m + 15) / 16));
-  int rpw = (m + nwg - 1) / nwg;
-  while ((size_t)rpw * w * sizeof(cplx<T>) > lds_cap && nwg < 2 * lw->num_cu) {
-    nwg = std::min(2 * nwg, 2 * lw->num_cu);
-    rpw = (m + nwg - 1) / nwg;
+  // As few workgroups as the LDS allows (fewer arrivals per grid barrier):
+  // up to ~150 KB of rows per workgroup, at most one workgroup per CU.
+  static int attr_done[2] = {0, 0};
+  const int ti = sizeof(T) == 8 ? 0 : 1;
+  size_t lds_cap = 150 * 1024;
+  if (!attr_done[ti]) {
+    attr_done[ti] = hipFuncSetAttribute((const void *)k_panel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)lds_cap) == hipSuccess ? 1 : -1;
   }
+  if (attr_done[ti] < 0) lds_cap = 64 * 1024;
+  const int row_bytes = w * (int)sizeof(cplx<T>);
+  int rpw = std::max(16, (int)(lds_cap / row_bytes));
+  int nwg = (m + rpw - 1) / rpw;
+  if (nwg > lw->num_cu) { nwg = lw->num_cu; rpw = (m + nwg - 1) / nwg; }
+  if (const char *e = getenv("BFA_PANEL_WG")) {
+    const int want = atoi(e);
+    if (want > 0 && want > nwg && want <= lw->num_cu) { nwg = want; rpw = std::max(16, (m + nwg - 1) / nwg); }
+  }
+  rpw = (m + nwg - 1) / nwg;
   nwg = (m + rpw - 1) / rpw;
-  if ((size_t)rpw * w * sizeof(cplx<T>) > lds_cap || nwg > lw->maxwg)
+  if ((size_t)rpw * row_bytes > lds_cap || nwg > lw->maxwg)
     return fail(BFA_ERR_ARG, "panel of %d rows too tall for LDS-resident factorisation", m);
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, lw->pw.bar, 16);
   size_t lds = (size_t)rpw * w * sizeof(cplx<T>);
@@ -528,11 +700,100 @@ __global__ void k_gather_rows(const cplx<T> *B, int64_t ldb, cplx<T> *X, int64_t
   if (i < n && j < nrhs) X[(size_t)j * ldx + i] = B[(size_t)j * ldb + perm[i]];
 }
 
+// Diagonal block (kb <= SV_KB rows) of a triangular solve with up to SV_RHS
+// right-hand sides: block and RHS slice held in LDS, one workgroup.
+#define SV_KB 64
+#define SV_RHS 8
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void k_trsv_diag(int kb, int nrhs, const cplx<T> *L, int64_t ldl, cplx<T> *X,
+                                                   int64_t ldx)
+{
+  __shared__ cplx<T> Ls[SV_KB][SV_KB + 1];
+  __shared__ cplx<T> Xs[SV_RHS][SV_KB];
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < kb * kb; idx += blockDim.x) {
+    const int r = idx % kb, c = idx / kb;
+    Ls[r][c] = L[(size_t)c * ldl + r];
+  }
+  for (int idx = tid; idx < kb * nrhs; idx += blockDim.x) {
+    const int r = idx % kb, j = idx / kb;
+    Xs[j][r] = X[(size_t)j * ldx + r];
+  }
+  __syncthreads();
+  const int r = tid % SV_KB, jg = tid / SV_KB;     // 4 RHS groups of 64 rows
+  if (LOWER) {
+    for (int i = 0; i < kb - 1; i++) {
+      if (r > i && r < kb)
+        for (int j = jg; j < nrhs; j += 4) Xs[j][r] = c_sub(Xs[j][r], c_mul(Ls[r][i], Xs[j][i]));
+      __syncthreads();
+    }
+  } else {
+    for (int i = kb - 1; i >= 0; i--) {
+      if (r == i)
+        for (int j = jg; j < nrhs; j += 4) Xs[j][i] = c_mul(Xs[j][i], c_inv(Ls[i][i]));
+      __syncthreads();
+      if (r < i)
+        for (int j = jg; j < nrhs; j += 4) Xs[j][r] = c_sub(Xs[j][r], c_mul(Ls[r][i], Xs[j][i]));
+      __syncthreads();
+    }
+  }
+  for (int idx = tid; idx < kb * nrhs; idx += blockDim.x) {
+    const int rr = idx % kb, j = idx / kb;
+    X[(size_t)j * ldx + rr] = Xs[j][rr];
+  }
+}
+
+// Y(m x nrhs) -= L(m x kb) * Xk(kb x nrhs), kb <= SV_KB: a workgroup owns 64
+// rows; its 4 waves take interleaved quarters of the columns (coalesced
+// column walks of L, Xk broadcast from LDS) and are reduced through LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsv_update(int m, int kb, int nrhs, const cplx<T> *__restrict__ L,
+                                                     int64_t ldl, const cplx<T> *__restrict__ Xk, int64_t ldx,
+                                                     cplx<T> *Y, int64_t ldy)
+{
+  __shared__ cplx<T> xs[SV_RHS][SV_KB];
+  __shared__ cplx<T> part[4][SV_RHS][64];
+  for (int idx = threadIdx.x; idx < kb * nrhs; idx += blockDim.x) {
+    const int c = idx % kb, j = idx / kb;
+    xs[j][c] = Xk[(size_t)j * ldx + c];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r = blockIdx.x * 64 + lane;
+  T ar[SV_RHS], ai[SV_RHS];
+#pragma unroll
+  for (int j = 0; j < SV_RHS; j++) { ar[j] = 0; ai[j] = 0; }
+  if (r < m) {
+#pragma unroll 4
+    for (int c = wv; c < kb; c += 4) {
+      const cplx<T> l = L[(size_t)c * ldl + r];
+#pragma unroll
+      for (int j = 0; j < SV_RHS; j++) {
+        if (j < nrhs) {
+          const cplx<T> x = xs[j][c];
+          ar[j] += l.re * x.re - l.im * x.im;
+          ai[j] += l.re * x.im + l.im * x.re;
+        }
+      }
+    }
+  }
+  for (int j = 0; j < nrhs; j++) part[wv][j][lane] = {ar[j], ai[j]};
+  __syncthreads();
+  if (wv == 0 && r < m) {
+    for (int j = 0; j < nrhs; j++) {
+      const T sr = part[0][j][lane].re + part[1][j][lane].re + part[2][j][lane].re + part[3][j][lane].re;
+      const T si = part[0][j][lane].im + part[1][j][lane].im + part[2][j][lane].im + part[3][j][lane].im;
+      cplx<T> &y = Y[(size_t)j * ldy + r];
+      y.re -= sr;
+      y.im -= si;
+    }
+  }
+}
+
 template <typename T>
 int lu_solve(const cplx<T> *A, int n, int64_t lda, const int *ipiv, cplx<T> *B, int nrhs, int64_t ldb, hipStream_t st,
              void *scratch)
 {
-  const int nb = 256;
   int *perm = (int *)scratch;
   cplx<T> *X = (cplx<T> *)((char *)scratch + (((size_t)n * 4 + 255) / 256) * 256);
   {
@@ -546,22 +807,33 @@ int lu_solve(const cplx<T> *A, int n, int64_t lda, const int *ipiv, cplx<T> *B, 
       if (p != i) std::swap(pm[i], pm[p]);
     }
     HIPCHECK(hipMemcpyAsync(perm, pm.data(), (size_t)n * 4, hipMemcpyHostToDevice, st));
-    HIPCHECK(hipStreamSynchronize(st));
   }
   hipLaunchKernelGGL(k_gather_rows<T>, dim3((n + 255) / 256, nrhs), dim3(256), 0, st, B, ldb, X, (int64_t)n, n, nrhs,
                      (const int *)perm);
-  for (int k = 0; k < n; k += nb) {
-    const int kb = std::min(nb, n - k);
-    trsm<T, true>(kb, nrhs, A + (size_t)k * lda + k, lda, X + k, n, st);
-    if (k + kb < n) gemm_sub<T>(n - k - kb, nrhs, kb, A + (size_t)k * lda + k + kb, lda, X + k, n, X + k + kb, n, st);
-  }
+  const int nb = SV_KB;
   const int nblk = (n + nb - 1) / nb;
-  for (int bi = nblk - 1; bi >= 0; bi--) {
-    const int k = bi * nb, kb = std::min(nb, n - k);
-    trsm<T, false>(kb, nrhs, A + (size_t)k * lda + k, lda, X + k, n, st);
-    if (k > 0) gemm_sub<T>(k, nrhs, kb, A + (size_t)k * lda, lda, X + k, n, X, n, st);
+  for (int j0 = 0; j0 < nrhs; j0 += SV_RHS) {
+    const int nj = std::min(SV_RHS, nrhs - j0);
+    cplx<T> *Xj = X + (size_t)j0 * n;
+    for (int bi = 0; bi < nblk; bi++) {              // L y = P b
+      const int k = bi * nb, kb = std::min(nb, n - k);
+      hipLaunchKernelGGL((k_trsv_diag<T, true>), dim3(1), dim3(256), 0, st, kb, nj, A + (size_t)k * lda + k, lda,
+                         Xj + k, (int64_t)n);
+      const int m = n - k - kb;
+      if (m > 0)
+        hipLaunchKernelGGL(k_trsv_update<T>, dim3((m + 63) / 64), dim3(256), 0, st, m, kb, nj,
+                           A + (size_t)k * lda + k + kb, lda, (const cplx<T> *)(Xj + k), (int64_t)n, Xj + k + kb,
+                           (int64_t)n);
+    }
+    for (int bi = nblk - 1; bi >= 0; bi--) {         // U x = y
+      const int k = bi * nb, kb = std::min(nb, n - k);
+      hipLaunchKernelGGL((k_trsv_diag<T, false>), dim3(1), dim3(256), 0, st, kb, nj, A + (size_t)k * lda + k, lda,
+                         Xj + k, (int64_t)n);
+      if (k > 0)
+        hipLaunchKernelGGL(k_trsv_update<T>, dim3((k + 63) / 64), dim3(256), 0, st, k, kb, nj, A + (size_t)k * lda,
+                           lda, (const cplx<T> *)(Xj + k), (int64_t)n, Xj, (int64_t)n);
+    }
   }
-  // copy back
   HIPCHECK(hipMemcpy2DAsync(B, ldb * sizeof(cplx<T>), X, n * sizeof(cplx<T>), n * sizeof(cplx<T>), nrhs,
                             hipMemcpyDeviceToDevice, st));
   HIPCHECK(hipGetLastError());

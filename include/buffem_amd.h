@@ -115,6 +115,23 @@ int  bfa_lu_factor_z(void *d_A, int n, int64_t lda, int *d_ipiv, int *info,
 int  bfa_lu_solve_z(const void *d_A, int n, int64_t lda, const int *d_ipiv,
                     void *d_B, int nrhs, int64_t ldb, void *stream);
 
+/* ---- precision form (no reference counterpart) -------------------------- */
+/* complex128 -> complex64 copy of an m x n column-major matrix             */
+int  bfa_convert_z2c(const void *d_A64, int m, int n, int64_t lda, void *d_A32,
+                     int64_t lda32, void *stream);
+/* complex64 LU / solve: the same blocked algorithm on FP32 MFMA             */
+int  bfa_lu_factor_c(void *d_A, int n, int64_t lda, int *d_ipiv, int *info, void *stream);
+int  bfa_lu_solve_c(const void *d_A, int n, int64_t lda, const int *d_ipiv,
+                    void *d_B, int nrhs, int64_t ldb, void *stream);
+/* Solve M64 x = b with the complex64 LU factors of M64 and FP64 iterative
+ * refinement (residual b - M64 x in FP64 against the resident FP64 matrix)
+ * until max_j ||r_j||/||b_j|| <= tol or maxit iterations.  d_B: RHS in,
+ * solution out (complex128).  iters/resid (host, may be NULL) report the
+ * refinement steps taken and the final relative residual.                  */
+int  bfa_solve_refine(const void *d_M64, int64_t ldm, const void *d_LU32, int64_t ldlu,
+                      const int *d_ipiv, int n, void *d_B, int nrhs, int64_t ldb,
+                      double tol, int maxit, int *iters, double *resid, void *stream);
+
 /* Dense complex128 product C(m x n) -= A(m x k) * B(k x n), column-major;
  * the MFMA kernel of the LU trailing update (HMatrix::Multiply-style use).  */
 int  bfa_zgemm_sub(int m, int n, int k, const void *d_A, int64_t lda, const void *d_B,

@@ -61,7 +61,7 @@ def test_vie_matrix(bfa, orc, torch_cuda, n, matkind):
 def test_zgemm(bfa, torch_cuda):
     torch = torch_cuda
     rng = np.random.default_rng(2)
-    for (m, n, k) in [(16, 16, 4), (64, 64, 16), (70, 33, 21), (200, 130, 256)]:
+    for (m, n, k) in [(16, 16, 4), (64, 64, 16), (70, 33, 21), (200, 130, 256), (300, 260, 100), (512, 768, 64)]:
         print(f"[gemm] {m}x{n}x{k}", flush=True)
         A = rng.standard_normal((m, k)) + 1j * rng.standard_normal((m, k))
         B = rng.standard_normal((k, n)) + 1j * rng.standard_normal((k, n))
@@ -80,7 +80,7 @@ def test_zgemm(bfa, torch_cuda):
 def test_cgemm(bfa, torch_cuda):
     torch = torch_cuda
     rng = np.random.default_rng(3)
-    for (m, n, k) in [(16, 16, 4), (70, 33, 21), (200, 130, 256)]:
+    for (m, n, k) in [(16, 16, 4), (70, 33, 21), (200, 130, 256), (520, 384, 48)]:
         print(f"[cgemm] {m}x{n}x{k}", flush=True)
         A = (rng.standard_normal((m, k)) + 1j * rng.standard_normal((m, k))).astype(np.complex64)
         B = (rng.standard_normal((k, n)) + 1j * rng.standard_normal((k, n))).astype(np.complex64)
@@ -99,7 +99,7 @@ def test_cgemm(bfa, torch_cuda):
 def test_lu_random(bfa, torch_cuda):
     torch = torch_cuda
     rng = np.random.default_rng(1)
-    for n in (5, 64, 300, 700):
+    for n in (5, 64, 300, 700, 1500):
         print(f"[lu] n={n}", flush=True)
         A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
         B = rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3))
@@ -144,3 +144,31 @@ def test_pipeline_power(bfa, orc, torch_cuda):
     Po = o.emt_power(1.0, opw, Jo)
     print("power gpu", Pg, "oracle", Po)
     assert np.all(np.abs(Pg - Po) / np.abs(Po) < POW_TOL)
+
+
+def test_mixed_precision_pipeline(bfa, orc, torch_cuda):
+    """FP32 LU + FP64 refinement reproduces the FP64 oracle's power to 1e-8."""
+    torch = torch_cuda
+    print("[mixed] n=3", flush=True)
+    g, o = setup_geom(bfa, orc, 3)
+    N = g.N
+    spec = [((1, 0, 0), (0, 0, 1)), ((0, 1, 0), (0, 0, 1))]
+    pw = bfa.planewaves(spec)
+    dM = torch.zeros((N, N), dtype=torch.complex128, device="cuda")
+    dM32 = torch.zeros((N, N), dtype=torch.complex64, device="cuda")
+    dB = torch.zeros((2, N), dtype=torch.complex128, device="cuda")
+    dp = torch.zeros(N, dtype=torch.int32, device="cuda")
+    g.assemble_vie(1.0, bfa.material("iso", eps=4 + 0.1j), dM)
+    g.assemble_rhs(1.0, pw, dB)
+    bfa.convert_z2c(dM, N, N, dM32)
+    assert bfa.lu_factor_c(dM32, N, dp) == 0
+    it, res = bfa.solve_refine(dM, dM32, dp, N, dB, 2, tol=1e-12, maxit=12)
+    print("refinement iterations", it, "residual", res)
+    assert res <= 1e-12
+    Jg = colmajor(dB)
+    Mo = o.assemble(1.0, orc.material("iso", eps=4 + 0.1j), o.static_table(10))
+    Jo = orc.lu_solve(Mo, o.rhs(1.0, orc.planewaves(spec)))
+    assert np.linalg.norm(Jg - Jo) / np.linalg.norm(Jo) < 1e-10
+    Pg = g.emt_power(1.0, pw, dB)
+    Po = o.emt_power(1.0, orc.planewaves(spec), Jo)
+    assert np.all(np.abs(Pg - Po) / np.abs(Po) < 1e-8)
