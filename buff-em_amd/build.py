@@ -56,6 +56,15 @@ def build(jobs=4, verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # host driver on the drop-in C++ API (buff-scatter's frequency loop)
+    drv = os.path.join(HERE, "lib", "buff_scatter_amd")
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(HERE, "host"),
+           os.path.join(HERE, "host", "buff_scatter_amd.cpp"), "-o", drv + ".tmp",
+           "-L" + os.path.dirname(LIB), "-lbuffem_amd", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(drv + ".tmp", drv)
     return LIB
 
 

@@ -172,3 +172,28 @@ def test_mixed_precision_pipeline(bfa, orc, torch_cuda):
     Pg = g.emt_power(1.0, pw, dB)
     Po = o.emt_power(1.0, orc.planewaves(spec), Jo)
     assert np.all(np.abs(Pg - Po) / np.abs(Po) < 1e-8)
+
+
+def test_cpp_driver_matches_python_path(bfa, orc, torch_cuda):
+    """buff_scatter_amd (C++ drop-in, buff-scatter's loop) reproduces the oracle's power."""
+    import os
+    import subprocess
+    from conftest import ROOT
+    print("[driver] n=2", flush=True)
+    drv = os.path.join(ROOT, "buff-em_amd", "lib", "buff_scatter_amd")
+    r = subprocess.run([drv, "--kuhn", "2", "1.0", "--eps", "4", "0.1", "--omega", "1", "--omega", "0.5",
+                        "--pw", "1", "0", "0", "0", "0", "1"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = [l.split() for l in r.stdout.strip().splitlines()]
+    assert len(lines) == 2
+    V, T = kuhn(bfa, 2)
+    o = orc.Geometry(V, T)
+    tab = o.static_table(12)
+    for l in lines:
+        w = float(l[0])
+        M = o.assemble(w, orc.material("iso", eps=4 + 0.1j), tab)
+        pw = orc.planewaves([((1, 0, 0), (0, 0, 1))])
+        J = orc.lu_solve(M, o.rhs(w, pw))
+        P = o.emt_power(w, pw, J)[0]
+        got = np.array([float(l[2]), float(l[3])])
+        assert np.all(np.abs(got - P) / np.abs(P) < 1e-8), (got, P)

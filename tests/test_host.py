@@ -131,3 +131,12 @@ def test_error_codes(bfa, tmp_path):
     with pytest.raises(bfa.BFAError):
         bfa.Geometry(V, np.array([[0, 1, 2, 9]], np.int32))
     assert L.bfa_status_string(-3) == b"singular matrix"
+
+
+def test_cpp_dropin_builds():
+    """The C++ drop-in header and the buff-scatter driver compile against the C ABI."""
+    import subprocess
+    drv = os.path.join(ROOT, "buff-em_amd", "lib", "buff_scatter_amd")
+    assert os.path.exists(drv)
+    r = subprocess.run([drv], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr
