@@ -79,6 +79,15 @@ _SIGS = {
     "bfa_memcpy_htod": (I, [P, P, ctypes.c_size_t, P]),
     "bfa_memcpy_dtoh": (I, [P, P, ctypes.c_size_t, P]),
     "bfa_stream_synchronize": (I, [P]),
+    "bfa_batch_create": (I, [I, P, P, P, I, P]),
+    "bfa_batch_destroy": (I, [P]),
+    "bfa_batch_info": (I, [P, P, P, P, P]),
+    "bfa_batch_tune": (I, [P, I]),
+    "bfa_batch_tuning": (I, [P, P, P]),
+    "bfa_batch_run": (I, [P, P, I, I, D, I, P, P, P, P]),
+    "bfa_batch_timing": (I, [P, P]),
+    "bfa_batch_solution": (I, [P, I, P]),
+    "bfa_batch_matrix": (I, [P, I, P]),
 }
 
 
@@ -327,3 +336,67 @@ def solve_refine(d_M64, d_LU32, d_ipiv, n, d_B, nrhs, tol=1e-10, maxit=10, ldm=N
                                   _ptr(d_B), int(nrhs), int(ldb or n), float(tol), int(maxit), ctypes.byref(it),
                                   ctypes.byref(res), _stream_ptr(stream)), "bfa_solve_refine")
     return it.value, res.value
+
+
+class Batch:
+    """The batching form: many instances (Geometry, material) swept over
+    frequencies in grouped launches, FP32 LU + FP64 refinement, EMT power.
+
+    geoms must be uploaded with their static tables built; they must outlive
+    the batch.  pws: planewaves(...) result (at most 8)."""
+
+    def __init__(self, geoms, mats, pws):
+        self._geoms = list(geoms)
+        n = len(self._geoms)
+        gh = (P * n)(*[g.h for g in self._geoms])
+        ma = (bfa_material * n)(*mats)
+        self.npw = len(pws)
+        h = P()
+        _check(lib().bfa_batch_create(n, gh, ma, pws, self.npw, ctypes.byref(h)), "bfa_batch_create")
+        self._h = h
+        self.count = n
+        self.N = np.zeros(n, np.int32)
+        self.Np = np.zeros(n, np.int32)
+        _check(lib().bfa_batch_info(h, None, None, self.N.ctypes.data, self.Np.ctypes.data), "bfa_batch_info")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.bfa_batch_destroy(h)
+            self._h = None
+
+    def tune(self, reps=3):
+        _check(lib().bfa_batch_tune(self._h, int(reps)), "bfa_batch_tune")
+        tile = np.zeros(16, np.int32)
+        ms = np.zeros((16, 2))
+        _check(lib().bfa_batch_tuning(self._h, tile.ctypes.data, ms.ctypes.data), "bfa_batch_tuning")
+        return tile, ms
+
+    def run(self, omegas, use_graph=True, tol=1e-10, maxit=6):
+        """Returns dict with power [nf, count, npw, 2], iters/resid/info [nf, count]."""
+        w = np.ascontiguousarray(omegas, dtype=np.float64)
+        nf = len(w)
+        power = np.zeros((nf, self.count, self.npw, 2))
+        iters = np.zeros((nf, self.count), np.int32)
+        resid = np.zeros((nf, self.count))
+        info = np.zeros((nf, self.count), np.int32)
+        _check(lib().bfa_batch_run(self._h, w.ctypes.data, nf, int(bool(use_graph)), float(tol), int(maxit),
+                                   power.ctypes.data, iters.ctypes.data, resid.ctypes.data, info.ctypes.data),
+               "bfa_batch_run")
+        return {"power": power, "iters": iters, "resid": resid, "info": info}
+
+    def timing(self):
+        ms = np.zeros(5)
+        _check(lib().bfa_batch_timing(self._h, ms.ctypes.data), "bfa_batch_timing")
+        return {"total_ms": ms[0], "assembly_ms": ms[1], "factor_ms": ms[2], "solve_ms": ms[3], "power_ms": ms[4]}
+
+    def solution(self, i):
+        J = np.zeros((self.npw, int(self.N[i])), np.complex128)
+        _check(lib().bfa_batch_solution(self._h, int(i), J.ctypes.data), "bfa_batch_solution")
+        return J.T
+
+    def matrix(self, i):
+        n = int(self.N[i])
+        T = np.zeros((n, n), np.complex128)
+        _check(lib().bfa_batch_matrix(self._h, int(i), T.ctypes.data), "bfa_batch_matrix")
+        return T.T

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "bfa_internal.h"
+#include "bfa_group.h"
 #include "bfa_rules.h"
 #define TD_QUAL __constant__
 #include "td_regions.h"
@@ -590,12 +591,9 @@ __device__ __forceinline__ void tile_coords(int64_t bid, int ntiles, int &ti, in
 }
 
 template <bool REALK>
-__global__ __launch_bounds__(256) void k_far(DevGeom g, double kre, double kim, cd *M, int64_t ld, int ntiles)
+__device__ __forceinline__ void far_tile(const DevGeom &g, double kre, double kim, cd *M, int64_t ld, int ti, int tj,
+                                         double (&cpts)[TILE][8][7], int (&cverts)[TILE][5])
 {
-  __shared__ double cpts[TILE][8][7];
-  __shared__ int cverts[TILE][5];
-  int ti, tj;
-  tile_coords(blockIdx.x, ntiles, ti, tj);
   const int tid = threadIdx.x, N = g.nbf;
   for (int idx = tid; idx < TILE * 8; idx += blockDim.x) {
     const int bl = idx >> 3, pt = idx & 7, b = tj * TILE + bl;
@@ -651,6 +649,16 @@ __global__ __launch_bounds__(256) void k_far(DevGeom g, double kre, double kim, 
     M[(size_t)b * ld + a] = v;
     M[(size_t)a * ld + b] = v;
   }
+}
+
+template <bool REALK>
+__global__ __launch_bounds__(256) void k_far(DevGeom g, double kre, double kim, cd *M, int64_t ld, int ntiles)
+{
+  __shared__ double cpts[TILE][8][7];
+  __shared__ int cverts[TILE][5];
+  int ti, tj;
+  tile_coords(blockIdx.x, ntiles, ti, tj);
+  far_tile<REALK>(g, kre, kim, M, ld, ti, tj, cpts, cverts);
 }
 
 // ---------------------------------------------------------------------------
@@ -737,11 +745,9 @@ __device__ cd overlap_tet(const DevGeom &g, int t, const double *QA, double pfA,
   return cscale(acc, -1.0 / (omega * omega));
 }
 
-__global__ __launch_bounds__(256) void k_near(DevGeom g, double kre, double kim, const double *table,
-                                              bfa_material mat, cd *M, int64_t ld)
+__device__ __forceinline__ void near_entry(const DevGeom &g, double kre, double kim, const double *table,
+                                           const bfa_material &mat, cd *M, int64_t ld, int64_t j)
 {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= g.nnz) return;
   const int a = g.pairRow[j], b = g.cols[j];
   const cd k = {kre, kim}, ik = {-kim, kre};
   const cd k2 = cmul(k, k);
@@ -778,6 +784,13 @@ __global__ __launch_bounds__(256) void k_near(DevGeom g, double kre, double kim,
     }
   M[(size_t)b * ld + a] = acc;
   M[(size_t)a * ld + b] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_near(DevGeom g, double kre, double kim, const double *table,
+                                              bfa_material mat, cd *M, int64_t ld)
+{
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < g.nnz) near_entry(g, kre, kim, table, mat, M, ld, j);
 }
 
 extern "C" int bfa_assemble_vie(bfa_geometry *g, double omega_re, double omega_im, const bfa_material *mat,
@@ -983,6 +996,161 @@ extern "C" int bfa_emt_power(bfa_geometry *g, double omega, const bfa_planewave 
   }
   return BFA_OK;
 }
+
+// ===========================================================================
+// Grouped (batch) kernels: every instance's work in one launch; the
+// frequency comes from device memory so that one captured graph serves a
+// whole sweep.
+// ===========================================================================
+enum { GF_FAR = 0, GF_NEAR, GF_ROW, GF_EMT };
+__device__ __forceinline__ int inst_first(const AsmInst &q, int which)
+{
+  return which == GF_FAR ? q.far0 : which == GF_NEAR ? q.near0 : which == GF_ROW ? q.row0 : q.emt0;
+}
+__device__ __forceinline__ int find_inst(const AsmInst *__restrict__ d, int n, int bid, int which)
+{
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (inst_first(d[mid], which) <= bid) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void kg_far(const AsmInst *__restrict__ d, int ni, const FreqState *fs)
+{
+  __shared__ double cpts[TILE][8][7];
+  __shared__ int cverts[TILE][5];
+  const AsmInst &q = d[find_inst(d, ni, blockIdx.x, GF_FAR)];
+  int ti, tj;
+  tile_coords(blockIdx.x - q.far0, q.ntiles, ti, tj);
+  const double kre = fs->omega_re, kim = fs->omega_im;
+  if (kim == 0.0) far_tile<true>(q.g, kre, 0.0, (cd *)q.M, q.ldm, ti, tj, cpts, cverts);
+  else far_tile<false>(q.g, kre, kim, (cd *)q.M, q.ldm, ti, tj, cpts, cverts);
+}
+
+__global__ __launch_bounds__(256) void kg_near(const AsmInst *__restrict__ d, int ni, const FreqState *fs)
+{
+  const AsmInst &q = d[find_inst(d, ni, blockIdx.x, GF_NEAR)];
+  const int64_t j = (int64_t)(blockIdx.x - q.near0) * blockDim.x + threadIdx.x;
+  if (j < q.g.nnz) near_entry(q.g, fs->omega_re, fs->omega_im, q.table, q.mat, (cd *)q.M, q.ldm, j);
+}
+
+__global__ __launch_bounds__(256) void kg_rhs(const AsmInst *__restrict__ d, int ni, const FreqState *fs,
+                                              PWBatch pws, int npw)
+{
+  const AsmInst &q = d[find_inst(d, ni, blockIdx.x, GF_ROW)];
+  const int a = (blockIdx.x - q.row0) * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (a >= q.g.nbf || j >= npw) return;
+  const double omega = fs->omega_re;
+  const cd v = bf_dot_pw(q.g, a, pws.pw[j], omega);
+  const double s = 1.0 / (omega * ZVAC);
+  ((cd *)q.B)[(size_t)j * q.ldb + a] = {-v.im * s, v.re * s};
+}
+
+__global__ __launch_bounds__(256) void kg_emt_q(const AsmInst *__restrict__ d, int ni, const FreqState *fs)
+{
+  const AsmInst &q = d[find_inst(d, ni, blockIdx.x, GF_EMT)];
+  const int N = q.g.nbf;
+  const int64_t idx = (int64_t)(blockIdx.x - q.emt0) * blockDim.x + threadIdx.x;
+  const int b = (int)(idx / N), a = (int)(idx % N);
+  if (b >= N || a > b) return;
+  const DevGeom &g = q.g;
+  const double omega = fs->omega_re;
+  const int ncv = bf_ncv(g.bfV + 5 * a, g.bfV + 5 * b);
+  const double kR = fabs(omega) * fmax(g.bfRad[a], g.bfRad[b]);
+  const bool hf = kR > 5.0;
+  const int np = hf ? (ncv > 0 ? 33 : 16) : (ncv > 0 ? 16 : 4);
+  double v = 0.0;
+  for (int sa = 0; sa < 2; sa++)
+    for (int sb = 0; sb < 2; sb++)
+      v += emt_tetpair(g, g.bfT[2 * a + sa], g.bfPF[2 * a + sa], g.bfQ + 6 * a + 3 * sa, g.bfT[2 * b + sb],
+                       g.bfPF[2 * b + sb], g.bfQ + 6 * b + 3 * sb, np, omega);
+  q.Q[(size_t)b * N + a] = v;
+  q.Q[(size_t)a * N + b] = v;
+}
+
+__global__ __launch_bounds__(256) void kg_emt_rows(const AsmInst *__restrict__ d, int ni, const FreqState *fs,
+                                                   PWBatch pws, int npw)
+{
+  __shared__ double rs[256], re[256];
+  const AsmInst &q = d[find_inst(d, ni, blockIdx.x, GF_ROW)];
+  const int rb = blockIdx.x - q.row0, j = blockIdx.y;
+  const int a = rb * blockDim.x + threadIdx.x;
+  const int N = q.g.nbf;
+  const double omega = fs->omega_re;
+  double s = 0.0, e = 0.0;
+  if (a < N && j < npw) {
+    const cd *Jj = (const cd *)q.B + (size_t)j * q.ldb;
+    cd y = {0.0, 0.0};
+    for (int b = 0; b < N; b++) {
+      const double qq = q.Q[(size_t)b * N + a];
+      y.re += qq * Jj[b].re; y.im += qq * Jj[b].im;
+    }
+    const cd ja = Jj[a];
+    s = ja.re * y.re + ja.im * y.im + (ja.re * ja.re + ja.im * ja.im) * q.Q[(size_t)a * N + a];
+    const cd qe = bf_dot_pw(q.g, a, pws.pw[j], omega);
+    e = ja.re * qe.re + ja.im * qe.im;
+  }
+  rs[threadIdx.x] = s; re[threadIdx.x] = e;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) { rs[threadIdx.x] += rs[threadIdx.x + off]; re[threadIdx.x] += re[threadIdx.x + off]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && j < npw) {
+    q.part[(size_t)j * q.nrowblk + rb] = rs[0];
+    q.part[(size_t)(npw + j) * q.nrowblk + rb] = re[0];
+  }
+}
+
+// one workgroup per instance: fixed-order sums of the row-block partials
+__global__ __launch_bounds__(64) void kg_emt_final(const AsmInst *__restrict__ d, int ni, const FreqState *fs, int npw,
+                                                   double *out)
+{
+  const AsmInst &q = d[blockIdx.x];
+  const int j = threadIdx.x;
+  if (j >= npw) return;
+  double s = 0.0, x = 0.0;
+  for (int b = 0; b < q.nrowblk; b++) {
+    s += q.part[(size_t)j * q.nrowblk + b];
+    x += q.part[(size_t)(npw + j) * q.nrowblk + b];
+  }
+  const double pscat = ZVAC * 0.5 * s, pext = 0.5 * x;
+  double *o = out + 2 * (((size_t)fs->fidx * ni + blockIdx.x) * npw + j);
+  o[0] = pext - pscat;
+  o[1] = pscat;
+}
+
+namespace bfa {
+void launch_far_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, hipStream_t st)
+{
+  if (nblk > 0) hipLaunchKernelGGL(kg_far, dim3(nblk), dim3(256), 0, st, d, ni, fs);
+}
+void launch_near_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, hipStream_t st)
+{
+  if (nblk > 0) hipLaunchKernelGGL(kg_near, dim3(nblk), dim3(256), 0, st, d, ni, fs);
+}
+void launch_rhs_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, const PWBatch &pw, int npw, hipStream_t st)
+{
+  if (nblk > 0) hipLaunchKernelGGL(kg_rhs, dim3(nblk, npw), dim3(256), 0, st, d, ni, fs, pw, npw);
+}
+void launch_emt_q_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, hipStream_t st)
+{
+  if (nblk > 0) hipLaunchKernelGGL(kg_emt_q, dim3(nblk), dim3(256), 0, st, d, ni, fs);
+}
+void launch_emt_rows_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, const PWBatch &pw, int npw,
+                       hipStream_t st)
+{
+  if (nblk > 0) hipLaunchKernelGGL(kg_emt_rows, dim3(nblk, npw), dim3(256), 0, st, d, ni, fs, pw, npw);
+}
+void launch_emt_final_g(const AsmInst *d, int ni, const FreqState *fs, int npw, double *out, hipStream_t st)
+{
+  if (ni > 0) hipLaunchKernelGGL(kg_emt_final, dim3(ni), dim3(64), 0, st, d, ni, fs, npw, out);
+}
+}  // namespace bfa
 
 // ---------------------------------------------------------------------------
 extern "C" int bfa_device_malloc(void **ptr, size_t bytes)

@@ -1,0 +1,129 @@
+// bfa_group.h -- grouped ("batched, uneven sizes") launches of the dense
+// LU / solve kernels.  One launch covers the same operation on many
+// independent matrices: the launch's workgroups are split among problems by
+// a prefix (blk0) stored in each problem descriptor, and every workgroup
+// looks up its problem by binary search.  Descriptors live in device memory
+// prepared once per plan, so a launch sequence can be replayed (or captured
+// into a hipGraph) with no host work in between.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+#include "bfa_internal.h"
+
+namespace bfa {
+
+// C(m x n) -= A(m x k) * B(k x n); tiles_m x tiles_n workgroups from blk0
+struct GemmProb {
+  const void *A, *B;
+  void *C;
+  int64_t lda, ldb, ldc;
+  int m, n, k, tiles_m, blk0, pad;
+};
+
+// B(kb x n) := L^-1 B (unit lower) or U^-1 B (upper), kb <= 32; one
+// workgroup per 64 columns of B from blk0
+struct TrsmProb {
+  const void *L;
+  void *B;
+  int64_t ldl, ldb;
+  int kb, n, blk0, pad;
+};
+
+// rows k0..k0+kb-1 <-> ipiv[.] on columns [c0, c1) of A; 128 columns per
+// workgroup from blk0
+struct SwapProb {
+  void *A;
+  const int *ipiv;
+  int64_t lda;
+  int c0, c1, k0, kb, blk0, pad;
+};
+
+// Panel workspace of one problem (cross-workgroup slots and barrier)
+struct PanelSlots {
+  double *val;      // [2][maxwg]
+  int *row;         // [2][maxwg]
+  void *rowdata;    // [2][maxwg][32] complex
+  void *rowj;       // [2][32] complex
+  unsigned *bar;    // barrier counter, zero at launch
+  int *flags;       // [0] spin timeout, [1] first zero pivot (1-based)
+};
+
+// m x w panel (w <= 32) factored by nwg workgroups from blk0, rpw rows each
+struct PanelProb {
+  void *A;
+  int *ipiv;
+  int64_t lda;
+  int m, w, rpw, blk0, nwg, piv_off;
+  PanelSlots slots;
+};
+
+// Triangular solve pieces of the multi-RHS solve, nrhs <= 8:
+//   diag:   X(kb x nrhs) := L^-1 X or U^-1 X, kb <= 64, one workgroup
+//   update: Y(m x nrhs) -= L(m x kb) Xk, 64 rows per workgroup from blk0
+// skip (optional): the problem is idle when *skip != 0.
+struct TrsvProb {
+  const void *L;
+  const void *Xk;
+  void *Y;
+  const int *skip;
+  int64_t ldl, ldx, ldy;
+  int m, kb, nrhs, blk0;
+};
+
+// launchers (f64 selects complex128, else complex64); nblk = total workgroups
+void launch_gemm_g(bool f64, int tile, const GemmProb *d, int np, int nblk, hipStream_t st);
+void launch_trsm_g(bool f64, bool lower, const TrsmProb *d, int np, int nblk, hipStream_t st);
+void launch_swap_g(bool f64, const SwapProb *d, int np, int nblk, hipStream_t st);
+void launch_panel_g(bool f64, const PanelProb *d, int np, int nblk, size_t lds, hipStream_t st);
+void launch_trsv_diag_g(bool f64, bool lower, const TrsvProb *d, int np, hipStream_t st);
+void launch_trsv_update_g(bool f64, const TrsvProb *d, int np, int nblk, hipStream_t st);
+// panel capacity: LDS bytes available per workgroup and the co-resident
+// workgroup count of the grouped panel kernel (0 on error)
+size_t panel_lds_cap(bool f64);
+int panel_max_resident(bool f64, size_t lds);
+
+// ---------------------------------------------------------------------------
+// Grouped assembly / RHS / power over the instances of a batch.  Each
+// instance carries its own geometry, material and matrices, plus the first
+// workgroup of each grouped launch that it owns.
+struct AsmInst {
+  DevGeom g;
+  bfa_material mat;
+  const double *table;   // static records [nnz][6]
+  void *M;               // complex128 VIE matrix, N x N used, ld = ldm
+  void *B;               // complex128 right-hand sides / solutions, ld = ldb
+  double *Q;             // EMT scratch, N x N doubles, ld = N
+  double *part;          // EMT partial sums [2][npw][row blocks]
+  int64_t ldm, ldb;
+  int far0, ntiles;      // far-field tiles (upper triangle of 64x64) from far0
+  int near0, row0;       // touching-pair blocks (256) / row blocks (256)
+  int emt0, nrowblk;     // EMT Q blocks (256 entries of N x N) from emt0
+};
+
+// Frequency of the current replay: written by a one-thread kernel before a
+// graph launch, read by every frequency-dependent kernel.
+struct FreqState {
+  double omega_re, omega_im;
+  int fidx, pad;
+};
+
+#define BFA_MAXPW_BATCH 8
+struct PWBatch { bfa_planewave pw[BFA_MAXPW_BATCH]; };
+
+void launch_far_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, hipStream_t st);
+void launch_near_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, hipStream_t st);
+void launch_rhs_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, const PWBatch &pw, int npw,
+                  hipStream_t st);
+void launch_emt_q_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, hipStream_t st);
+void launch_emt_rows_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, const PWBatch &pw, int npw,
+                       hipStream_t st);
+// out[(fidx * ni + i) * npw + j][2] = {P_abs, P_scat}
+void launch_emt_final_g(const AsmInst *d, int ni, const FreqState *fs, int npw, double *out, hipStream_t st);
+
+// GEMM tile variants for the autotuner
+enum { TILE_64 = 0, TILE_128 = 1, NUM_TILES = 2 };
+inline int tile_rows(int t) { return t == TILE_128 ? 128 : 64; }
+
+}  // namespace bfa

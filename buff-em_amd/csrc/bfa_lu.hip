@@ -7,8 +7,8 @@
 //
 // Layout: column-major, interleaved complex, leading dimension ld.
 // Algorithm: right-looking blocking (outer block NB); the panel is factored
-// recursively down to width PW, where a cooperative kernel keeps the panel's
-// rows resident in LDS across all workgroups and performs one grid barrier
+// recursively down to width PW, where a persistent kernel (all workgroups
+// co-resident) keeps the panel's rows in LDS and performs one grid barrier
 // per column (pivot = first index of max |re|+|im|, the izamax rule).  The
 // trailing update C -= A*B runs on MFMA (4 real products per complex
 // product, v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).
@@ -20,6 +20,7 @@
 
 #include "bfa_internal.h"
 #include "bfa_lu.h"
+#include "bfa_group.h"
 
 using namespace bfa;
 
@@ -73,15 +74,13 @@ template <> struct mf<float> {
 #define GBK 16
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_gemm_sub(int m, int n, int k, const cplx<T> *__restrict__ A, int64_t lda,
-                                                  const cplx<T> *__restrict__ B, int64_t ldb, cplx<T> *C, int64_t ldc)
+__device__ __forceinline__ void gemm64_body(int m, int n, int k, const cplx<T> *__restrict__ A, int64_t lda,
+                                            const cplx<T> *__restrict__ B, int64_t ldb, cplx<T> *C, int64_t ldc,
+                                            int row0, int col0, T (&As)[2][GBK][GBM], T (&Bs)[2][GBK][GBN])
 {
   typedef typename mf<T>::acc acc_t;
-  __shared__ T As[2][GBK][GBM];
-  __shared__ T Bs[2][GBK][GBN];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
-  const int row0 = blockIdx.x * GBM, col0 = blockIdx.y * GBN;
   const int l16 = lane & 15, lq = lane >> 4;
   acc_t cr[2][2], ci[2][2];
   // load C into accumulators
@@ -155,6 +154,15 @@ __global__ __launch_bounds__(256) void k_gemm_sub(int m, int n, int k, const cpl
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemm_sub(int m, int n, int k, const cplx<T> *__restrict__ A, int64_t lda,
+                                                  const cplx<T> *__restrict__ B, int64_t ldb, cplx<T> *C, int64_t ldc)
+{
+  __shared__ T As[2][GBK][GBM];
+  __shared__ T Bs[2][GBK][GBN];
+  gemm64_body<T>(m, n, k, A, lda, B, ldb, C, ldc, blockIdx.x * GBM, blockIdx.y * GBN, As, Bs);
+}
+
 // Large-tile variant: workgroup tile 128 x 128 complex, 8 waves (2 x 4) of
 // 64 x 32 each, K staged 16 at a time through double-buffered LDS with the
 // next stage prefetched into registers; workgroups are renumbered so that
@@ -162,24 +170,23 @@ __global__ __launch_bounds__(256) void k_gemm_sub(int m, int n, int k, const cpl
 #define G2M 128
 #define G2N 128
 #define G2K 16
+// XCD-aware renumbering of a linear workgroup id: the dispatcher deals
+// workgroups round-robin over the 8 XCDs, so XCD x gets a contiguous band of
+// tiles (bijective when the tile count is a multiple of 8, identity otherwise)
+__device__ __forceinline__ int xcd_remap(int bid, int ntile)
+{
+  return (ntile % 8 == 0) ? (bid % 8) * (ntile / 8) + bid / 8 : bid;
+}
+
 template <typename T>
-__global__ __launch_bounds__(512) void k_gemm_sub2(int m, int n, int k, const cplx<T> *__restrict__ A, int64_t lda,
-                                                   const cplx<T> *__restrict__ B, int64_t ldb, cplx<T> *C, int64_t ldc,
-                                                   int tiles_m, int tiles_n)
+__device__ __forceinline__ void gemm128_body(int m, int n, int k, const cplx<T> *__restrict__ A, int64_t lda,
+                                             const cplx<T> *__restrict__ B, int64_t ldb, cplx<T> *C, int64_t ldc,
+                                             int row0, int col0, T (&As)[2][2][G2K][G2M], T (&Bs)[2][2][G2K][G2N])
 {
   typedef typename mf<T>::acc acc_t;
-  __shared__ T As[2][2][G2K][G2M];
-  __shared__ T Bs[2][2][G2K][G2N];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 2, wc = wave & 3;
   const int l16 = lane & 15, lq = lane >> 4;
-  // XCD-aware renumbering of the linear workgroup id (bijective when the
-  // tile count is a multiple of the 8 XCDs, identity otherwise)
-  const int ntile = tiles_m * tiles_n;
-  const int bid = blockIdx.x;
-  const int t = (ntile % 8 == 0) ? (bid % 8) * (ntile / 8) + bid / 8 : bid;
-  const int tm = t % tiles_m, tn = t / tiles_m;
-  const int row0 = tm * G2M, col0 = tn * G2N;
   acc_t cr[4][2], ci[4][2];
 #pragma unroll
   for (int rb = 0; rb < 4; rb++)
@@ -259,6 +266,17 @@ __global__ __launch_bounds__(512) void k_gemm_sub2(int m, int n, int k, const cp
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(512) void k_gemm_sub2(int m, int n, int k, const cplx<T> *__restrict__ A, int64_t lda,
+                                                   const cplx<T> *__restrict__ B, int64_t ldb, cplx<T> *C, int64_t ldc,
+                                                   int tiles_m, int tiles_n)
+{
+  __shared__ T As[2][2][G2K][G2M];
+  __shared__ T Bs[2][2][G2K][G2N];
+  const int t = xcd_remap(blockIdx.x, tiles_m * tiles_n);
+  gemm128_body<T>(m, n, k, A, lda, B, ldb, C, ldc, (t % tiles_m) * G2M, (t / tiles_m) * G2N, As, Bs);
+}
+
 // Optional instrumentation: HIP events around every GEMM launch, so a caller
 // can read the kernel's total time and algorithmic flop count.
 struct GemmProfile {
@@ -313,12 +331,10 @@ static int gemm_sub(int m, int n, int k, const cplx<T> *A, int64_t lda, const cp
 #define TRSM_KB 32
 #define TRSM_COLS 64
 template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) void k_trsm(int kb, int n, const cplx<T> *L, int64_t ldl, cplx<T> *B, int64_t ldb)
+__device__ __forceinline__ void trsm_body(int kb, int n, const cplx<T> *L, int64_t ldl, cplx<T> *B, int64_t ldb,
+                                          int c0, cplx<T> (&X)[TRSM_KB][TRSM_COLS], cplx<T> (&Ls)[TRSM_KB][TRSM_KB])
 {
-  __shared__ cplx<T> X[TRSM_KB][TRSM_COLS];
-  __shared__ cplx<T> Ls[TRSM_KB][TRSM_KB];
   const int tid = threadIdx.x;
-  const int c0 = blockIdx.x * TRSM_COLS;
   const int nc = min(TRSM_COLS, n - c0);
   for (int idx = tid; idx < kb * TRSM_COLS; idx += blockDim.x) {
     const int r = idx % kb, c = idx / kb;
@@ -349,6 +365,14 @@ __global__ __launch_bounds__(256) void k_trsm(int kb, int n, const cplx<T> *L, i
     const int r = idx % kb, cc = idx / kb;
     if (cc < nc) B[(size_t)(c0 + cc) * ldb + r] = X[r][cc];
   }
+}
+
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void k_trsm(int kb, int n, const cplx<T> *L, int64_t ldl, cplx<T> *B, int64_t ldb)
+{
+  __shared__ cplx<T> X[TRSM_KB][TRSM_COLS];
+  __shared__ cplx<T> Ls[TRSM_KB][TRSM_KB];
+  trsm_body<T, LOWER>(kb, n, L, ldl, B, ldb, blockIdx.x * TRSM_COLS, X, Ls);
 }
 
 template <typename T, bool LOWER>
@@ -388,10 +412,8 @@ static void trsm(int kb, int n, const cplx<T> *L, int64_t ldl, cplx<T> *B, int64
 // Row interchanges rows k0..k0+kb-1 <-> ipiv[.] (absolute, 0-based) on the
 // columns [c0, c1) of A; one thread per column, swaps applied in order.
 template <typename T>
-__global__ void k_laswp(cplx<T> *A, int64_t lda, int c0, int c1, int k0, int kb, const int *ipiv)
+__device__ __forceinline__ void laswp_col(cplx<T> *A, int64_t lda, int c, int k0, int kb, const int *ipiv)
 {
-  const int c = c0 + blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= c1) return;
   cplx<T> *col = A + (size_t)c * lda;
   for (int i = 0; i < kb; i++) {
     const int p = ipiv[k0 + i];
@@ -404,6 +426,13 @@ __global__ void k_laswp(cplx<T> *A, int64_t lda, int c0, int c1, int k0, int kb,
 }
 
 template <typename T>
+__global__ void k_laswp(cplx<T> *A, int64_t lda, int c0, int c1, int k0, int kb, const int *ipiv)
+{
+  const int c = c0 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < c1) laswp_col<T>(A, lda, c, k0, kb, ipiv);
+}
+
+template <typename T>
 static void laswp(cplx<T> *A, int64_t lda, int c0, int c1, int k0, int kb, const int *ipiv, hipStream_t st)
 {
   if (c1 <= c0 || kb <= 0) return;
@@ -411,21 +440,14 @@ static void laswp(cplx<T> *A, int64_t lda, int c0, int c1, int k0, int kb, const
 }
 
 // ---------------------------------------------------------------------------
-// Cooperative panel factorisation of an m x w panel (w <= PW).  Workgroup g
+// Persistent panel factorisation of an m x w panel (w <= PW).  Workgroup g
 // owns rows [g*rpw, (g+1)*rpw) in LDS.  Per column: local argmax, publish
 // (value, row, row data) and, by the owner, the current row j; grid barrier;
 // every workgroup resolves the same pivot, swaps in LDS, scales and updates.
 #define PW 32
-#define SPIN_LIMIT (1u << 28)
+#define SPIN_LIMIT (1u << 24)
 
-struct PanelWork {
-  double *val;      // [2][maxwg]
-  int *row;         // [2][maxwg]
-  void *rowdata;    // [2][maxwg][PW] complex
-  void *rowj;       // [2][PW] complex
-  unsigned *bar;    // barrier counter
-  int *flags;       // [0] spin timeout, [1] first zero pivot (1-based, 0 none)
-};
+typedef bfa::PanelSlots PanelWork;
 
 // Grid barrier: every thread's stores are drained by the workgroup barrier,
 // one thread per workgroup publishes them device-wide (release fence) and
@@ -445,19 +467,30 @@ __device__ __forceinline__ void grid_barrier(unsigned *bar, unsigned target, int
   __syncthreads();
 }
 
+template <typename T> struct PanelShared {
+  cplx<T> prow[PW], rj[PW];
+  T sval[256];
+  int srow[256], swin[256];
+  int piv_s, win_s;
+  T pval_s;
+};
+
+// Body shared by the single-matrix and the grouped panel kernels: workgroup
+// g of the nwg that factor this panel; rows = the workgroup's LDS block.
 template <typename T>
-__global__ __launch_bounds__(256) void k_panel(cplx<T> *A, int m, int w, int64_t lda, int *ipiv, int rpw, PanelWork pw,
-                                               int piv_off)
+__device__ __forceinline__ void panel_body(cplx<T> *A, int m, int w, int64_t lda, int *ipiv, int rpw,
+                                           const PanelWork &pw, int piv_off, int g, int nwg, cplx<T> *rows,
+                                           PanelShared<T> &sh)
 {
-  extern __shared__ unsigned char smem_raw[];
-  cplx<T> *rows = (cplx<T> *)smem_raw;                 // column-major [w][rpw]
 #define RW(r, c) rows[(size_t)(c) * rpw + (r)]
-  __shared__ cplx<T> prow[PW], rj[PW];
-  __shared__ T sval[256];
-  __shared__ int srow[256], swin[256];
-  __shared__ int piv_s, win_s;
-  __shared__ T pval_s;
-  const int tid = threadIdx.x, g = blockIdx.x, nwg = gridDim.x;
+  cplx<T>(&prow)[PW] = sh.prow;
+  cplx<T>(&rj)[PW] = sh.rj;
+  T(&sval)[256] = sh.sval;
+  int(&srow)[256] = sh.srow;
+  int(&swin)[256] = sh.swin;
+  int &piv_s = sh.piv_s, &win_s = sh.win_s;
+  T &pval_s = sh.pval_s;
+  const int tid = threadIdx.x;
   const int r0 = g * rpw, r1 = min(m, r0 + rpw), nr = max(0, r1 - r0);
   cplx<T> *rdata = (cplx<T> *)pw.rowdata;
   cplx<T> *rowjg = (cplx<T> *)pw.rowj;
@@ -554,6 +587,15 @@ __global__ __launch_bounds__(256) void k_panel(cplx<T> *A, int m, int w, int64_t
 #undef RW
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_panel(cplx<T> *A, int m, int w, int64_t lda, int *ipiv, int rpw, PanelWork pw,
+                                               int piv_off)
+{
+  extern __shared__ unsigned char smem_raw[];
+  __shared__ PanelShared<T> sh;
+  panel_body<T>(A, m, w, lda, ipiv, rpw, pw, piv_off, blockIdx.x, gridDim.x, (cplx<T> *)smem_raw, sh);
+}
+
 __global__ void k_zero_u32(unsigned *p, int n)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -565,6 +607,73 @@ __global__ void k_add_offset(int *ipiv, int n, int off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) ipiv[i] += off;
 }
+
+// ---------------------------------------------------------------------------
+// Grouped kernels: one launch, many independent problems (descriptors in
+// device memory, sorted by their first workgroup blk0).
+template <typename P>
+__device__ __forceinline__ int find_prob(const P *__restrict__ d, int np, int bid)
+{
+  int lo = 0, hi = np - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d[mid].blk0 <= bid) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kg_gemm64(const GemmProb *__restrict__ d, int np)
+{
+  __shared__ T As[2][GBK][GBM];
+  __shared__ T Bs[2][GBK][GBN];
+  const GemmProb q = d[find_prob(d, np, blockIdx.x)];
+  const int t = blockIdx.x - q.blk0;
+  gemm64_body<T>(q.m, q.n, q.k, (const cplx<T> *)q.A, q.lda, (const cplx<T> *)q.B, q.ldb, (cplx<T> *)q.C, q.ldc,
+                 (t % q.tiles_m) * GBM, (t / q.tiles_m) * GBN, As, Bs);
+}
+
+template <typename T>
+__global__ __launch_bounds__(512) void kg_gemm128(const GemmProb *__restrict__ d, int np)
+{
+  __shared__ T As[2][2][G2K][G2M];
+  __shared__ T Bs[2][2][G2K][G2N];
+  const int bid = xcd_remap(blockIdx.x, gridDim.x);
+  const GemmProb q = d[find_prob(d, np, bid)];
+  const int t = bid - q.blk0;
+  gemm128_body<T>(q.m, q.n, q.k, (const cplx<T> *)q.A, q.lda, (const cplx<T> *)q.B, q.ldb, (cplx<T> *)q.C, q.ldc,
+                  (t % q.tiles_m) * G2M, (t / q.tiles_m) * G2N, As, Bs);
+}
+
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void kg_trsm(const TrsmProb *__restrict__ d, int np)
+{
+  __shared__ cplx<T> X[TRSM_KB][TRSM_COLS];
+  __shared__ cplx<T> Ls[TRSM_KB][TRSM_KB];
+  const TrsmProb q = d[find_prob(d, np, blockIdx.x)];
+  trsm_body<T, LOWER>(q.kb, q.n, (const cplx<T> *)q.L, q.ldl, (cplx<T> *)q.B, q.ldb,
+                      (blockIdx.x - q.blk0) * TRSM_COLS, X, Ls);
+}
+
+template <typename T>
+__global__ __launch_bounds__(128) void kg_swap(const SwapProb *__restrict__ d, int np)
+{
+  const SwapProb q = d[find_prob(d, np, blockIdx.x)];
+  const int c = q.c0 + (blockIdx.x - q.blk0) * 128 + threadIdx.x;
+  if (c < q.c1) laswp_col<T>((cplx<T> *)q.A, q.lda, c, q.k0, q.kb, q.ipiv);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kg_panel(const PanelProb *__restrict__ d, int np)
+{
+  extern __shared__ unsigned char smem_raw[];
+  __shared__ PanelShared<T> sh;
+  const PanelProb q = d[find_prob(d, np, blockIdx.x)];
+  panel_body<T>((cplx<T> *)q.A, q.m, q.w, q.lda, q.ipiv, q.rpw, q.slots, q.piv_off, blockIdx.x - q.blk0, q.nwg,
+                (cplx<T> *)smem_raw, sh);
+}
+
 
 }  // namespace
 
@@ -628,12 +737,21 @@ static int panel_base(cplx<T> *A, int m, int w, int64_t lda, int *ipiv, int piv_
   nwg = (m + rpw - 1) / rpw;
   if ((size_t)rpw * row_bytes > lds_cap || nwg > lw->maxwg)
     return fail(BFA_ERR_ARG, "panel of %d rows too tall for LDS-resident factorisation", m);
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, lw->pw.bar, 16);
   size_t lds = (size_t)rpw * w * sizeof(cplx<T>);
-  void *args[] = {(void *)&A, (void *)&m, (void *)&w, (void *)&lda, (void *)&ipiv, (void *)&rpw, (void *)&lw->pw,
-                  (void *)&piv_off};
-  hipError_t e = hipLaunchCooperativeKernel((const void *)k_panel<T>, dim3(nwg), dim3(256), args, (unsigned)lds, st);
-  if (e != hipSuccess) return fail(BFA_ERR_HIP, "cooperative panel launch (%d workgroups): %s", nwg, hipGetErrorString(e));
+  // The grid barrier needs every workgroup resident at once: the grid never
+  // exceeds what the occupancy calculator says fits on the device, and the
+  // barrier's bounded spin reports (flags[0]) rather than hangs otherwise.
+  static int resident[2] = {0, 0};
+  if (!resident[ti]) {
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_panel<T>, 256, lds_cap) != hipSuccess) per = 0;
+    resident[ti] = per * lw->num_cu;
+  }
+  if (nwg > resident[ti])
+    return fail(BFA_ERR_HIP, "panel needs %d co-resident workgroups, device fits %d", nwg, resident[ti]);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, lw->pw.bar, 16);
+  hipLaunchKernelGGL(k_panel<T>, dim3(nwg), dim3(256), lds, st, A, m, w, lda, ipiv, rpw, lw->pw, piv_off);
+  HIPCHECK(hipGetLastError());
   return BFA_OK;
 }
 
@@ -705,11 +823,10 @@ __global__ void k_gather_rows(const cplx<T> *B, int64_t ldb, cplx<T> *X, int64_t
 #define SV_KB 64
 #define SV_RHS 8
 template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) void k_trsv_diag(int kb, int nrhs, const cplx<T> *L, int64_t ldl, cplx<T> *X,
-                                                   int64_t ldx)
+__device__ __forceinline__ void trsv_diag_body(int kb, int nrhs, const cplx<T> *L, int64_t ldl, cplx<T> *X,
+                                               int64_t ldx, cplx<T> (&Ls)[SV_KB][SV_KB + 1],
+                                               cplx<T> (&Xs)[SV_RHS][SV_KB])
 {
-  __shared__ cplx<T> Ls[SV_KB][SV_KB + 1];
-  __shared__ cplx<T> Xs[SV_RHS][SV_KB];
   const int tid = threadIdx.x;
   for (int idx = tid; idx < kb * kb; idx += blockDim.x) {
     const int r = idx % kb, c = idx / kb;
@@ -743,23 +860,36 @@ __global__ __launch_bounds__(256) void k_trsv_diag(int kb, int nrhs, const cplx<
   }
 }
 
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void k_trsv_diag(int kb, int nrhs, const cplx<T> *L, int64_t ldl, cplx<T> *X,
+                                                   int64_t ldx)
+{
+  __shared__ cplx<T> Ls[SV_KB][SV_KB + 1];
+  __shared__ cplx<T> Xs[SV_RHS][SV_KB];
+  trsv_diag_body<T, LOWER>(kb, nrhs, L, ldl, X, ldx, Ls, Xs);
+}
+
 // Y(m x nrhs) -= L(m x kb) * Xk(kb x nrhs), kb <= SV_KB: a workgroup owns 64
 // rows; its 4 waves take interleaved quarters of the columns (coalesced
 // column walks of L, Xk broadcast from LDS) and are reduced through LDS.
+template <typename T> struct TrsvShared {
+  cplx<T> xs[SV_RHS][SV_KB];
+  cplx<T> part[4][SV_RHS][64];
+};
 template <typename T>
-__global__ __launch_bounds__(256) void k_trsv_update(int m, int kb, int nrhs, const cplx<T> *__restrict__ L,
-                                                     int64_t ldl, const cplx<T> *__restrict__ Xk, int64_t ldx,
-                                                     cplx<T> *Y, int64_t ldy)
+__device__ __forceinline__ void trsv_update_body(int m, int kb, int nrhs, const cplx<T> *__restrict__ L, int64_t ldl,
+                                                 const cplx<T> *__restrict__ Xk, int64_t ldx, cplx<T> *Y, int64_t ldy,
+                                                 int rblk, TrsvShared<T> &sh)
 {
-  __shared__ cplx<T> xs[SV_RHS][SV_KB];
-  __shared__ cplx<T> part[4][SV_RHS][64];
+  cplx<T>(&xs)[SV_RHS][SV_KB] = sh.xs;
+  cplx<T>(&part)[4][SV_RHS][64] = sh.part;
   for (int idx = threadIdx.x; idx < kb * nrhs; idx += blockDim.x) {
     const int c = idx % kb, j = idx / kb;
     xs[j][c] = Xk[(size_t)j * ldx + c];
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int r = blockIdx.x * 64 + lane;
+  const int r = rblk * 64 + lane;
   T ar[SV_RHS], ai[SV_RHS];
 #pragma unroll
   for (int j = 0; j < SV_RHS; j++) { ar[j] = 0; ai[j] = 0; }
@@ -788,6 +918,35 @@ __global__ __launch_bounds__(256) void k_trsv_update(int m, int kb, int nrhs, co
       y.im -= si;
     }
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsv_update(int m, int kb, int nrhs, const cplx<T> *__restrict__ L,
+                                                     int64_t ldl, const cplx<T> *__restrict__ Xk, int64_t ldx,
+                                                     cplx<T> *Y, int64_t ldy)
+{
+  __shared__ TrsvShared<T> sh;
+  trsv_update_body<T>(m, kb, nrhs, L, ldl, Xk, ldx, Y, ldy, blockIdx.x, sh);
+}
+
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void kg_trsv_diag(const TrsvProb *__restrict__ d, int np)
+{
+  __shared__ cplx<T> Ls[SV_KB][SV_KB + 1];
+  __shared__ cplx<T> Xs[SV_RHS][SV_KB];
+  const TrsvProb q = d[blockIdx.x];
+  if (q.skip && *q.skip) return;
+  trsv_diag_body<T, LOWER>(q.kb, q.nrhs, (const cplx<T> *)q.L, q.ldl, (cplx<T> *)q.Y, q.ldy, Ls, Xs);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kg_trsv_update(const TrsvProb *__restrict__ d, int np)
+{
+  __shared__ TrsvShared<T> sh;
+  const TrsvProb q = d[find_prob(d, np, blockIdx.x)];
+  if (q.skip && *q.skip) return;
+  trsv_update_body<T>(q.m, q.kb, q.nrhs, (const cplx<T> *)q.L, q.ldl, (const cplx<T> *)q.Xk, q.ldx, (cplx<T> *)q.Y,
+                      q.ldy, blockIdx.x - q.blk0, sh);
 }
 
 template <typename T>
@@ -864,6 +1023,89 @@ int lu_solve_z(const void *A, int n, int64_t lda, const int *ipiv, void *B, int 
 { return lu_solve<double>((const cplx<double> *)A, n, lda, ipiv, (cplx<double> *)B, nrhs, ldb, st, s); }
 int lu_solve_c(const void *A, int n, int64_t lda, const int *ipiv, void *B, int nrhs, int64_t ldb, hipStream_t st, void *s)
 { return lu_solve<float>((const cplx<float> *)A, n, lda, ipiv, (cplx<float> *)B, nrhs, ldb, st, s); }
+
+void launch_gemm_g(bool f64, int tile, const GemmProb *d, int np, int nblk, hipStream_t st)
+{
+  if (np <= 0 || nblk <= 0) return;
+  if (tile == TILE_128) {
+    if (f64) hipLaunchKernelGGL(kg_gemm128<double>, dim3(nblk), dim3(512), 0, st, d, np);
+    else hipLaunchKernelGGL(kg_gemm128<float>, dim3(nblk), dim3(512), 0, st, d, np);
+  } else {
+    if (f64) hipLaunchKernelGGL(kg_gemm64<double>, dim3(nblk), dim3(256), 0, st, d, np);
+    else hipLaunchKernelGGL(kg_gemm64<float>, dim3(nblk), dim3(256), 0, st, d, np);
+  }
+}
+
+void launch_trsm_g(bool f64, bool lower, const TrsmProb *d, int np, int nblk, hipStream_t st)
+{
+  if (np <= 0 || nblk <= 0) return;
+  if (f64) {
+    if (lower) hipLaunchKernelGGL((kg_trsm<double, true>), dim3(nblk), dim3(256), 0, st, d, np);
+    else hipLaunchKernelGGL((kg_trsm<double, false>), dim3(nblk), dim3(256), 0, st, d, np);
+  } else {
+    if (lower) hipLaunchKernelGGL((kg_trsm<float, true>), dim3(nblk), dim3(256), 0, st, d, np);
+    else hipLaunchKernelGGL((kg_trsm<float, false>), dim3(nblk), dim3(256), 0, st, d, np);
+  }
+}
+
+void launch_swap_g(bool f64, const SwapProb *d, int np, int nblk, hipStream_t st)
+{
+  if (np <= 0 || nblk <= 0) return;
+  if (f64) hipLaunchKernelGGL(kg_swap<double>, dim3(nblk), dim3(128), 0, st, d, np);
+  else hipLaunchKernelGGL(kg_swap<float>, dim3(nblk), dim3(128), 0, st, d, np);
+}
+
+size_t panel_lds_cap(bool f64)
+{
+  static int done[2] = {0, 0};
+  const int i = f64 ? 0 : 1;
+  const size_t cap = 150 * 1024;
+  if (!done[i]) {
+    hipError_t e = f64 ? hipFuncSetAttribute((const void *)kg_panel<double>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)cap)
+                       : hipFuncSetAttribute((const void *)kg_panel<float>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)cap);
+    done[i] = e == hipSuccess ? 1 : -1;
+  }
+  return done[i] > 0 ? cap : 64 * 1024;
+}
+
+int panel_max_resident(bool f64, size_t lds)
+{
+  int dev = 0, ncu = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  hipError_t e = f64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kg_panel<double>, 256, lds)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kg_panel<float>, 256, lds);
+  if (e != hipSuccess) return 0;
+  return per * ncu;
+}
+
+void launch_panel_g(bool f64, const PanelProb *d, int np, int nblk, size_t lds, hipStream_t st)
+{
+  if (np <= 0 || nblk <= 0) return;
+  if (f64) hipLaunchKernelGGL(kg_panel<double>, dim3(nblk), dim3(256), lds, st, d, np);
+  else hipLaunchKernelGGL(kg_panel<float>, dim3(nblk), dim3(256), lds, st, d, np);
+}
+
+void launch_trsv_diag_g(bool f64, bool lower, const TrsvProb *d, int np, hipStream_t st)
+{
+  if (np <= 0) return;
+  if (f64) {
+    if (lower) hipLaunchKernelGGL((kg_trsv_diag<double, true>), dim3(np), dim3(256), 0, st, d, np);
+    else hipLaunchKernelGGL((kg_trsv_diag<double, false>), dim3(np), dim3(256), 0, st, d, np);
+  } else {
+    if (lower) hipLaunchKernelGGL((kg_trsv_diag<float, true>), dim3(np), dim3(256), 0, st, d, np);
+    else hipLaunchKernelGGL((kg_trsv_diag<float, false>), dim3(np), dim3(256), 0, st, d, np);
+  }
+}
+
+void launch_trsv_update_g(bool f64, const TrsvProb *d, int np, int nblk, hipStream_t st)
+{
+  if (np <= 0 || nblk <= 0) return;
+  if (f64) hipLaunchKernelGGL(kg_trsv_update<double>, dim3(nblk), dim3(256), 0, st, d, np);
+  else hipLaunchKernelGGL(kg_trsv_update<float>, dim3(nblk), dim3(256), 0, st, d, np);
+}
 
 }  // namespace bfa
 

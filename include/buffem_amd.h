@@ -145,6 +145,46 @@ int  bfa_cgemm_sub(int m, int n, int k, const void *d_A, int64_t lda, const void
 int  bfa_emt_power(bfa_geometry *g, double omega, const bfa_planewave *pw,
                    int npw, const void *d_J, int64_t ldj, double *out, void *stream);
 
+/* ---- batching form (no reference counterpart: the reference solves one
+ * problem per process, one frequency at a time, buff-scatter.cc:236-363) --
+ * A batch holds `count` independent instances (each an uploaded geometry
+ * with its static table built, and a material) excited by the same `npw`
+ * plane waves (npw <= 8).  Per frequency every instance is assembled
+ * (complex128), converted to a complex64 copy padded to a multiple of 256,
+ * LU-factored on FP32 MFMA and solved with FP64 iterative refinement
+ * against its complex128 matrix, then its EMT power is computed -- all in
+ * grouped launches covering every instance, replayed per frequency from
+ * one captured hipGraph when use_graph != 0.  Device memory per instance:
+ * N*Np*16 + Np*Np*8 bytes (Np = N rounded up to 256) plus O(N) vectors.   */
+typedef struct bfa_batch bfa_batch;
+int  bfa_batch_create(int count, bfa_geometry *const *geoms, const bfa_material *mats,
+                      const bfa_planewave *pw, int npw, bfa_batch **out);
+int  bfa_batch_destroy(bfa_batch *b);
+/* count, npw, and per instance N[count], Np[count] (host, may be NULL).    */
+int  bfa_batch_info(const bfa_batch *b, int *count, int *npw, int *N, int *Np);
+/* Autotuner: times each GEMM tile variant on the batch's own grouped
+ * trailing updates, per size bucket floor(log2 m), and keeps the fastest
+ * (reps timed launches each).  Without it a fixed rule picks the tile.    */
+int  bfa_batch_tune(bfa_batch *b, int reps);
+/* tile[16]: chosen variant per bucket (-1 = untuned; 0 = 64x64, 1 =
+ * 128x128); ms[16*2]: measured time per bucket and variant.                */
+int  bfa_batch_tuning(const bfa_batch *b, int *tile, double *ms);
+/* Sweep nfreq (<= 256) real frequencies.  Host outputs (may be NULL):
+ * power[((f*count + i)*npw + j)*2 + {0,1}] = {P_abs, P_scat};
+ * iters[f*count + i], resid[f*count + i]: refinement rounds and final
+ * relative residual (refinement stops at resid <= tol or maxit rounds);
+ * info[f*count + i]: 0, or the 1-based column of the first zero pivot.
+ * Synchronous.                                                             */
+int  bfa_batch_run(bfa_batch *b, const double *omega, int nfreq, int use_graph,
+                   double tol, int maxit, double *power, int *iters, double *resid, int *info);
+/* ms[5]: total device time of the last run, then per phase (assembly,
+ * convert+factor, solve, power) summed over frequencies (eager runs only). */
+int  bfa_batch_timing(const bfa_batch *b, double *ms);
+/* Host copies of the last frequency's solution J (N x npw, complex128,
+ * column-major) and complex128 VIE matrix (N x N) of instance `inst`.      */
+int  bfa_batch_solution(const bfa_batch *b, int inst, void *J);
+int  bfa_batch_matrix(const bfa_batch *b, int inst, void *M);
+
 /* ---- instrumentation (no reference counterpart) ------------------------ */
 /* When enabled, every trailing-update GEMM launch is bracketed by HIP
  * events; bfa_profile_read returns the summed kernel time (ms), the summed

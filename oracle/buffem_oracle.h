@@ -69,6 +69,12 @@ void orc_fibbi(const orc_geom *g, int a, int b, int td_order, double G[6]);
 void orc_gme(const orc_geom *g, int a, int b, double omr, double omi,
              const double *fibbi, double out[2]);
 
+/* 4x4 complex block of a tetrahedron pair (oracle_surr.c): out[2*(4i+j)]
+ * = G_ij, face i of VS / face j of VT, product TetCR(npts) rule on each
+ * tetrahedron refined `level` times by 8-way subdivision.                */
+void orc_tp_block(const double *VS, const double *VT, double kr, double ki, int npts, int level,
+                  double out[32]);
+
 /* ---------------- materials ------------------------------------------- */
 /* kind: 0 const isotropic eps=p[0]+i p[1]
  *       1 const diagonal   eps_xx=p0+ip1, yy=p2+ip3, zz=p4+ip5

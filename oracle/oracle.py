@@ -60,6 +60,7 @@ def lib():
             "orc_overlaps": (I, [P, I, D, P, P, P]),
             "orc_rhs": (None, [P, D, P, I, P]),
             "orc_emt_power": (None, [P, D, P, I, P, P]),
+            "orc_tp_block": (None, [P, P, D, D, I, I, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -122,6 +123,16 @@ def td_moments(ncv, VA, VB, cA, cB, order):
     lib().orc_td_moments(ncv, VA.ctypes.data, VB.ctypes.data, cA.ctypes.data, cB.ctypes.data, order,
                          out.ctypes.data)
     return out
+
+
+def tp_block(VS, VT, k, npts=4, level=0):
+    """4x4 complex block of a tetrahedron pair (face i of VS, face j of VT)."""
+    out = np.zeros(32)
+    VS = np.ascontiguousarray(VS, float).reshape(12)
+    VT = np.ascontiguousarray(VT, float).reshape(12)
+    k = complex(k)
+    lib().orc_tp_block(VS.ctypes.data, VT.ctypes.data, k.real, k.imag, int(npts), int(level), out.ctypes.data)
+    return (out[0::2] + 1j * out[1::2]).reshape(4, 4)
 
 
 class Geometry:
