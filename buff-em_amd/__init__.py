@@ -88,6 +88,20 @@ _SIGS = {
     "bfa_batch_timing": (I, [P, P]),
     "bfa_batch_solution": (I, [P, I, P]),
     "bfa_batch_matrix": (I, [P, I, P]),
+    "bfa_mlp_param_count": (I, [I, I, I, P, P]),
+    "bfa_mlp_grad": (I, [I, I, I, I, P, I, P, P, P, P, P]),
+    "bfa_surr_create": (I, [P, D, I, I, ctypes.c_uint64, P]),
+    "bfa_surr_destroy": (I, [P]),
+    "bfa_surr_pairs": (I, [P, P, P, P]),
+    "bfa_surr_cubature": (I, [P, P, P, I, I, I, P]),
+    "bfa_surr_features": (I, [P, P, P, I, P]),
+    "bfa_surr_dataset": (I, [P, I, D, D, I, I]),
+    "bfa_surr_train": (I, [P, I, I, D, P]),
+    "bfa_surr_table": (I, [P, D, I, D, I, I, P, P]),
+    "bfa_surr_table_download": (I, [P, P]),
+    "bfa_assemble_vie_table": (I, [P, D, P, P, I64, P]),
+    "bfa_surr_info": (I, [P, P, P, P, P]),
+    "bfa_surr_params": (I, [P, P, P]),
 }
 
 
@@ -372,23 +386,25 @@ class Batch:
         _check(lib().bfa_batch_tuning(self._h, tile.ctypes.data, ms.ctypes.data), "bfa_batch_tuning")
         return tile, ms
 
-    def run(self, omegas, use_graph=True, tol=1e-10, maxit=6):
-        """Returns dict with power [nf, count, npw, 2], iters/resid/info [nf, count]."""
+    def run(self, omegas, use_graph=True, tol=1e-10, maxit=6, with_power=True):
+        """Returns dict with power [nf, count, npw, 2] (None without the EMT pass),
+        iters/resid/info [nf, count]."""
         w = np.ascontiguousarray(omegas, dtype=np.float64)
         nf = len(w)
-        power = np.zeros((nf, self.count, self.npw, 2))
+        power = np.zeros((nf, self.count, self.npw, 2)) if with_power else None
         iters = np.zeros((nf, self.count), np.int32)
         resid = np.zeros((nf, self.count))
         info = np.zeros((nf, self.count), np.int32)
         _check(lib().bfa_batch_run(self._h, w.ctypes.data, nf, int(bool(use_graph)), float(tol), int(maxit),
-                                   power.ctypes.data, iters.ctypes.data, resid.ctypes.data, info.ctypes.data),
-               "bfa_batch_run")
+                                   power.ctypes.data if with_power else None, iters.ctypes.data,
+                                   resid.ctypes.data, info.ctypes.data), "bfa_batch_run")
         return {"power": power, "iters": iters, "resid": resid, "info": info}
 
     def timing(self):
-        ms = np.zeros(5)
+        ms = np.zeros(6)
         _check(lib().bfa_batch_timing(self._h, ms.ctypes.data), "bfa_batch_timing")
-        return {"total_ms": ms[0], "assembly_ms": ms[1], "factor_ms": ms[2], "solve_ms": ms[3], "power_ms": ms[4]}
+        return {"solve_path_ms": ms[0], "assembly_ms": ms[1], "factor_ms": ms[2], "solve_ms": ms[3],
+                "power_phase_ms": ms[4], "power_ms": ms[5]}
 
     def solution(self, i):
         J = np.zeros((self.npw, int(self.N[i])), np.complex128)
@@ -400,3 +416,105 @@ class Batch:
         T = np.zeros((n, n), np.complex128)
         _check(lib().bfa_batch_matrix(self._h, int(i), T.ctypes.data), "bfa_batch_matrix")
         return T.T
+
+
+def mlp_param_count(nin, hidden=256, nout=32):
+    n, kp = I64(), I()
+    _check(lib().bfa_mlp_param_count(int(nin), int(hidden), int(nout), ctypes.byref(n), ctypes.byref(kp)),
+           "bfa_mlp_param_count")
+    return n.value, kp.value
+
+
+def mlp_grad(params, X, T, nin, hidden=256, nout=32):
+    """GPU forward + MSE + backward of E ensemble members on one batch.
+    params [E][nparam] float32, X [E][B][nin], T [E][B][nout] -> (loss [E], grad [E][nparam], Y [E][B][nout])."""
+    params = np.ascontiguousarray(params, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    T = np.ascontiguousarray(T, np.float32)
+    E, B = X.shape[0], X.shape[1]
+    loss = np.zeros(E)
+    grad = np.zeros_like(params)
+    Y = np.zeros((E, B, nout), np.float32)
+    _check(lib().bfa_mlp_grad(int(nin), int(hidden), int(nout), E, params.ctypes.data, B, X.ctypes.data,
+                              T.ctypes.data, loss.ctypes.data, grad.ctypes.data, Y.ctypes.data), "bfa_mlp_grad")
+    return loss, grad, Y
+
+
+class Surrogate:
+    """Learned near-field surrogate over the non-touching tetrahedron pairs of
+    one geometry closer than `factor` diameters (4x4 complex blocks)."""
+
+    RULE, CUBATURE, SURROGATE = 0, 1, 2
+
+    def __init__(self, geom, factor=3.0, ensemble=4, hidden=256, seed=1):
+        self.geom = geom
+        h = P()
+        _check(lib().bfa_surr_create(geom.h, float(factor), int(ensemble), int(hidden), int(seed),
+                                     ctypes.byref(h)), "bfa_surr_create")
+        self._h = h
+        n = I()
+        _check(lib().bfa_surr_pairs(h, ctypes.byref(n), None, None), "bfa_surr_pairs")
+        self.npairs = n.value
+        self.ensemble = int(ensemble)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.bfa_surr_destroy(h)
+            self._h = None
+
+    def pairs(self):
+        ps = np.zeros(self.npairs, np.int32)
+        pt = np.zeros(self.npairs, np.int32)
+        _check(lib().bfa_surr_pairs(self._h, None, ps.ctypes.data, pt.ctypes.data), "bfa_surr_pairs")
+        return ps, pt
+
+    def cubature(self, pair, omega, npts=33, level=1):
+        pair = np.ascontiguousarray(pair, np.int32)
+        omega = np.ascontiguousarray(np.broadcast_to(omega, pair.shape), np.float64)
+        out = np.zeros((len(pair), 32))
+        _check(lib().bfa_surr_cubature(self._h, pair.ctypes.data, omega.ctypes.data, len(pair), int(npts),
+                                       int(level), out.ctypes.data), "bfa_surr_cubature")
+        return (out[:, 0::2] + 1j * out[:, 1::2]).reshape(-1, 4, 4)
+
+    def features(self, pair, omega):
+        pair = np.ascontiguousarray(pair, np.int32)
+        omega = np.ascontiguousarray(np.broadcast_to(omega, pair.shape), np.float64)
+        out = np.zeros((len(pair), 13), np.float32)
+        _check(lib().bfa_surr_features(self._h, pair.ctypes.data, omega.ctypes.data, len(pair), out.ctypes.data),
+               "bfa_surr_features")
+        return out
+
+    def dataset(self, per_pair=2, wmin=0.1, wmax=10.0, label_npts=33, label_level=1):
+        _check(lib().bfa_surr_dataset(self._h, int(per_pair), float(wmin), float(wmax), int(label_npts),
+                                      int(label_level)), "bfa_surr_dataset")
+
+    def train(self, epochs=10, batch=4096, lr=1e-3):
+        hist = np.zeros((epochs, self.ensemble))
+        _check(lib().bfa_surr_train(self._h, int(epochs), int(batch), float(lr), hist.ctypes.data), "bfa_surr_train")
+        return hist
+
+    def table(self, omega, mode=2, tol=1e-7, npts=33, level=1):
+        """Build the block table at omega; returns (cubature fallbacks, max ensemble disagreement)."""
+        nf, dm = I(), D()
+        _check(lib().bfa_surr_table(self._h, float(omega), int(mode), float(tol), int(npts), int(level),
+                                    ctypes.byref(nf), ctypes.byref(dm)), "bfa_surr_table")
+        return nf.value, dm.value
+
+    def table_download(self):
+        out = np.zeros((self.npairs, 32))
+        _check(lib().bfa_surr_table_download(self._h, out.ctypes.data), "bfa_surr_table_download")
+        return (out[:, 0::2] + 1j * out[:, 1::2]).reshape(-1, 4, 4)
+
+    def assemble(self, omega, mat, d_M, ld=None, stream=None):
+        _check(lib().bfa_assemble_vie_table(self._h, float(omega), ctypes.byref(mat), _ptr(d_M),
+                                            int(ld or self.geom.N), _stream_ptr(stream)), "bfa_assemble_vie_table")
+
+    def params(self, new=None):
+        E, n = I(), I64()
+        _check(lib().bfa_surr_info(self._h, ctypes.byref(E), ctypes.byref(n), None, None), "bfa_surr_info")
+        out = np.zeros((E.value, n.value), np.float32)
+        src = None if new is None else np.ascontiguousarray(new, np.float32)
+        _check(lib().bfa_surr_params(self._h, out.ctypes.data, None if src is None else src.ctypes.data),
+               "bfa_surr_params")
+        return out

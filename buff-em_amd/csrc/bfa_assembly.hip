@@ -818,6 +818,18 @@ extern "C" int bfa_assemble_vie(bfa_geometry *g, double omega_re, double omega_i
   return BFA_OK;
 }
 
+int bfa::assemble_near_only(bfa_geometry *g, double omega_re, double omega_im, const bfa_material *mat, void *d_M,
+                            int64_t ld, void *stream)
+{
+  if (!g || !mat || !d_M) return fail(BFA_ERR_ARG, "assemble_near_only: null argument");
+  if (!g->uploaded || !g->static_built) return fail(BFA_ERR_STATE, "assemble_near_only: static table not built");
+  const int64_t nnz = g->dg.nnz;
+  hipLaunchKernelGGL(k_near, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, (hipStream_t)stream, g->dg, omega_re,
+                     omega_im, (const double *)g->d.table, *mat, (cd *)d_M, ld);
+  HIPCHECK(hipGetLastError());
+  return BFA_OK;
+}
+
 // ===========================================================================
 // RHS: v_a = -1/(i omega Z0) int b_a . E_inc (TetCR33 on both tetrahedra)
 // ===========================================================================

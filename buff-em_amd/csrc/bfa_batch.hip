@@ -301,10 +301,10 @@ struct bfa_batch {
   double plan_tol = 0.0;
   // graph of one frequency
   hipStream_t cap = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr, graph_p = nullptr;
+  hipGraphExec_t exec = nullptr, exec_p = nullptr;
   double phase_ms[4] = {0, 0, 0, 0};
-  double run_ms = 0.0;
+  double run_ms = 0.0, power_ms = 0.0;
 
   template <typename T> T *at(size_t o) const { return (T *)((char *)small + o); }
   int *ipiv(int i) const { return at<int>(off[i].ipiv); }
@@ -323,6 +323,8 @@ struct bfa_batch {
   {
     if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
     if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+    if (exec_p) { (void)hipGraphExecDestroy(exec_p); exec_p = nullptr; }
+    if (graph_p) { (void)hipGraphDestroy(graph_p); graph_p = nullptr; }
     if (ddesc) { (void)hipFree(ddesc); ddesc = nullptr; }
     hdesc.clear();
     ops.clear();
@@ -1032,50 +1034,79 @@ extern "C" int bfa_batch_run(bfa_batch *B, const double *omega, int nfreq, int u
                        (const int *)B->iters(), (const double *)B->resid(), (const int *)B->flags(), hit, hres, hinfo);
   };
   if (use_graph && !B->exec) {
-    HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    for (auto &f : B->ops) f(st);
-    record(st);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(st, &g);
-    if (e != hipSuccess) return fail(BFA_ERR_HIP, "bfa_batch_run: graph capture failed: %s", hipGetErrorString(e));
-    B->graph = g;
-    e = hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0);
-    if (e != hipSuccess) return fail(BFA_ERR_HIP, "bfa_batch_run: graph instantiation failed: %s", hipGetErrorString(e));
+    // graph 1: assembly, convert, factor, solve, status record; graph 2: power
+    for (int part = 0; part < 2; part++) {
+      HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      for (size_t k = 0; k < B->ops.size(); k++)
+        if ((B->op_phase[k] == 3) == (part == 1)) B->ops[k](st);
+      if (part == 0) record(st);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(st, &g);
+      if (e != hipSuccess) return fail(BFA_ERR_HIP, "bfa_batch_run: graph capture failed: %s", hipGetErrorString(e));
+      hipGraphExec_t x = nullptr;
+      e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+      if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(BFA_ERR_HIP, "bfa_batch_run: graph instantiation failed: %s", hipGetErrorString(e));
+      }
+      if (part == 0) { B->graph = g; B->exec = x; }
+      else { B->graph_p = g; B->exec_p = x; }
+    }
   }
-  hipEvent_t ev[6];
+  // events per frequency: [start, after solve, after power]
+  std::vector<hipEvent_t> ev(3 * (size_t)nfreq + 5);
   for (auto &x : ev) HIPCHECK(hipEventCreate(&x));
+  hipEvent_t *ph_ev = ev.data() + 3 * (size_t)nfreq;   // eager phase marks [4] + end
   double ph[4] = {0, 0, 0, 0};
-  HIPCHECK(hipEventRecord(ev[0], st));
   for (int f = 0; f < nfreq; f++) {
+    hipEvent_t *e3 = ev.data() + 3 * (size_t)f;
+    HIPCHECK(hipEventRecord(e3[0], st));
     hipLaunchKernelGGL(k_set_state, dim3(1), dim3(1), 0, st, B->fs(), omega[f], 0.0, f);
     if (use_graph) {
       HIPCHECK(hipGraphLaunch(B->exec, st));
+      HIPCHECK(hipEventRecord(e3[1], st));
+      if (power) HIPCHECK(hipGraphLaunch(B->exec_p, st));
+      HIPCHECK(hipEventRecord(e3[2], st));
     } else {
       int cur = -1;
       for (size_t k = 0; k < B->ops.size(); k++) {
         if (B->op_phase[k] != cur) {
           cur = B->op_phase[k];
-          HIPCHECK(hipEventRecord(ev[1 + cur], st));
+          HIPCHECK(hipEventRecord(ph_ev[cur], st));
+          if (cur == 3) {
+            record(st);
+            HIPCHECK(hipEventRecord(e3[1], st));
+            if (!power) break;
+          }
         }
         B->ops[k](st);
       }
-      HIPCHECK(hipEventRecord(ev[5], st));
-      record(st);
-      HIPCHECK(hipEventSynchronize(ev[5]));
+      if (cur != 3) {   // no power ops ran
+        HIPCHECK(hipEventRecord(ph_ev[3], st));
+      }
+      HIPCHECK(hipEventRecord(e3[2], st));
+      HIPCHECK(hipEventRecord(ph_ev[4], st));
+      HIPCHECK(hipEventSynchronize(ph_ev[4]));
       float ms;
       for (int p = 0; p < 4; p++) {
-        HIPCHECK(hipEventElapsedTime(&ms, ev[1 + p], p < 3 ? ev[2 + p] : ev[5]));
+        HIPCHECK(hipEventElapsedTime(&ms, ph_ev[p], ph_ev[p + 1]));
         ph[p] += ms;
       }
     }
   }
-  HIPCHECK(hipEventRecord(ev[5], st));
-  HIPCHECK(hipEventSynchronize(ev[5]));
-  float tot = 0.f;
-  HIPCHECK(hipEventElapsedTime(&tot, ev[0], ev[5]));
+  HIPCHECK(hipEventSynchronize(ev[3 * (size_t)nfreq - 1]));
+  double tsolve = 0.0, tpower = 0.0;
+  for (int f = 0; f < nfreq; f++) {
+    float a = 0.f, b = 0.f;
+    HIPCHECK(hipEventElapsedTime(&a, ev[3 * f], ev[3 * f + 1]));
+    HIPCHECK(hipEventElapsedTime(&b, ev[3 * f + 1], ev[3 * f + 2]));
+    tsolve += a;
+    tpower += b;
+  }
   for (auto &x : ev) (void)hipEventDestroy(x);
   HIPCHECK(hipGetLastError());
-  B->run_ms = tot;
+  B->run_ms = tsolve;
+  B->power_ms = tpower;
   for (int p = 0; p < 4; p++) B->phase_ms[p] = use_graph ? 0.0 : ph[p];
   if (power) HIPCHECK(hipMemcpy(power, B->power, (size_t)nfreq * ni * B->npw * 16, hipMemcpyDeviceToHost));
   if (iters) HIPCHECK(hipMemcpy(iters, hit, (size_t)nfreq * ni * 4, hipMemcpyDeviceToHost));
@@ -1093,6 +1124,7 @@ extern "C" int bfa_batch_timing(const bfa_batch *B, double *ms)
   if (!B || !ms) return fail(BFA_ERR_ARG, "bfa_batch_timing: bad argument");
   ms[0] = B->run_ms;
   for (int p = 0; p < 4; p++) ms[1 + p] = B->phase_ms[p];
+  ms[5] = B->power_ms;
   return BFA_OK;
 }
 

@@ -51,6 +51,13 @@ struct DevBuffers {
 
 }  // namespace bfa
 
+struct bfa_geometry;
+namespace bfa {
+// touching-pair pass of the assembly alone (near field + static + Gram)
+int assemble_near_only(bfa_geometry *g, double omega_re, double omega_im, const bfa_material *mat, void *d_M,
+                       int64_t ld, void *stream);
+}  // namespace bfa
+
 struct bfa_geometry {
   int nv = 0, nt = 0, nbf = 0, nftot = 0;
   std::vector<double> V;                      // 3*nv

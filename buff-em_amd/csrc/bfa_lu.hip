@@ -282,8 +282,10 @@ __global__ __launch_bounds__(512) void k_gemm_sub2(int m, int n, int k, const cp
 struct GemmProfile {
   bool on = false;
   std::vector<hipEvent_t> pool;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> rec[2];
-  double flops[2] = {0, 0};
+  // [type * 2 + variant]: type 0 complex128 / 1 complex64; variant 0 the
+  // 64x64 kernel (k_gemm_sub), 1 the 128x128 kernel (k_gemm_sub2)
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> rec[4];
+  double flops[4] = {0, 0, 0, 0};
   size_t used = 0;
 };
 static GemmProfile g_prof;
@@ -309,7 +311,8 @@ static int gemm_sub(int m, int n, int k, const cplx<T> *A, int64_t lda, const cp
   hipEvent_t e0 = nullptr, e1 = nullptr;
   const int p = sizeof(T) == 8 ? 0 : 1;
   if (g_prof.on) prof_begin(st, e0);
-  if (m >= 256 && n >= 256) {
+  const bool big = m >= 256 && n >= 256;
+  if (big) {
     const int tm = (m + G2M - 1) / G2M, tn = (n + G2N - 1) / G2N;
     hipLaunchKernelGGL(k_gemm_sub2<T>, dim3(tm * tn), dim3(512), 0, st, m, n, k, A, lda, B, ldb, C, ldc, tm, tn);
   } else {
@@ -317,8 +320,8 @@ static int gemm_sub(int m, int n, int k, const cplx<T> *A, int64_t lda, const cp
   }
   if (g_prof.on) {
     prof_begin(st, e1);
-    g_prof.rec[p].push_back({e0, e1});
-    g_prof.flops[p] += 8.0 * m * (double)n * k;
+    g_prof.rec[2 * p + (big ? 1 : 0)].push_back({e0, e1});
+    g_prof.flops[2 * p + (big ? 1 : 0)] += 8.0 * m * (double)n * k;
   }
   return BFA_OK;
 }
@@ -1159,19 +1162,34 @@ extern "C" int bfa_profile_enable(int on)
 
 extern "C" int bfa_profile_read(int which, double *ms, double *flops, int *launches)
 {
-  if (which < 0 || which > 1) return fail(BFA_ERR_ARG, "bfa_profile_read: which must be 0 (z) or 1 (c)");
-  double t = 0.0;
-  for (auto &pr : g_prof.rec[which]) {
-    HIPCHECK(hipEventSynchronize(pr.second));
-    float x = 0.0f;
-    HIPCHECK(hipEventElapsedTime(&x, pr.first, pr.second));
-    t += x;
+  // which: 0 / 1 = every complex128 / complex64 trailing-update launch;
+  // 2 / 3 = only the 128x128-tile kernel (k_gemm_sub2) of that type
+  if (which < 0 || which > 3) return fail(BFA_ERR_ARG, "bfa_profile_read: which must be 0..3");
+  const int type = which & 1;
+  const int lo = which < 2 ? 2 * type : 2 * type + 1, hi = 2 * type + 1;
+  double t = 0.0, f = 0.0;
+  int n = 0;
+  for (int r = lo; r <= hi; r++) {
+    for (auto &pr : g_prof.rec[r]) {
+      HIPCHECK(hipEventSynchronize(pr.second));
+      float x = 0.0f;
+      HIPCHECK(hipEventElapsedTime(&x, pr.first, pr.second));
+      t += x;
+    }
+    f += g_prof.flops[r];
+    n += (int)g_prof.rec[r].size();
   }
   if (ms) *ms = t;
-  if (flops) *flops = g_prof.flops[which];
-  if (launches) *launches = (int)g_prof.rec[which].size();
-  g_prof.rec[which].clear();
-  g_prof.flops[which] = 0.0;
-  if (g_prof.rec[0].empty() && g_prof.rec[1].empty()) g_prof.used = 0;
+  if (flops) *flops = f;
+  if (launches) *launches = n;
+  // reading 0 / 1 clears the records of that type (both kernels); 2 / 3 keep them
+  if (which >= 2) return BFA_OK;
+  for (int r = 2 * type; r <= 2 * type + 1; r++) {
+    g_prof.rec[r].clear();
+    g_prof.flops[r] = 0.0;
+  }
+  bool empty = true;
+  for (auto &v : g_prof.rec) empty = empty && v.empty();
+  if (empty) g_prof.used = 0;
   return BFA_OK;
 }

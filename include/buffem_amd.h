@@ -174,22 +174,80 @@ int  bfa_batch_tuning(const bfa_batch *b, int *tile, double *ms);
  * iters[f*count + i], resid[f*count + i]: refinement rounds and final
  * relative residual (refinement stops at resid <= tol or maxit rounds);
  * info[f*count + i]: 0, or the 1-based column of the first zero pivot.
- * Synchronous.                                                             */
+ * With power == NULL the EMT pass is skipped.  Synchronous.               */
 int  bfa_batch_run(bfa_batch *b, const double *omega, int nfreq, int use_graph,
                    double tol, int maxit, double *power, int *iters, double *resid, int *info);
-/* ms[5]: total device time of the last run, then per phase (assembly,
- * convert+factor, solve, power) summed over frequencies (eager runs only). */
+/* ms[6] for the last run, summed over its frequencies: [0] device time of
+ * assembly + convert + factor + solve; [1..4] per phase (assembly,
+ * convert+factor, solve, power; eager runs only, else 0); [5] power.       */
 int  bfa_batch_timing(const bfa_batch *b, double *ms);
 /* Host copies of the last frequency's solution J (N x npw, complex128,
  * column-major) and complex128 VIE matrix (N x N) of instance `inst`.      */
 int  bfa_batch_solution(const bfa_batch *b, int inst, void *J);
 int  bfa_batch_matrix(const bfa_batch *b, int inst, void *M);
 
+/* ---- learned near-field surrogate: MLP kernels (no reference counterpart)
+ * Ensemble of E networks nin -> hidden -> hidden -> hidden -> nout, GELU,
+ * FP16 weights/activations into FP32-accumulating MFMA; parameter block per
+ * member (FP32, row-major, kp = nin rounded up to 16):
+ *   W1[hidden][kp] b1 W2[hidden][hidden] b2 W3[hidden][hidden] b3 W4[nout][hidden] b4 */
+int  bfa_mlp_param_count(int nin, int hidden, int nout, int64_t *count, int *kp);
+/* One forward + MSE + backward pass on host data: params [E][count],
+ * X [E][B][nin], T [E][B][nout]; outputs (may be NULL) loss[E],
+ * grad [E][count], Y [E][B][nout].  hidden <= 256 (multiple of 64),
+ * nout a multiple of 16.                                                   */
+int  bfa_mlp_grad(int nin, int hidden, int nout, int E, const float *params, int B,
+                  const float *X, const float *T, double *loss, float *grad, float *Y);
+
+/* ---- learned near-field surrogate (no reference counterpart) -----------
+ * Over the non-touching tetrahedron pairs of one geometry whose centroids
+ * are closer than `factor` x the larger diameter, a table of 4x4 complex
+ * blocks G_ij = int_S int_T [f_i.g_j - div f_i div g_j/k^2] e^{ikr}/(4 pi r)
+ * (face i of S, face j of T) feeds the far-field assembly.  The blocks come
+ * from the reference's 4-point rule, from cubature (product TetCR rule of
+ * npts in {4,16,33} on tetrahedra subdivided `level` in {0,1,2} times), or
+ * from an ensemble of MLPs (13 inputs: the target's vertices in the source
+ * frame over the source's mean edge, log(k h); 32 outputs: the 4-point
+ * rule's error over the rule block's norm) with cubature fallback.         */
+typedef struct bfa_surrogate bfa_surrogate;
+int  bfa_surr_create(bfa_geometry *g, double factor, int ensemble, int hidden, uint64_t seed,
+                     bfa_surrogate **out);
+int  bfa_surr_destroy(bfa_surrogate *s);
+/* npairs, and the pair list (tetrahedron indices, host, may be NULL).      */
+int  bfa_surr_pairs(const bfa_surrogate *s, int *npairs, int *ps, int *pt);
+/* Cubature blocks of pair[n] at omega[n] (host): blocks[n][32] (re, im of
+ * G_00, G_01, ..., G_33).                                                  */
+int  bfa_surr_cubature(bfa_surrogate *s, const int *pair, const double *omega, int n, int npts,
+                       int level, double *blocks);
+/* Network inputs of pair[n] at omega[n]: feat[n][13] (host).               */
+int  bfa_surr_features(bfa_surrogate *s, const int *pair, const double *omega, int n, float *feat);
+/* Training set: every pair at per_pair seeded log-uniform frequencies in
+ * [wmin, wmax], labelled by cubature (label_npts, label_level).             */
+int  bfa_surr_dataset(bfa_surrogate *s, int per_pair, double wmin, double wmax, int label_npts,
+                      int label_level);
+/* Adam with cosine step decay; loss_hist[epochs][ensemble] (host, may be NULL). */
+int  bfa_surr_train(bfa_surrogate *s, int epochs, int batch, double lr, double *loss_hist);
+/* Block table at omega: mode 0 4-point rule, 1 cubature (npts, level),
+ * 2 surrogate with cubature (npts, level) where the ensemble disagreement
+ * exceeds tol or an input lies outside the training range.               */
+int  bfa_surr_table(bfa_surrogate *s, double omega, int mode, double tol, int npts, int level,
+                    int *nfallback, double *max_disagree);
+int  bfa_surr_table_download(const bfa_surrogate *s, double *blocks /* npairs*32 */);
+/* bfa_assemble_vie with the far field read from the block table.           */
+int  bfa_assemble_vie_table(bfa_surrogate *s, double omega, const bfa_material *mat, void *d_M,
+                            int64_t ld, void *stream);
+/* ensemble size, parameters per member, training samples; stats[4*13+2*32]
+ * = input mean, sd, lower, upper bound; target mean, sd.                   */
+int  bfa_surr_info(const bfa_surrogate *s, int *ensemble, int64_t *nparam, int *nsamples, float *stats);
+/* Get (out) and/or set (in) the FP32 parameters [ensemble][nparam].        */
+int  bfa_surr_params(bfa_surrogate *s, float *out, const float *in);
+
 /* ---- instrumentation (no reference counterpart) ------------------------ */
 /* When enabled, every trailing-update GEMM launch is bracketed by HIP
  * events; bfa_profile_read returns the summed kernel time (ms), the summed
  * algorithmic flops (8mnk per complex product) and the launch count for
- * which = 0 (complex128 kernel) or 1 (complex64 kernel), then resets.      */
+ * which = 0 (complex128) or 1 (complex64), all launches, or 2 / 3 for only
+ * the 128x128-tile kernel of that type; reading 0 / 1 resets that type.  */
 int  bfa_profile_enable(int on);
 int  bfa_profile_read(int which, double *ms, double *flops, int *launches);
 

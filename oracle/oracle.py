@@ -135,6 +135,25 @@ def tp_block(VS, VT, k, npts=4, level=0):
     return (out[0::2] + 1j * out[1::2]).reshape(4, 4)
 
 
+def tp_features(VS, VT, omega):
+    """Canonical surrogate inputs of a tetrahedron pair: the 4 target vertices
+    in the source frame (origin = source centroid, e1 along v1 - v0, e2 in the
+    v0 v1 v2 plane, e3 = e1 x e2) over the source's mean edge length h, then
+    log(omega h)."""
+    VS = np.asarray(VS, float).reshape(4, 3)
+    VT = np.asarray(VT, float).reshape(4, 3)
+    c = VS.mean(0)
+    h = np.mean([np.linalg.norm(VS[j] - VS[i]) for i in range(4) for j in range(i + 1, 4)])
+    e1 = (VS[1] - VS[0]) / np.linalg.norm(VS[1] - VS[0])
+    u = VS[2] - VS[0]
+    e2 = u - (u @ e1) * e1
+    e2 /= np.linalg.norm(e2)
+    e3 = np.cross(e1, e2)
+    R = np.stack([e1, e2, e3])
+    f = ((VT - c) @ R.T / h).ravel()
+    return np.concatenate([f, [np.log(omega * h)]])
+
+
 class Geometry:
     def __init__(self, V, T):
         self.V = np.ascontiguousarray(V, float)
