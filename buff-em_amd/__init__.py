@@ -63,6 +63,7 @@ _SIGS = {
     "bfa_static_download": (I, [P, P]),
     "bfa_assemble_vie": (I, [P, D, D, P, P, I64, P]),
     "bfa_assemble_rhs": (I, [P, D, P, I, P, I64, P]),
+    "bfa_assemble_vie_far32": (I, [P, D, D, P, P, I64, P]),
     "bfa_lu_factor_z": (I, [P, I, I64, P, P, P]),
     "bfa_lu_solve_z": (I, [P, I, I64, P, P, I, I64, P]),
     "bfa_emt_power": (I, [P, D, P, I, P, I64, P, P]),
@@ -273,10 +274,12 @@ class Geometry:
         _check(lib().bfa_static_download(self.h, t.ctypes.data), "bfa_static_download")
         return t
 
-    def assemble_vie(self, omega, mat, d_M, ld=None, stream=None):
+    def assemble_vie(self, omega, mat, d_M, ld=None, stream=None, far32=False):
+        """VIE matrix at omega; far32 evaluates the far-field cubature in FP32."""
         omega = complex(omega)
-        _check(lib().bfa_assemble_vie(self.h, omega.real, omega.imag, ctypes.byref(mat), _ptr(d_M),
-                                      int(ld or self.N), _stream_ptr(stream)), "bfa_assemble_vie")
+        fn = lib().bfa_assemble_vie_far32 if far32 else lib().bfa_assemble_vie
+        _check(fn(self.h, omega.real, omega.imag, ctypes.byref(mat), _ptr(d_M), int(ld or self.N),
+                  _stream_ptr(stream)), "bfa_assemble_vie")
 
     def assemble_rhs(self, omega, pws, d_B, ldb=None, stream=None):
         _check(lib().bfa_assemble_rhs(self.h, float(omega), pws, len(pws), _ptr(d_B), int(ldb or self.N),
@@ -480,7 +483,7 @@ class Surrogate:
     def features(self, pair, omega):
         pair = np.ascontiguousarray(pair, np.int32)
         omega = np.ascontiguousarray(np.broadcast_to(omega, pair.shape), np.float64)
-        out = np.zeros((len(pair), 13), np.float32)
+        out = np.zeros((len(pair), 19), np.float32)
         _check(lib().bfa_surr_features(self._h, pair.ctypes.data, omega.ctypes.data, len(pair), out.ctypes.data),
                "bfa_surr_features")
         return out

@@ -661,6 +661,81 @@ __global__ __launch_bounds__(256) void k_far(DevGeom g, double kre, double kim, 
   far_tile<REALK>(g, kre, kim, M, ld, ti, tj, cpts, cverts);
 }
 
+// FP32 variant of the far-field tile (the precision form's far field):
+// identical rule and pairing, cubature arithmetic and accumulation in FP32,
+// the sum stored as complex128.
+template <bool REALK>
+__global__ __launch_bounds__(256) void k_far32(DevGeom g, double kre_d, double kim_d, cd *M, int64_t ld, int ntiles)
+{
+  __shared__ float cpts[TILE][8][7];
+  __shared__ int cverts[TILE][5];
+  int ti, tj;
+  tile_coords(blockIdx.x, ntiles, ti, tj);
+  const int tid = threadIdx.x, N = g.nbf;
+  for (int idx = tid; idx < TILE * 8; idx += blockDim.x) {
+    const int bl = idx >> 3, pt = idx & 7, b = tj * TILE + bl;
+    if (b < N) {
+      double o[7];
+      bf_points4(g, b, pt, o);
+      for (int c = 0; c < 7; c++) cpts[bl][pt][c] = (float)o[c];
+    }
+  }
+  for (int idx = tid; idx < TILE * 5; idx += blockDim.x) {
+    const int bl = idx / 5, b = tj * TILE + bl;
+    cverts[bl][idx % 5] = b < N ? g.bfV[5 * b + idx % 5] : -1;
+  }
+  __syncthreads();
+  const int a = ti * TILE + (tid & 63);
+  if (a >= N) return;
+  float rp[8][7];
+#pragma unroll
+  for (int pt = 0; pt < 8; pt++) {
+    double o[7];
+    bf_points4(g, a, pt, o);
+    for (int c = 0; c < 7; c++) rp[pt][c] = (float)o[c];
+  }
+  int va[5];
+#pragma unroll
+  for (int i = 0; i < 5; i++) va[i] = g.bfV[5 * a + i];
+  const float kre = (float)kre_d, kim = (float)kim_d;
+  const double k2re_d = kre_d * kre_d - kim_d * kim_d, k2im_d = 2.0 * kre_d * kim_d;
+  const double ik2d = k2re_d * k2re_d + k2im_d * k2im_d;
+  const float ik2re = (float)(k2re_d / ik2d), ik2im = (float)(-k2im_d / ik2d);
+  const int b0 = (tid >> 6) * 16;
+  for (int jj = 0; jj < 16; jj++) {
+    const int bl = b0 + jj, b = tj * TILE + bl;
+    if (b >= N || b < a) continue;
+    if (bf_ncv(va, cverts[bl]) > 0) continue;
+    float accr = 0.f, acci = 0.f;
+#pragma unroll 2
+    for (int ip = 0; ip < 8; ip++) {
+#pragma unroll
+      for (int jp = 0; jp < 8; jp++) {
+        const float *c = cpts[bl][jp];
+        const float R0 = rp[ip][0] - c[0], R1 = rp[ip][1] - c[1], R2 = rp[ip][2] - c[2];
+        const float r = sqrtf(R0 * R0 + R1 * R1 + R2 * R2);
+        const float inv = 1.0f / r;
+        const float dot = rp[ip][3] * c[3] + rp[ip][4] * c[4] + rp[ip][5] * c[5];
+        const float dd = rp[ip][6] * c[6];
+        float sn, co;
+        sincosf(kre * r, &sn, &co);
+        if (REALK) {
+          const float pe = (dot - dd * ik2re) * inv;
+          accr += pe * co; acci += pe * sn;
+        } else {
+          const float amp = expf(-kim * r) * inv;
+          const float per = dot - dd * ik2re, pei = -dd * ik2im;
+          const float er = amp * co, ei = amp * sn;
+          accr += per * er - pei * ei; acci += per * ei + pei * er;
+        }
+      }
+    }
+    const cd v = {(double)accr / FOURPI, (double)acci / FOURPI};
+    M[(size_t)b * ld + a] = v;
+    M[(size_t)a * ld + b] = v;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // near field + static record + Gram, one thread per touching pair
 __device__ __forceinline__ cd exprel3(cd x)
@@ -793,8 +868,8 @@ __global__ __launch_bounds__(256) void k_near(DevGeom g, double kre, double kim,
   if (j < g.nnz) near_entry(g, kre, kim, table, mat, M, ld, j);
 }
 
-extern "C" int bfa_assemble_vie(bfa_geometry *g, double omega_re, double omega_im, const bfa_material *mat,
-                                void *d_M, int64_t ld, void *stream)
+static int assemble_vie(bfa_geometry *g, double omega_re, double omega_im, const bfa_material *mat, void *d_M,
+                        int64_t ld, bool far32, void *stream)
 {
   if (!g || !mat || !d_M) return fail(BFA_ERR_ARG, "bfa_assemble_vie: null argument");
   if (!g->uploaded || !g->static_built)
@@ -806,16 +881,35 @@ extern "C" int bfa_assemble_vie(bfa_geometry *g, double omega_re, double omega_i
   const int ntiles = (N + TILE - 1) / TILE;
   const int64_t nblk = (int64_t)ntiles * (ntiles + 1) / 2;
   cd *M = (cd *)d_M;
-  if (omega_im == 0.0)
+  if (far32) {
+    if (omega_im == 0.0)
+      hipLaunchKernelGGL(k_far32<true>, dim3((unsigned)nblk), dim3(256), 0, st, g->dg, omega_re, 0.0, M, ld, ntiles);
+    else
+      hipLaunchKernelGGL(k_far32<false>, dim3((unsigned)nblk), dim3(256), 0, st, g->dg, omega_re, omega_im, M, ld,
+                         ntiles);
+  } else if (omega_im == 0.0) {
     hipLaunchKernelGGL(k_far<true>, dim3((unsigned)nblk), dim3(256), 0, st, g->dg, omega_re, 0.0, M, ld, ntiles);
-  else
+  } else {
     hipLaunchKernelGGL(k_far<false>, dim3((unsigned)nblk), dim3(256), 0, st, g->dg, omega_re, omega_im, M, ld,
                        ntiles);
+  }
   const int64_t nnz = g->dg.nnz;
   hipLaunchKernelGGL(k_near, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st, g->dg, omega_re, omega_im,
                      (const double *)g->d.table, *mat, M, ld);
   HIPCHECK(hipGetLastError());
   return BFA_OK;
+}
+
+extern "C" int bfa_assemble_vie(bfa_geometry *g, double omega_re, double omega_im, const bfa_material *mat,
+                                void *d_M, int64_t ld, void *stream)
+{
+  return assemble_vie(g, omega_re, omega_im, mat, d_M, ld, false, stream);
+}
+
+extern "C" int bfa_assemble_vie_far32(bfa_geometry *g, double omega_re, double omega_im, const bfa_material *mat,
+                                      void *d_M, int64_t ld, void *stream)
+{
+  return assemble_vie(g, omega_re, omega_im, mat, d_M, ld, true, stream);
 }
 
 int bfa::assemble_near_only(bfa_geometry *g, double omega_re, double omega_im, const bfa_material *mat, void *d_M,

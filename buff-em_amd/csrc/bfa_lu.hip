@@ -472,28 +472,40 @@ __device__ __forceinline__ void grid_barrier(unsigned *bar, unsigned target, int
 
 template <typename T> struct PanelShared {
   cplx<T> prow[PW], rj[PW];
-  T sval[256];
-  int srow[256], swin[256];
-  int piv_s, win_s;
-  T pval_s;
+  T wval[4];
+  int wrow[4], wwin[4];
 };
+
+// first index of the maximum: (v, r) wins over (ov, orr) unless ov is larger
+// or equal with a smaller row
+template <typename T> __device__ __forceinline__ void argmax_merge(T &v, int &r, int &w, T ov, int orr, int ow)
+{
+  if (ov > v || (ov == v && orr < r)) { v = ov; r = orr; w = ow; }
+}
+template <typename T> __device__ __forceinline__ void wave_argmax(T &v, int &r, int &w)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const T ov = __shfl_xor(v, off, 64);
+    const int orr = __shfl_xor(r, off, 64), ow = __shfl_xor(w, off, 64);
+    argmax_merge(v, r, w, ov, orr, ow);
+  }
+}
 
 // Body shared by the single-matrix and the grouped panel kernels: workgroup
 // g of the nwg that factor this panel; rows = the workgroup's LDS block.
+// Per column: the workgroup's candidate (found while updating the previous
+// column) is reduced across waves, published with its row data, one grid
+// barrier, every workgroup reduces the slots to the same pivot, then each
+// thread swaps/scales/updates its own rows and tracks the next column's
+// candidate.  Five workgroup barriers per column.
 template <typename T>
 __device__ __forceinline__ void panel_body(cplx<T> *A, int m, int w, int64_t lda, int *ipiv, int rpw,
                                            const PanelWork &pw, int piv_off, int g, int nwg, cplx<T> *rows,
                                            PanelShared<T> &sh)
 {
 #define RW(r, c) rows[(size_t)(c) * rpw + (r)]
-  cplx<T>(&prow)[PW] = sh.prow;
-  cplx<T>(&rj)[PW] = sh.rj;
-  T(&sval)[256] = sh.sval;
-  int(&srow)[256] = sh.srow;
-  int(&swin)[256] = sh.swin;
-  int &piv_s = sh.piv_s, &win_s = sh.win_s;
-  T &pval_s = sh.pval_s;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
   const int r0 = g * rpw, r1 = min(m, r0 + rpw), nr = max(0, r1 - r0);
   cplx<T> *rdata = (cplx<T> *)pw.rowdata;
   cplx<T> *rowjg = (cplx<T> *)pw.rowj;
@@ -502,87 +514,81 @@ __device__ __forceinline__ void panel_body(cplx<T> *A, int m, int w, int64_t lda
     RW(r, c) = A[(size_t)c * lda + r0 + r];
   }
   __syncthreads();
+  // this thread's candidate for column 0
+  T best = -1;
+  int brow = INT_MAX;
+  for (int r = tid; r < nr; r += blockDim.x) {
+    int dummy = 0;
+    argmax_merge(best, brow, dummy, c_abs1(RW(r, 0)), r0 + r, 0);
+  }
   for (int j = 0; j < w; j++) {
     const int buf = j & 1;
-    // local argmax over rows >= j (first index of the maximum |re|+|im|)
-    T best = -1;
-    int brow = INT_MAX;
-    for (int r = tid; r < nr; r += blockDim.x) {
-      const int gr = r0 + r;
-      if (gr < j) continue;
-      const T v = c_abs1(RW(r, j));
-      if (v > best || (v == best && gr < brow)) { best = v; brow = gr; }
+    {
+      int wdummy = 0;
+      wave_argmax(best, brow, wdummy);
+      if (lane == 0) { sh.wval[wave] = best; sh.wrow[wave] = brow; }
     }
-    sval[tid] = best; srow[tid] = brow;
     __syncthreads();
-    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
-      if (tid < off) {
-        const T v = sval[tid + off];
-        const int rr = srow[tid + off];
-        if (v > sval[tid] || (v == sval[tid] && rr < srow[tid])) { sval[tid] = v; srow[tid] = rr; }
-      }
-      __syncthreads();
-    }
-    const int lrow = srow[0];
-    if (tid == 0) { pw.val[buf * nwg + g] = (double)sval[0]; pw.row[buf * nwg + g] = lrow; }
+    T lbest = sh.wval[0];
+    int lrow = sh.wrow[0], ld = 0;
+    for (int q = 1; q < nwave; q++) argmax_merge(lbest, lrow, ld, sh.wval[q], sh.wrow[q], q);
+    if (tid == 0) { pw.val[buf * nwg + g] = (double)lbest; pw.row[buf * nwg + g] = lrow; }
     if (lrow != INT_MAX && tid < w) rdata[((size_t)buf * nwg + g) * PW + tid] = RW(lrow - r0, tid);
     if (j >= r0 && j < r1 && tid < w) rowjg[buf * PW + tid] = RW(j - r0, tid);
     grid_barrier(pw.bar, (unsigned)((j + 1) * nwg), pw.flags);
-    // global pivot (same result in every workgroup): parallel scan of the
-    // workgroup slots, first index of the maximum
+    // global pivot (same in every workgroup): first index of the maximum
     {
       T bv = -1;
       int br = INT_MAX, bw = -1;
       for (int q = tid; q < nwg; q += blockDim.x) {
         const int rr = pw.row[buf * nwg + q];
         if (rr == INT_MAX) continue;
-        const T v = (T)pw.val[buf * nwg + q];
-        if (v > bv || (v == bv && rr < br)) { bv = v; br = rr; bw = q; }
+        argmax_merge(bv, br, bw, (T)pw.val[buf * nwg + q], rr, q);
       }
-      sval[tid] = bv; srow[tid] = br; swin[tid] = bw;
-      __syncthreads();
-      for (int off = blockDim.x / 2; off > 0; off >>= 1) {
-        if (tid < off) {
-          const T v = sval[tid + off];
-          const int rr = srow[tid + off];
-          if (v > sval[tid] || (v == sval[tid] && rr < srow[tid])) {
-            sval[tid] = v; srow[tid] = rr; swin[tid] = swin[tid + off];
-          }
-        }
-        __syncthreads();
-      }
-      if (tid == 0) { piv_s = srow[0]; win_s = swin[0]; pval_s = sval[0]; }
+      wave_argmax(bv, br, bw);
+      if (lane == 0) { sh.wval[wave] = bv; sh.wrow[wave] = br; sh.wwin[wave] = bw; }
     }
     __syncthreads();
-    const int p = piv_s, win = win_s;
+    T pval = sh.wval[0];
+    int p = sh.wrow[0], win = sh.wwin[0];
+    for (int q = 1; q < nwave; q++) argmax_merge(pval, p, win, sh.wval[q], sh.wrow[q], sh.wwin[q]);
     if (tid < w) {
-      prow[tid] = rdata[((size_t)buf * nwg + win) * PW + tid];
-      rj[tid] = rowjg[buf * PW + tid];
+      sh.prow[tid] = rdata[((size_t)buf * nwg + win) * PW + tid];
+      sh.rj[tid] = rowjg[buf * PW + tid];
     }
-    __syncthreads();
     if (g == 0 && tid == 0) {
       ipiv[j] = p + piv_off;
-      if (pval_s == (T)0 && pw.flags[1] == 0) pw.flags[1] = piv_off + j + 1;
-    }
-    // interchange rows j and p in LDS
-    if (p != j) {
-      if (j >= r0 && j < r1 && tid < w) RW(j - r0, tid) = prow[tid];
-      if (p >= r0 && p < r1 && tid < w) RW(p - r0, tid) = rj[tid];
+      if (pval == (T)0 && pw.flags[1] == 0) pw.flags[1] = piv_off + j + 1;
     }
     __syncthreads();
-    // scale column j below the diagonal and update the rest of the panel
-    if (pval_s != (T)0) {
-      const cplx<T> inv = c_inv(prow[j]);
-      for (int r = tid; r < nr; r += blockDim.x) {
-        const int gr = r0 + r;
-        if (gr <= j) continue;
+    // swap rows j and p, scale column j below the diagonal, update the rest;
+    // each row belongs to one thread, which also tracks column j+1's candidate
+    best = -1;
+    brow = INT_MAX;
+    const bool scale = pval != (T)0;
+    const cplx<T> inv = scale ? c_inv(sh.prow[j]) : cplx<T>{0, 0};
+    for (int r = tid; r < nr; r += blockDim.x) {
+      const int gr = r0 + r;
+      if (gr < j) continue;
+      if (gr == j) {
+        if (p != j)
+          for (int c = 0; c < w; c++) RW(r, c) = sh.prow[c];
+        continue;
+      }
+      if (gr == p)
+        for (int c = 0; c < w; c++) RW(r, c) = sh.rj[c];
+      if (scale) {
         const cplx<T> l = c_mul(RW(r, j), inv);
         RW(r, j) = l;
-        for (int c = j + 1; c < w; c++) RW(r, c) = c_sub(RW(r, c), c_mul(l, prow[c]));
+        for (int c = j + 1; c < w; c++) RW(r, c) = c_sub(RW(r, c), c_mul(l, sh.prow[c]));
+      }
+      if (j + 1 < w) {
+        int dummy = 0;
+        argmax_merge(best, brow, dummy, c_abs1(RW(r, j + 1)), gr, 0);
       }
     }
-    __syncthreads();
   }
+  __syncthreads();
   for (int idx = tid; idx < nr * w; idx += blockDim.x) {
     const int r = idx % nr, c = idx / nr;
     A[(size_t)c * lda + r0 + r] = RW(r, c);
@@ -688,6 +694,11 @@ struct LUWork {
   PanelWork pw{};
   void *mem = nullptr;
   int num_cu = 0;
+  unsigned *svflags = nullptr;   // [2][maxsv] block flags of the one-launch triangular solves
+  int maxsv = 0;
+  unsigned epoch = 0;
+  hipStream_t side = nullptr;    // bulk trailing updates of the look-ahead LU
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
 };
 static LUWork g_lu[16];
 
@@ -702,15 +713,19 @@ static int lu_work(LUWork *&w)
   HIPCHECK(hipGetDeviceProperties(&prop, dev));
   w->num_cu = prop.multiProcessorCount;
   w->maxwg = 1024;
-  const size_t bytes = 2 * w->maxwg * 8 + 2 * w->maxwg * 4 + 2 * (size_t)w->maxwg * PW * 16 + 2 * PW * 16 + 64 + 64;
+  w->maxsv = 1 << 16;   // 64-row blocks: n up to 4.2 M
+  const size_t bytes = 2 * w->maxwg * 8 + 2 * w->maxwg * 4 + 2 * (size_t)w->maxwg * PW * 16 + 2 * PW * 16 + 64 + 64 +
+                       (size_t)2 * w->maxsv * 4;
   HIPCHECK(hipMalloc(&w->mem, bytes));
+  HIPCHECK(hipMemset(w->mem, 0, bytes));
   char *p = (char *)w->mem;
   w->pw.val = (double *)p; p += 2 * w->maxwg * 8;
   w->pw.row = (int *)p; p += 2 * w->maxwg * 4;
   w->pw.rowdata = p; p += 2 * (size_t)w->maxwg * PW * 16;
   w->pw.rowj = p; p += 2 * PW * 16;
   w->pw.bar = (unsigned *)p; p += 64;
-  w->pw.flags = (int *)p;
+  w->pw.flags = (int *)p; p += 64;
+  w->svflags = (unsigned *)p;
   w->device = dev;
   return BFA_OK;
 }
@@ -780,28 +795,61 @@ static int panel_rec(cplx<T> *A, int m, int w, int64_t lda, int *ipiv, int piv_o
   return BFA_OK;
 }
 
+// Blocked right-looking LU with one block of look-ahead.  Stream st carries
+// the critical path: the update of the next block column, then its panel.
+// A second stream carries the bulk trailing update of the remaining columns
+// (row swaps, TRSM, GEMM) and the swaps left of each panel, so each panel
+// factorisation overlaps the previous step's big GEMM.  Column ranges
+// touched concurrently are disjoint: the panel owns its nb columns, the bulk
+// update the columns to its right; the left swaps of panel j+1 are queued
+// behind bulk update j on the same stream because both touch panel j's L.
 template <typename T>
 int lu_factor(cplx<T> *A, int n, int64_t lda, int *ipiv, int *info, hipStream_t st, int nb)
 {
   LUWork *lw = nullptr;
   int rc = lu_work(lw);
   if (rc) return rc;
+  if (!lw->side) {
+    HIPCHECK(hipStreamCreateWithFlags(&lw->side, hipStreamNonBlocking));
+    for (auto &e : lw->ev) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  hipStream_t sb = lw->side;
+  hipEvent_t ev_p = lw->ev[0], ev_b = lw->ev[1], ev_s = lw->ev[2];
   HIPCHECK(hipMemsetAsync(lw->pw.flags, 0, 8, st));
+  HIPCHECK(hipEventRecord(ev_s, st));
+  HIPCHECK(hipStreamWaitEvent(sb, ev_s, 0));
+  rc = panel_rec<T>(A, n, std::min(nb, n), lda, ipiv, 0, lw, st);
+  if (rc) return rc;
   for (int j = 0; j < n; j += nb) {
     const int jb = std::min(nb, n - j);
     cplx<T> *Ajj = A + (size_t)j * lda + j;
-    // pivots of the panel are produced relative to row j, stored absolute
-    rc = panel_rec<T>(Ajj, n - j, jb, lda, ipiv + j, j, lw, st);
-    if (rc) return rc;
-    // rows j..j+jb-1 of the columns left and right of the panel
-    laswp<T>(A, lda, 0, j, j, jb, ipiv, st);
-    laswp<T>(A, lda, j + jb, n, j, jb, ipiv, st);
-    if (j + jb < n) {
-      trsm<T, true>(jb, n - j - jb, Ajj, lda, Ajj + (size_t)jb * lda, lda, st);
-      gemm_sub<T>(n - j - jb, n - j - jb, jb, Ajj + jb, lda, Ajj + (size_t)jb * lda, lda,
-                  Ajj + (size_t)jb * lda + jb, lda, st);
+    const int next = j + jb;
+    HIPCHECK(hipEventRecord(ev_p, st));          // panel j done
+    HIPCHECK(hipStreamWaitEvent(sb, ev_p, 0));
+    laswp<T>(A, lda, 0, j, j, jb, ipiv, sb);    // rows of panel j, columns left of it
+    if (next >= n) break;
+    const int nb2 = std::min(nb, n - next);
+    cplx<T> *Anext = A + (size_t)next * lda;
+    // critical path: block column next, then its panel
+    laswp<T>(A, lda, next, next + nb2, j, jb, ipiv, st);
+    trsm<T, true>(jb, nb2, Ajj, lda, Anext + j, lda, st);
+    gemm_sub<T>(n - next, nb2, jb, Ajj + jb, lda, Anext + j, lda, Anext + next, lda, st);
+    // bulk: the columns right of block column next
+    const int rest = n - next - nb2;
+    if (rest > 0) {
+      cplx<T> *Arest = A + (size_t)(next + nb2) * lda;
+      laswp<T>(A, lda, next + nb2, n, j, jb, ipiv, sb);
+      trsm<T, true>(jb, rest, Ajj, lda, Arest + j, lda, sb);
+      gemm_sub<T>(n - next, rest, jb, Ajj + jb, lda, Arest + j, lda, Arest + next, lda, sb);
     }
+    HIPCHECK(hipEventRecord(ev_b, sb));
+    rc = panel_rec<T>(Anext + next, n - next, nb2, lda, ipiv + next, next, lw, st);
+    if (rc) return rc;
+    // the next block column was updated by this bulk pass
+    HIPCHECK(hipStreamWaitEvent(st, ev_b, 0));
   }
+  HIPCHECK(hipEventRecord(ev_b, sb));
+  HIPCHECK(hipStreamWaitEvent(st, ev_b, 0));
   HIPCHECK(hipGetLastError());
   int flags[2] = {0, 0};
   HIPCHECK(hipMemcpyAsync(flags, lw->pw.flags, 8, hipMemcpyDeviceToHost, st));
@@ -952,6 +1000,97 @@ __global__ __launch_bounds__(256) void kg_trsv_update(const TrsvProb *__restrict
                       q.ldy, blockIdx.x - q.blk0, sh);
 }
 
+// Triangular solve of all 64-row blocks in ONE launch ("sync-free"):
+// workgroup b owns block row b (dispatch order = dependency order: forward
+// for L, reversed for U).  Its 4 waves take the finished blocks k it depends
+// on round-robin: wait for flag[k] == epoch (bounded spin, acquire), then
+// accumulate L(rows of b, cols of k) * y_k; the partial sums are reduced,
+// the 64x64 diagonal block is solved in LDS, y_b is stored and flag[b] is
+// released.  A workgroup only waits on lower-numbered workgroups, which the
+// in-order dispatcher has already placed or retired, so the chain always
+// progresses.  L unit lower (LOWER) or U upper with its diagonal.
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void k_trsv_sf(int n, int nrhs, const cplx<T> *__restrict__ L, int64_t ldl,
+                                                 cplx<T> *X, int64_t ldx, unsigned *flags, unsigned epoch, int *err)
+{
+  __shared__ cplx<T> Ls[SV_KB][SV_KB + 1];
+  __shared__ cplx<T> Xs[SV_RHS][SV_KB];
+  __shared__ cplx<T> part[4][SV_RHS][64];
+  const int nb = (n + SV_KB - 1) / SV_KB;
+  const int bi = LOWER ? (int)blockIdx.x : nb - 1 - (int)blockIdx.x;
+  const int r0 = bi * SV_KB, kb = min(SV_KB, n - r0);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int idx = tid; idx < kb * kb; idx += blockDim.x) {
+    const int r = idx % kb, c = idx / kb;
+    Ls[r][c] = L[(size_t)(r0 + c) * ldl + r0 + r];
+  }
+  T ar[SV_RHS], ai[SV_RHS];
+#pragma unroll
+  for (int j = 0; j < SV_RHS; j++) { ar[j] = 0; ai[j] = 0; }
+  const int r = r0 + lane;
+  const int ndep = LOWER ? bi : nb - 1 - bi;
+  for (int q = wv; q < ndep; q += 4) {
+    const int k = LOWER ? q : bi + 1 + q;
+    if (lane == 0) {
+      unsigned spins = 0;
+      while (__hip_atomic_load(&flags[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+        if (++spins > SPIN_LIMIT) { atomicOr(err, 1); break; }
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const int c0 = k * SV_KB, kc = min(SV_KB, n - c0);
+    if (r < r0 + kb) {
+      for (int c = 0; c < kc; c++) {
+        const cplx<T> l = L[(size_t)(c0 + c) * ldl + r];
+#pragma unroll
+        for (int j = 0; j < SV_RHS; j++) {
+          if (j < nrhs) {
+            const cplx<T> y = X[(size_t)j * ldx + c0 + c];
+            ar[j] += l.re * y.re - l.im * y.im;
+            ai[j] += l.re * y.im + l.im * y.re;
+          }
+        }
+      }
+    }
+  }
+  for (int j = 0; j < nrhs; j++) part[wv][j][lane] = {ar[j], ai[j]};
+  __syncthreads();
+  for (int idx = tid; idx < kb * nrhs; idx += blockDim.x) {
+    const int rr = idx % kb, j = idx / kb;
+    const cplx<T> b = X[(size_t)j * ldx + r0 + rr];
+    const T sr = part[0][j][rr].re + part[1][j][rr].re + part[2][j][rr].re + part[3][j][rr].re;
+    const T si = part[0][j][rr].im + part[1][j][rr].im + part[2][j][rr].im + part[3][j][rr].im;
+    Xs[j][rr] = {b.re - sr, b.im - si};
+  }
+  __syncthreads();
+  const int rl = tid % SV_KB, jg = tid / SV_KB;
+  if (LOWER) {
+    for (int i = 0; i < kb - 1; i++) {
+      if (rl > i && rl < kb)
+        for (int j = jg; j < nrhs; j += 4) Xs[j][rl] = c_sub(Xs[j][rl], c_mul(Ls[rl][i], Xs[j][i]));
+      __syncthreads();
+    }
+  } else {
+    for (int i = kb - 1; i >= 0; i--) {
+      if (rl == i)
+        for (int j = jg; j < nrhs; j += 4) Xs[j][i] = c_mul(Xs[j][i], c_inv(Ls[i][i]));
+      __syncthreads();
+      if (rl < i)
+        for (int j = jg; j < nrhs; j += 4) Xs[j][rl] = c_sub(Xs[j][rl], c_mul(Ls[rl][i], Xs[j][i]));
+      __syncthreads();
+    }
+  }
+  for (int idx = tid; idx < kb * nrhs; idx += blockDim.x) {
+    const int rr = idx % kb, j = idx / kb;
+    X[(size_t)j * ldx + r0 + rr] = Xs[j][rr];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_store(&flags[bi], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 template <typename T>
 int lu_solve(const cplx<T> *A, int n, int64_t lda, const int *ipiv, cplx<T> *B, int nrhs, int64_t ldb, hipStream_t st,
              void *scratch)
@@ -972,33 +1111,35 @@ int lu_solve(const cplx<T> *A, int n, int64_t lda, const int *ipiv, cplx<T> *B, 
   }
   hipLaunchKernelGGL(k_gather_rows<T>, dim3((n + 255) / 256, nrhs), dim3(256), 0, st, B, ldb, X, (int64_t)n, n, nrhs,
                      (const int *)perm);
-  const int nb = SV_KB;
-  const int nblk = (n + nb - 1) / nb;
+  LUWork *lw = nullptr;
+  int rc = lu_work(lw);
+  if (rc) return rc;
+  const int nblk = (n + SV_KB - 1) / SV_KB;
+  if (nblk > lw->maxsv) return fail(BFA_ERR_ARG, "lu_solve: n = %d exceeds the solve flag pool", n);
   for (int j0 = 0; j0 < nrhs; j0 += SV_RHS) {
     const int nj = std::min(SV_RHS, nrhs - j0);
     cplx<T> *Xj = X + (size_t)j0 * n;
-    for (int bi = 0; bi < nblk; bi++) {              // L y = P b
-      const int k = bi * nb, kb = std::min(nb, n - k);
-      hipLaunchKernelGGL((k_trsv_diag<T, true>), dim3(1), dim3(256), 0, st, kb, nj, A + (size_t)k * lda + k, lda,
-                         Xj + k, (int64_t)n);
-      const int m = n - k - kb;
-      if (m > 0)
-        hipLaunchKernelGGL(k_trsv_update<T>, dim3((m + 63) / 64), dim3(256), 0, st, m, kb, nj,
-                           A + (size_t)k * lda + k + kb, lda, (const cplx<T> *)(Xj + k), (int64_t)n, Xj + k + kb,
-                           (int64_t)n);
-    }
-    for (int bi = nblk - 1; bi >= 0; bi--) {         // U x = y
-      const int k = bi * nb, kb = std::min(nb, n - k);
-      hipLaunchKernelGGL((k_trsv_diag<T, false>), dim3(1), dim3(256), 0, st, kb, nj, A + (size_t)k * lda + k, lda,
-                         Xj + k, (int64_t)n);
-      if (k > 0)
-        hipLaunchKernelGGL(k_trsv_update<T>, dim3((k + 63) / 64), dim3(256), 0, st, k, kb, nj, A + (size_t)k * lda,
-                           lda, (const cplx<T> *)(Xj + k), (int64_t)n, Xj, (int64_t)n);
-    }
+    unsigned e1 = ++lw->epoch, e2 = ++lw->epoch;
+    hipLaunchKernelGGL((k_trsv_sf<T, true>), dim3(nblk), dim3(256), 0, st, n, nj, A, lda, Xj, (int64_t)n,
+                       lw->svflags, e1, lw->pw.flags);
+    hipLaunchKernelGGL((k_trsv_sf<T, false>), dim3(nblk), dim3(256), 0, st, n, nj, A, lda, Xj, (int64_t)n,
+                       lw->svflags + lw->maxsv, e2, lw->pw.flags);
   }
   HIPCHECK(hipMemcpy2DAsync(B, ldb * sizeof(cplx<T>), X, n * sizeof(cplx<T>), n * sizeof(cplx<T>), nrhs,
                             hipMemcpyDeviceToDevice, st));
   HIPCHECK(hipGetLastError());
+  return BFA_OK;
+}
+
+// BFA_OK unless a one-launch triangular solve of this device gave up waiting
+int solve_status()
+{
+  LUWork *lw = nullptr;
+  int rc = lu_work(lw);
+  if (rc) return rc;
+  int f = 0;
+  HIPCHECK(hipMemcpy(&f, lw->pw.flags, 4, hipMemcpyDeviceToHost));
+  if (f) return fail(BFA_ERR_HIP, "triangular solve: a workgroup timed out waiting for its dependencies");
   return BFA_OK;
 }
 
@@ -1131,7 +1272,7 @@ extern "C" int bfa_lu_solve_z(const void *d_A, int n, int64_t lda, const int *d_
   hipFree(scratch);
   if (rc) return rc;
   if (e != hipSuccess) return fail(BFA_ERR_HIP, "bfa_lu_solve_z: %s", hipGetErrorString(e));
-  return BFA_OK;
+  return bfa::solve_status();
 }
 
 extern "C" int bfa_zgemm_sub(int m, int n, int k, const void *d_A, int64_t lda, const void *d_B, int64_t ldb,

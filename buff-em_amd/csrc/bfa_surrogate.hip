@@ -12,11 +12,12 @@
 //   rule       the reference's 4-point product rule,
 //   cubature   a product TetCR rule on regularly subdivided tetrahedra (the
 //              accurate integral: training labels, fallback, judge),
-//   surrogate  rule + the ensemble mean of an MLP that predicts the rule's
-//              error from the pair geometry canonicalised to the source
-//              frame and kr, with cubature wherever the members disagree by
-//              more than a tolerance or the input lies outside the training
-//              range.
+//   surrogate  the 16-point product rule R16 plus the ensemble mean of an
+//              MLP that predicts R16's remaining error in units of the
+//              per-pair error estimate ||R16 - R4||, from the pair geometry
+//              canonicalised to the source frame and kr; cubature wherever
+//              the members disagree by more than a tolerance or the input
+//              lies outside the training range.
 // The far-field assembly then reads the blocks instead of integrating.
 #include <hip/hip_runtime.h>
 #include <algorithm>
@@ -40,7 +41,7 @@ using namespace bfa;
   } while (0)
 
 #define FOURPI 12.566370614359172954
-#define NFEAT 13          // 12 canonical coordinates + log(k h)
+#define NFEAT 19          // 12 target coordinates + 6 source shape + log(k h)
 #define NOUT 32           // 16 complex block entries
 #define MAXPTS 33
 
@@ -226,9 +227,10 @@ __global__ __launch_bounds__(256) void k_tp_block(BlockJob J, int npts, int leve
 }
 
 // ---------------------------------------------------------------------------
-// Canonical features of (pair, omega): target vertices in the source frame
-// (origin = source centroid, e1 along v1-v0, e2 in the v0 v1 v2 plane,
-// e3 = e1 x e2), divided by the source's mean edge length h; and log(k h).
+// Canonical features of (pair, omega) in the source frame (origin = source
+// centroid, e1 along v1-v0, e2 in the v0 v1 v2 plane, e3 = e1 x e2), lengths
+// over the source's mean edge h: the 4 target vertices (12), the source's
+// own shape v1-v0 = (a,0,0), v2-v0 = (b,c,0), v3-v0 = (d,e,f) (6), log(k h).
 __device__ void d_features(const double *VS, const double *VT, double omega, float *f)
 {
   double c[3], e1[3], e2[3], e3[3], u[3];
@@ -257,7 +259,16 @@ __device__ void d_features(const double *VS, const double *VT, double omega, flo
     f[3 * j + 1] = (float)(d_dot(x, e2) / h);
     f[3 * j + 2] = (float)(d_dot(x, e3) / h);
   }
-  f[12] = (float)log(omega * h);
+  double v2[3], v3[3];
+  d_sub(VS + 6, VS, v2);
+  d_sub(VS + 9, VS, v3);
+  f[12] = (float)(n1 / h);
+  f[13] = (float)(d_dot(v2, e1) / h);
+  f[14] = (float)(d_dot(v2, e2) / h);
+  f[15] = (float)(d_dot(v3, e1) / h);
+  f[16] = (float)(d_dot(v3, e2) / h);
+  f[17] = (float)(d_dot(v3, e3) / h);
+  f[18] = (float)log(omega * h);
 }
 
 struct FeatJob {
@@ -294,15 +305,27 @@ __global__ void k_features(FeatJob J, float *feat, _Float16 *x16, int kp, const 
   }
 }
 
-// training target: y = (label - rule) / ||rule||_F, standardised later
-__global__ void k_targets(const double *label, const double *rule, int n, float *y)
+// Per-sample scale of the residual: s = ||R16 - R4|| (the 4-point rule's
+// error estimate), floored at 1e-14 ||R16||.
+__device__ __forceinline__ double resid_scale(const double *r4, const double *r16)
+{
+  double d = 0.0, n = 0.0;
+  for (int k = 0; k < NOUT; k++) {
+    const double x = r16[k] - r4[k];
+    d += x * x;
+    n += r16[k] * r16[k];
+  }
+  return fmax(sqrt(d), 1e-14 * sqrt(n));
+}
+
+// training target: y = (label - R16) / s, standardised later
+__global__ void k_targets(const double *label, const double *r4, const double *r16, int n, float *y)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double nr = 0.0;
-  for (int k = 0; k < NOUT; k++) nr += rule[(size_t)i * NOUT + k] * rule[(size_t)i * NOUT + k];
-  nr = sqrt(nr);
-  for (int k = 0; k < NOUT; k++) y[(size_t)i * NOUT + k] = (float)((label[(size_t)i * NOUT + k] - rule[(size_t)i * NOUT + k]) / nr);
+  const double sc = resid_scale(r4 + (size_t)i * NOUT, r16 + (size_t)i * NOUT);
+  for (int k = 0; k < NOUT; k++)
+    y[(size_t)i * NOUT + k] = (float)((label[(size_t)i * NOUT + k] - r16[(size_t)i * NOUT + k]) / sc);
 }
 
 __global__ void k_standardise(float *y, int n, int m, const float *mu, const float *sd)
@@ -324,27 +347,27 @@ __global__ void k_gather(const _Float16 *src16, int kp, const float *srcT, const
   for (int k = 0; k < NOUT; k++) T[((size_t)e * B + b) * NOUT + k] = srcT[(size_t)s * NOUT + k];
 }
 
-// Ensemble combination: blocks = rule + ||rule|| * mean(y); disagreement
-// = max_e ||y_e - mean|| / ||rule/||rule|| + mean||; flag if > tol or the
-// input lies outside the training range (features beyond [lo, hi]).
-__global__ void k_combine(const float *Y, int E, int n, const float *ymu, const float *ysd, const double *rule,
-                          const float *feat, const float *flo, const float *fhi, double tol, double *blocks,
-                          float *disagree, int *flag)
+// Ensemble combination: block = R16 + s * mean_e(y_e) (s = ||R16 - R4||);
+// disagreement = max_e s ||y_e - mean|| / ||block||; flag if > tol or an
+// input lies outside the training range [lo, hi].
+__global__ void k_combine(const float *Y, int E, int n, const float *ymu, const float *ysd, const double *r4,
+                          const double *r16, const float *feat, const float *flo, const float *fhi, double tol,
+                          double *blocks, float *disagree, int *flag)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const double *a = r4 + (size_t)i * NOUT, *b = r16 + (size_t)i * NOUT;
+  const double sc = resid_scale(a, b);
   double mean[NOUT];
   for (int k = 0; k < NOUT; k++) {
     double s = 0.0;
     for (int e = 0; e < E; e++) s += (double)Y[((size_t)e * n + i) * NOUT + k] * ysd[k] + ymu[k];
     mean[k] = s / E;
   }
-  double nr = 0.0;
-  for (int k = 0; k < NOUT; k++) nr += rule[(size_t)i * NOUT + k] * rule[(size_t)i * NOUT + k];
-  nr = sqrt(nr);
   double nb = 0.0;
   for (int k = 0; k < NOUT; k++) {
-    const double v = rule[(size_t)i * NOUT + k] / nr + mean[k];
+    const double v = b[k] + sc * mean[k];
+    blocks[(size_t)i * NOUT + k] = v;
     nb += v * v;
   }
   nb = sqrt(nb);
@@ -355,7 +378,7 @@ __global__ void k_combine(const float *Y, int E, int n, const float *ymu, const 
       const double d = (double)Y[((size_t)e * n + i) * NOUT + k] * ysd[k] + ymu[k] - mean[k];
       d2 += d * d;
     }
-    worst = fmax(worst, sqrt(d2) / nb);
+    worst = fmax(worst, sc * sqrt(d2) / nb);
   }
   bool ood = false;
   for (int k = 0; k < NFEAT; k++) {
@@ -364,7 +387,6 @@ __global__ void k_combine(const float *Y, int E, int n, const float *ymu, const 
   }
   disagree[i] = (float)worst;
   flag[i] = (worst > tol || ood) ? 1 : 0;
-  for (int k = 0; k < NOUT; k++) blocks[(size_t)i * NOUT + k] = rule[(size_t)i * NOUT + k] + nr * mean[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -489,7 +511,7 @@ struct bfa_surrogate {
   double *d_somega = nullptr;
   _Float16 *d_X16 = nullptr;         // [nsamp][kp]
   float *d_Y = nullptr;              // [nsamp][32] standardised targets
-  float *d_F = nullptr;              // [nsamp][13] raw features
+  float *d_F = nullptr;              // [nsamp][NFEAT] raw features
   // last prediction statistics
   int last_flagged = 0;
   double last_disagree_max = 0.0;
@@ -701,7 +723,7 @@ extern "C" int bfa_surr_cubature(bfa_surrogate *S, const int *pair, const double
   return rc;
 }
 
-// Canonical features (host out [n][13]) of pair[n] at omega[n].
+// Canonical features (host out [n][NFEAT]) of pair[n] at omega[n].
 extern "C" int bfa_surr_features(bfa_surrogate *S, const int *pair, const double *omega, int n, float *feat)
 {
   if (!S || !pair || !omega || !feat || n <= 0) return fail(BFA_ERR_ARG, "bfa_surr_features: bad argument");
@@ -750,15 +772,18 @@ extern "C" int bfa_surr_dataset(bfa_surrogate *S, int per_pair, double wmin, dou
   int rc;
   if ((rc = dev_copy(&S->d_spair, sp)) || (rc = dev_copy(&S->d_somega, sw))) return rc;
   const int kp = S->shp.kp;
-  double *rule = nullptr, *label = nullptr;
+  double *rule = nullptr, *r16 = nullptr, *label = nullptr;
   HIPCHECK(hipMalloc(&S->d_X16, (size_t)n * kp * 2));
   HIPCHECK(hipMalloc(&S->d_Y, (size_t)n * NOUT * 4));
   HIPCHECK(hipMalloc(&S->d_F, (size_t)n * NFEAT * 4));
   HIPCHECK(hipMalloc(&rule, (size_t)n * NOUT * 8));
+  HIPCHECK(hipMalloc(&r16, (size_t)n * NOUT * 8));
   HIPCHECK(hipMalloc(&label, (size_t)n * NOUT * 8));
   S->nsamp = n;
   auto go = [&]() -> int {
     int r = launch_blocks(S, S->d_spair, S->d_somega, 0.0, nullptr, n, 4, 0, rule, nullptr);
+    if (r) return r;
+    r = launch_blocks(S, S->d_spair, S->d_somega, 0.0, nullptr, n, 16, 0, r16, nullptr);
     if (r) return r;
     r = launch_blocks(S, S->d_spair, S->d_somega, 0.0, nullptr, n, label_npts, label_level, label, nullptr);
     if (r) return r;
@@ -767,7 +792,7 @@ extern "C" int bfa_surr_dataset(bfa_surrogate *S, int per_pair, double wmin, dou
     hipLaunchKernelGGL(k_features, dim3((n + 255) / 256), dim3(256), 0, nullptr, J, S->d_F, (_Float16 *)nullptr, 0,
                        (const float *)nullptr, (const float *)nullptr);
     hipLaunchKernelGGL(k_targets, dim3((n + 255) / 256), dim3(256), 0, nullptr, (const double *)label,
-                       (const double *)rule, n, S->d_Y);
+                       (const double *)rule, (const double *)r16, n, S->d_Y);
     HIPCHECK(hipGetLastError());
     // statistics on the host (one pass over the raw features and targets)
     std::vector<float> F((size_t)n * NFEAT), Y((size_t)n * NOUT);
@@ -808,6 +833,7 @@ extern "C" int bfa_surr_dataset(bfa_surrogate *S, int per_pair, double wmin, dou
   };
   rc = go();
   (void)hipFree(rule);
+  (void)hipFree(r16);
   (void)hipFree(label);
   return rc;
 }
@@ -905,13 +931,14 @@ extern "C" int bfa_surr_table(bfa_surrogate *S, double omega, int mode, double t
   const int E = S->E, kp = S->shp.kp;
   const int chunk = std::min(n, 65536);
   char *mem = nullptr;
-  const size_t bytes = (size_t)n * NOUT * 8 + (size_t)chunk * NFEAT * 4 + (size_t)chunk * kp * 2 +
+  const size_t bytes = (size_t)2 * n * NOUT * 8 + (size_t)chunk * NFEAT * 4 + (size_t)chunk * kp * 2 +
                        (size_t)E * chunk * NOUT * 4 + (size_t)E * mlp_work_halfs(S->shp, chunk) * 2 +
                        (size_t)n * 4 * 3 + 16 * 256;
   HIPCHECK(hipMalloc(&mem, bytes));
   size_t o = 0;
   auto take = [&](size_t b) { size_t r = (o + 255) / 256 * 256; o = r + b; return mem + r; };
   double *rule = (double *)take((size_t)n * NOUT * 8);
+  double *r16 = (double *)take((size_t)n * NOUT * 8);
   float *F = (float *)take((size_t)chunk * NFEAT * 4);
   _Float16 *X16 = (_Float16 *)take((size_t)chunk * kp * 2);
   float *Y = (float *)take((size_t)E * chunk * NOUT * 4);
@@ -920,6 +947,7 @@ extern "C" int bfa_surr_table(bfa_surrogate *S, double omega, int mode, double t
   int *flag = (int *)take((size_t)n * 4);
   int *list = (int *)take((size_t)n * 4);
   int rc = launch_blocks(S, nullptr, nullptr, omega, nullptr, n, 4, 0, rule, nullptr);
+  if (!rc) rc = launch_blocks(S, nullptr, nullptr, omega, nullptr, n, 16, 0, r16, nullptr);
   const float *mu = S->d_stats, *sd = S->d_stats + NFEAT, *lo = S->d_stats + 2 * NFEAT, *hi = S->d_stats + 3 * NFEAT;
   const float *ym = S->d_stats + 4 * NFEAT, *ys = ym + NOUT;
   for (int c0 = 0; c0 < n && !rc; c0 += chunk) {
@@ -930,8 +958,8 @@ extern "C" int bfa_surr_table(bfa_surrogate *S, double omega, int mode, double t
     hipLaunchKernelGGL(k_features, dim3((m + 255) / 256), dim3(256), 0, nullptr, J, F, X16, kp, mu, sd);
     if ((rc = mlp_forward(S->shp, E, m, X16, 0, S->W16, S->W, Y, nullptr, A, nullptr))) break;
     hipLaunchKernelGGL(k_combine, dim3((m + 255) / 256), dim3(256), 0, nullptr, (const float *)Y, E, m, ym, ys,
-                       (const double *)(rule + (size_t)c0 * NOUT), (const float *)F, lo, hi, tol,
-                       S->d_blocks + (size_t)c0 * NOUT, dis + c0, flag + c0);
+                       (const double *)(rule + (size_t)c0 * NOUT), (const double *)(r16 + (size_t)c0 * NOUT),
+                       (const float *)F, lo, hi, tol, S->d_blocks + (size_t)c0 * NOUT, dis + c0, flag + c0);
   }
   std::vector<int> hf(n);
   std::vector<float> hd(n);

@@ -136,10 +136,11 @@ def tp_block(VS, VT, k, npts=4, level=0):
 
 
 def tp_features(VS, VT, omega):
-    """Canonical surrogate inputs of a tetrahedron pair: the 4 target vertices
-    in the source frame (origin = source centroid, e1 along v1 - v0, e2 in the
-    v0 v1 v2 plane, e3 = e1 x e2) over the source's mean edge length h, then
-    log(omega h)."""
+    """Canonical surrogate inputs of a tetrahedron pair in the source frame
+    (origin = source centroid, e1 along v1 - v0, e2 in the v0 v1 v2 plane,
+    e3 = e1 x e2), lengths over the source's mean edge h: the 4 target
+    vertices (12), the source's shape v1-v0 = (a,0,0), v2-v0 = (b,c,0),
+    v3-v0 = (d,e,f) (6), then log(omega h)."""
     VS = np.asarray(VS, float).reshape(4, 3)
     VT = np.asarray(VT, float).reshape(4, 3)
     c = VS.mean(0)
@@ -151,7 +152,8 @@ def tp_features(VS, VT, omega):
     e3 = np.cross(e1, e2)
     R = np.stack([e1, e2, e3])
     f = ((VT - c) @ R.T / h).ravel()
-    return np.concatenate([f, [np.log(omega * h)]])
+    sh = (VS[1:] - VS[0]) @ R.T / h
+    return np.concatenate([f, [sh[0, 0], sh[1, 0], sh[1, 1], sh[2, 0], sh[2, 1], sh[2, 2]], [np.log(omega * h)]])
 
 
 class Geometry:

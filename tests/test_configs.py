@@ -1,0 +1,126 @@
+"""The reference workloads' small configurations end to end against the oracle.
+
+- n = 5 Kuhn sphere (750 tetrahedra, N = 1,350), eps = 4+0.1i, omega = 1,
+  x-polarised plane wave along +z: FP64 matrix to 1e-10 (relative Frobenius),
+  powers to 1e-9.
+- The precision form on the same mesh with eps(x) = 4 + 2 sin(pi x) cos(pi y)
+  + 0.1i and 32 right-hand sides (16 seeded directions uniform on the sphere
+  x 2 orthonormal polarisations): complex64 LU + FP64 refinement to 1e-10
+  residual; every power within 1e-8 of the FP64 oracle.  With the far field
+  integrated in FP32 the error is reported (it is set by the FP32 cubature,
+  not by the solver) and bounded loosely.
+- An exactly singular matrix reports the LAPACK-style first zero pivot.
+"""
+import numpy as np
+import pytest
+
+from conftest import kuhn
+
+pytestmark = pytest.mark.gpu
+
+
+def colmajor(t):
+    return t.cpu().numpy().T
+
+
+@pytest.fixture(scope="module")
+def mesh5(bfa, orc, torch_cuda):
+    V, T = kuhn(bfa, 5)
+    g = bfa.Geometry(V, T)
+    g.upload()
+    g.build_static(10)
+    o = orc.Geometry(V, T)
+    tab = o.static_table(10)
+    return g, o, tab
+
+
+def directions(nd, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(nd):
+        v = rng.standard_normal(3)
+        n = v / np.linalg.norm(v)
+        a = np.array([0.0, 0.0, 1.0]) if abs(n[2]) < 0.9 else np.array([1.0, 0.0, 0.0])
+        e1 = np.cross(n, a)
+        e1 /= np.linalg.norm(e1)
+        e2 = np.cross(n, e1)
+        out += [(tuple(e1), tuple(n)), (tuple(e2), tuple(n))]
+    return out
+
+
+def test_config1_fp64(bfa, orc, torch_cuda, mesh5):
+    torch = torch_cuda
+    g, o, tab = mesh5
+    N = g.N
+    print(f"[config1] N={N}", flush=True)
+    assert N == 1350
+    spec = [((1, 0, 0), (0, 0, 1))]
+    pw, opw = bfa.planewaves(spec), orc.planewaves(spec)
+    mat, omat = bfa.material("iso", eps=4 + 0.1j), orc.material("iso", eps=4 + 0.1j)
+    dM = torch.zeros((N, N), dtype=torch.complex128, device="cuda")
+    dB = torch.zeros((1, N), dtype=torch.complex128, device="cuda")
+    dp = torch.zeros(N, dtype=torch.int32, device="cuda")
+    g.assemble_vie(1.0, mat, dM)
+    g.assemble_rhs(1.0, pw, dB)
+    Mo = o.assemble(1.0, omat, tab)
+    err = np.linalg.norm(colmajor(dM) - Mo) / np.linalg.norm(Mo)
+    print("matrix rel err", err)
+    assert err < 1e-10
+    assert bfa.lu_factor(dM, N, dp) == 0
+    bfa.lu_solve(dM, N, dp, dB, 1)
+    Jo = orc.lu_solve(Mo, o.rhs(1.0, opw))
+    Po = o.emt_power(1.0, opw, Jo)
+    Pg = g.emt_power(1.0, pw, dB)
+    print("power gpu", Pg, "oracle", Po)
+    assert np.all(np.abs(Pg - Po) / np.abs(Po) < 1e-9)
+    # absorbed and scattered power are positive for a lossy dielectric
+    assert (Po > 0).all()
+
+
+def test_config3_precision_form(bfa, orc, torch_cuda, mesh5):
+    torch = torch_cuda
+    g, o, tab = mesh5
+    N = g.N
+    spec = directions(16, 0xC0F3)
+    assert len(spec) == 32
+    pw, opw = bfa.planewaves(spec), orc.planewaves(spec)
+    kw = dict(base=4.0, amp=2.0, imag=0.1)
+    mat, omat = bfa.material("sincos", **kw), orc.material("sincos", **kw)
+    Mo = o.assemble(1.0, omat, tab)
+    Jo = orc.lu_solve(Mo, o.rhs(1.0, opw))
+    Po = o.emt_power(1.0, opw, Jo)
+    for far32 in (False, True):
+        print(f"[config3] far32={far32}", flush=True)
+        dM = torch.zeros((N, N), dtype=torch.complex128, device="cuda")
+        dM32 = torch.zeros((N, N), dtype=torch.complex64, device="cuda")
+        dB = torch.zeros((32, N), dtype=torch.complex128, device="cuda")
+        dp = torch.zeros(N, dtype=torch.int32, device="cuda")
+        g.assemble_vie(1.0, mat, dM, far32=far32)
+        g.assemble_rhs(1.0, pw, dB)
+        merr = np.linalg.norm(colmajor(dM) - Mo) / np.linalg.norm(Mo)
+        bfa.convert_z2c(dM, N, N, dM32)
+        assert bfa.lu_factor_c(dM32, N, dp) == 0
+        it, res = bfa.solve_refine(dM, dM32, dp, N, dB, 32, tol=1e-10, maxit=12)
+        assert res <= 1e-10
+        Pg = g.emt_power(1.0, pw, dB)
+        perr = np.abs(Pg - Po) / np.abs(Po)
+        print(f"  matrix rel err {merr:.2e}; refinement {it} iters, residual {res:.2e}; "
+              f"power rel err max {perr.max():.2e} median {np.median(perr):.2e}", flush=True)
+        if far32:
+            assert merr < 1e-5 and perr.max() < 1e-4
+        else:
+            assert merr < 1e-10 and perr.max() < 1e-8
+
+
+def test_lu_singular_reports_first_zero_pivot(bfa, torch_cuda):
+    torch = torch_cuda
+    rng = np.random.default_rng(3)
+    for n, zc in ((40, 3), (600, 333)):
+        A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        A[:, zc] = 0.0
+        for f64 in (True, False):
+            dt = torch.complex128 if f64 else torch.complex64
+            dA = torch.from_numpy(np.ascontiguousarray(A.T)).to(dt).cuda()
+            dp = torch.zeros(n, dtype=torch.int32, device="cuda")
+            info = bfa.lu_factor(dA, n, dp) if f64 else bfa.lu_factor_c(dA, n, dp)
+            assert info == zc + 1, (n, f64, info)

@@ -80,6 +80,11 @@ def test_features_oracle_invariance(orc):
     sh, sc = rng.standard_normal(3), 2.5
     f1 = orc.tp_features(sc * VS @ Q.T + sh, sc * VT @ Q.T + sh, 1.3 / sc)
     np.testing.assert_allclose(f1, f0, atol=1e-12)
+    # source shape block: v1 - v0 on e1, v2 - v0 in the e1 e2 plane
+    h = np.mean([np.linalg.norm(VS[j] - VS[i]) for i in range(4) for j in range(i + 1, 4)])
+    assert abs(f0[12] - np.linalg.norm(VS[1] - VS[0]) / h) < 1e-12 and f0[17] > 0
+    assert abs(np.hypot(f0[13], f0[14]) - np.linalg.norm(VS[2] - VS[0]) / h) < 1e-12
+    assert abs(np.linalg.norm(f0[15:18]) - np.linalg.norm(VS[3] - VS[0]) / h) < 1e-12
     assert abs(np.linalg.norm(f0[:12].reshape(4, 3), axis=1) -
                np.linalg.norm(VT - VS.mean(0), axis=1) /
                np.mean([np.linalg.norm(VS[j] - VS[i]) for i in range(4) for j in range(i + 1, 4)])).max() < 1e-12
@@ -153,27 +158,32 @@ def test_surrogate_gain(bfa, n5, torch_cuda):
     t0 = time.time()
     S.dataset(per_pair=2, wmin=0.1, wmax=10.0, label_npts=33, label_level=1)
     t1 = time.time()
-    hist = S.train(epochs=12, batch=4096, lr=2e-3)
+    hist = S.train(epochs=40, batch=4096, lr=2e-3)
     t2 = time.time()
     print(f"[surrogate] dataset {t1 - t0:.1f}s train {t2 - t1:.1f}s loss {hist[0].mean():.3e} -> {hist[-1].mean():.3e}",
           flush=True)
-    assert hist[-1].mean() < 0.25 * hist[0].mean()
+    assert hist[-1].mean() < 0.3 * hist[0].mean()
     w = 1.0
     rng = np.random.default_rng(8)
     sub = rng.choice(S.npairs, 3000, replace=False)
     truth = S.cubature(sub, w, 33, 2)
     S.table(w, mode=S.RULE)
     rule = S.table_download()[sub]
+    S.table(w, mode=S.CUBATURE, npts=16, level=0)
+    r16 = S.table_download()[sub]
     nf, dmax = S.table(w, mode=S.SURROGATE, tol=np.inf)
     surr = S.table_download()[sub]
     err = lambda X: np.linalg.norm((X - truth).reshape(len(sub), -1), axis=1) / \
         np.linalg.norm(truth.reshape(len(sub), -1), axis=1)
-    e_rule, e_surr = err(rule), err(surr)
-    print(f"[surrogate] block error vs cubature(33,L2) at omega=1: rule median {np.median(e_rule):.2e} "
-          f"p90 {np.quantile(e_rule, .9):.2e}; surrogate median {np.median(e_surr):.2e} "
-          f"p90 {np.quantile(e_surr, .9):.2e}; max ensemble disagreement {dmax:.2e}", flush=True)
-    assert np.median(e_surr) < 0.2 * np.median(e_rule)
-    assert np.quantile(e_surr, 0.9) < 0.5 * np.quantile(e_rule, 0.9)
+    e_rule, e_16, e_surr = err(rule), err(r16), err(surr)
+    print(f"[surrogate] block error vs cubature(33,L2) at omega=1 (median / p90): 4-point rule "
+          f"{np.median(e_rule):.2e} / {np.quantile(e_rule, .9):.2e}; 16-point rule {np.median(e_16):.2e} / "
+          f"{np.quantile(e_16, .9):.2e}; surrogate {np.median(e_surr):.2e} / {np.quantile(e_surr, .9):.2e}; "
+          f"max ensemble disagreement {dmax:.2e}", flush=True)
+    # what the network adds over the rule it corrects, and over the reference's rule
+    assert np.median(e_surr) < 0.5 * np.median(e_16)
+    assert np.quantile(e_surr, 0.9) < 0.5 * np.quantile(e_16, 0.9)
+    assert np.median(e_surr) < 0.05 * np.median(e_rule)
     # fallback at the default disagreement tolerance: every flagged pair is
     # recomputed by cubature, so the table is at least as good as the rule
     nf7, _ = S.table(w, mode=S.SURROGATE, tol=1e-7)
@@ -184,10 +194,11 @@ def test_surrogate_gain(bfa, n5, torch_cuda):
     P_rule = _power(bfa, torch, g, mat, w, pw, lambda M: g.assemble_vie(w, mat, M))
     S.table(w, mode=S.CUBATURE, npts=33, level=1)
     P_cub = _power(bfa, torch, g, mat, w, pw, lambda M: S.assemble(w, mat, M))
+    S.table(w, mode=S.CUBATURE, npts=16, level=0)
+    P_16 = _power(bfa, torch, g, mat, w, pw, lambda M: S.assemble(w, mat, M))
     S.table(w, mode=S.SURROGATE, tol=np.inf)
     P_surr = _power(bfa, torch, g, mat, w, pw, lambda M: S.assemble(w, mat, M))
-    d_rule = np.abs(P_rule - P_cub) / np.abs(P_cub)
-    d_surr = np.abs(P_surr - P_cub) / np.abs(P_cub)
-    print(f"[surrogate] power vs all-cubature far field: rule {d_rule.ravel()}, surrogate {d_surr.ravel()}",
-          flush=True)
+    d_rule, d_16, d_surr = (np.abs(P - P_cub) / np.abs(P_cub) for P in (P_rule, P_16, P_surr))
+    print(f"[surrogate] power vs all-cubature far field: 4-point rule {d_rule.ravel()}, 16-point rule "
+          f"{d_16.ravel()}, surrogate {d_surr.ravel()}", flush=True)
     assert d_surr.max() < d_rule.max()
