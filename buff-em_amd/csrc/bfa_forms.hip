@@ -149,7 +149,7 @@ extern "C" int bfa_lu_solve_c(const void *d_A, int n, int64_t lda, const int *d_
   (void)hipFree(scratch);
   if (rc) return rc;
   if (e != hipSuccess) return fail(BFA_ERR_HIP, "bfa_lu_solve_c: %s", hipGetErrorString(e));
-  return bfa::solve_status();
+  return BFA_OK;
 }
 
 // Mixed-precision solve with FP64 iterative refinement.  d_B (n x nrhs,
@@ -233,6 +233,5 @@ extern "C" int bfa_solve_refine(const void *d_M64, int64_t ldm, const void *d_LU
   (void)hipFree(w);
   if (iters) *iters = it;
   if (resid) *resid = worst;
-  if (rc == BFA_OK) rc = bfa::solve_status();
   return rc;
 }

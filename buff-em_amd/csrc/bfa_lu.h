@@ -13,8 +13,7 @@ int gemm_sub_c(int m, int n, int k, const void *A, int64_t lda, const void *B, i
 int lu_factor_z(void *A, int n, int64_t lda, int *ipiv, int *info, hipStream_t st, int nb);
 int lu_factor_c(void *A, int n, int64_t lda, int *ipiv, int *info, hipStream_t st, int nb);
 size_t lu_solve_scratch_bytes(int n, int nrhs, size_t elem);
-// BFA_OK unless a one-launch triangular solve on this device timed out
-int solve_status();
+
 int lu_solve_z(const void *A, int n, int64_t lda, const int *ipiv, void *B, int nrhs, int64_t ldb, hipStream_t st,
                void *scratch);
 int lu_solve_c(const void *A, int n, int64_t lda, const int *ipiv, void *B, int nrhs, int64_t ldb, hipStream_t st,

@@ -128,14 +128,20 @@ __device__ __forceinline__ void gemm64_body(int m, int n, int k, const cplx<T> *
         br[cb] = Bs[0][kk + lq][wc * 32 + cb * 16 + l16];
         bi[cb] = Bs[1][kk + lq][wc * 32 + cb * 16 + l16];
       }
+      // Cr -= ar*br - ai*bi ; Ci -= ar*bi + ai*br  (ar holds -Re A); two
+      // passes so that consecutive MFMAs write different accumulators
 #pragma unroll
       for (int rb = 0; rb < 2; rb++)
 #pragma unroll
         for (int cb = 0; cb < 2; cb++) {
-          // Cr -= ar*br - ai*bi ; Ci -= ar*bi + ai*br  (ar holds -Re A)
           cr[rb][cb] = mf<T>::op(ar[rb], br[cb], cr[rb][cb]);
-          cr[rb][cb] = mf<T>::op(ai[rb], bi[cb], cr[rb][cb]);
           ci[rb][cb] = mf<T>::op(ar[rb], bi[cb], ci[rb][cb]);
+        }
+#pragma unroll
+      for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++) {
+          cr[rb][cb] = mf<T>::op(ai[rb], bi[cb], cr[rb][cb]);
           ci[rb][cb] = mf<T>::op(-ai[rb], br[cb], ci[rb][cb]);
         }
     }
@@ -240,13 +246,20 @@ __device__ __forceinline__ void gemm128_body(int m, int n, int k, const cplx<T> 
         br[cb] = Bs[buf][0][kk + lq][wc * 32 + cb * 16 + l16];
         bi[cb] = Bs[buf][1][kk + lq][wc * 32 + cb * 16 + l16];
       }
+      // two passes so that consecutive MFMAs write different accumulators
+      // (same per-accumulator order: Re A Re B first, then Im A Im B)
 #pragma unroll
       for (int rb = 0; rb < 4; rb++)
 #pragma unroll
         for (int cb = 0; cb < 2; cb++) {
           cr[rb][cb] = mf<T>::op(ar[rb], br[cb], cr[rb][cb]);
-          cr[rb][cb] = mf<T>::op(ai[rb], bi[cb], cr[rb][cb]);
           ci[rb][cb] = mf<T>::op(ar[rb], bi[cb], ci[rb][cb]);
+        }
+#pragma unroll
+      for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++) {
+          cr[rb][cb] = mf<T>::op(ai[rb], bi[cb], cr[rb][cb]);
           ci[rb][cb] = mf<T>::op(-ai[rb], br[cb], ci[rb][cb]);
         }
     }
@@ -452,20 +465,31 @@ static void laswp(cplx<T> *A, int64_t lda, int c0, int c1, int k0, int kb, const
 
 typedef bfa::PanelSlots PanelWork;
 
-// Grid barrier: every thread's stores are drained by the workgroup barrier,
-// one thread per workgroup publishes them device-wide (release fence) and
-// arrives; it then polls with relaxed agent-scope loads and acquires.
+// Cross-workgroup data of the panel (slots, candidate rows, row j) is
+// written and read only with agent-scope atomic stores/loads, which are
+// coherent across the XCDs without cache maintenance; all of it is written
+// by wave 0 of each workgroup.  Grid barrier: thread 0 waits until its
+// wave's stores have completed, arrives on the counter and polls it, so no
+// L2 write-back or invalidate is needed per column.
+template <typename V> __device__ __forceinline__ void st_agent(V *p, V v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename V> __device__ __forceinline__ V ld_agent(const V *p)
+{
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ void grid_barrier(unsigned *bar, unsigned target, int *flags)
 {
   __syncthreads();
   if (threadIdx.x == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_s_waitcnt(0);   // this wave's slot stores have completed
     __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     unsigned spins = 0;
-    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    while (ld_agent(bar) < target) {
       if (++spins > SPIN_LIMIT) { atomicOr(&flags[0], 1); break; }
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
 }
@@ -532,18 +556,31 @@ __device__ __forceinline__ void panel_body(cplx<T> *A, int m, int w, int64_t lda
     T lbest = sh.wval[0];
     int lrow = sh.wrow[0], ld = 0;
     for (int q = 1; q < nwave; q++) argmax_merge(lbest, lrow, ld, sh.wval[q], sh.wrow[q], q);
-    if (tid == 0) { pw.val[buf * nwg + g] = (double)lbest; pw.row[buf * nwg + g] = lrow; }
-    if (lrow != INT_MAX && tid < w) rdata[((size_t)buf * nwg + g) * PW + tid] = RW(lrow - r0, tid);
-    if (j >= r0 && j < r1 && tid < w) rowjg[buf * PW + tid] = RW(j - r0, tid);
+    // (w <= PW = 32: every store below is made by wave 0)
+    if (tid == 0) {
+      st_agent(&pw.val[buf * nwg + g], (double)lbest);
+      st_agent(&pw.row[buf * nwg + g], lrow);
+    }
+    if (lrow != INT_MAX && tid < w) {
+      const cplx<T> v = RW(lrow - r0, tid);
+      cplx<T> *d = &rdata[((size_t)buf * nwg + g) * PW + tid];
+      st_agent(&d->re, v.re);
+      st_agent(&d->im, v.im);
+    }
+    if (j >= r0 && j < r1 && tid < w) {
+      const cplx<T> v = RW(j - r0, tid);
+      st_agent(&rowjg[buf * PW + tid].re, v.re);
+      st_agent(&rowjg[buf * PW + tid].im, v.im);
+    }
     grid_barrier(pw.bar, (unsigned)((j + 1) * nwg), pw.flags);
     // global pivot (same in every workgroup): first index of the maximum
     {
       T bv = -1;
       int br = INT_MAX, bw = -1;
       for (int q = tid; q < nwg; q += blockDim.x) {
-        const int rr = pw.row[buf * nwg + q];
+        const int rr = ld_agent(&pw.row[buf * nwg + q]);
         if (rr == INT_MAX) continue;
-        argmax_merge(bv, br, bw, (T)pw.val[buf * nwg + q], rr, q);
+        argmax_merge(bv, br, bw, (T)ld_agent(&pw.val[buf * nwg + q]), rr, q);
       }
       wave_argmax(bv, br, bw);
       if (lane == 0) { sh.wval[wave] = bv; sh.wrow[wave] = br; sh.wwin[wave] = bw; }
@@ -553,8 +590,9 @@ __device__ __forceinline__ void panel_body(cplx<T> *A, int m, int w, int64_t lda
     int p = sh.wrow[0], win = sh.wwin[0];
     for (int q = 1; q < nwave; q++) argmax_merge(pval, p, win, sh.wval[q], sh.wrow[q], sh.wwin[q]);
     if (tid < w) {
-      sh.prow[tid] = rdata[((size_t)buf * nwg + win) * PW + tid];
-      sh.rj[tid] = rowjg[buf * PW + tid];
+      const cplx<T> *d = &rdata[((size_t)buf * nwg + win) * PW + tid];
+      sh.prow[tid] = {ld_agent(&d->re), ld_agent(&d->im)};
+      sh.rj[tid] = {ld_agent(&rowjg[buf * PW + tid].re), ld_agent(&rowjg[buf * PW + tid].im)};
     }
     if (g == 0 && tid == 0) {
       ipiv[j] = p + piv_off;
@@ -694,9 +732,6 @@ struct LUWork {
   PanelWork pw{};
   void *mem = nullptr;
   int num_cu = 0;
-  unsigned *svflags = nullptr;   // [2][maxsv] block flags of the one-launch triangular solves
-  int maxsv = 0;
-  unsigned epoch = 0;
   hipStream_t side = nullptr;    // bulk trailing updates of the look-ahead LU
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
 };
@@ -713,9 +748,7 @@ static int lu_work(LUWork *&w)
   HIPCHECK(hipGetDeviceProperties(&prop, dev));
   w->num_cu = prop.multiProcessorCount;
   w->maxwg = 1024;
-  w->maxsv = 1 << 16;   // 64-row blocks: n up to 4.2 M
-  const size_t bytes = 2 * w->maxwg * 8 + 2 * w->maxwg * 4 + 2 * (size_t)w->maxwg * PW * 16 + 2 * PW * 16 + 64 + 64 +
-                       (size_t)2 * w->maxsv * 4;
+  const size_t bytes = 2 * w->maxwg * 8 + 2 * w->maxwg * 4 + 2 * (size_t)w->maxwg * PW * 16 + 2 * PW * 16 + 64 + 64;
   HIPCHECK(hipMalloc(&w->mem, bytes));
   HIPCHECK(hipMemset(w->mem, 0, bytes));
   char *p = (char *)w->mem;
@@ -724,8 +757,7 @@ static int lu_work(LUWork *&w)
   w->pw.rowdata = p; p += 2 * (size_t)w->maxwg * PW * 16;
   w->pw.rowj = p; p += 2 * PW * 16;
   w->pw.bar = (unsigned *)p; p += 64;
-  w->pw.flags = (int *)p; p += 64;
-  w->svflags = (unsigned *)p;
+  w->pw.flags = (int *)p;
   w->device = dev;
   return BFA_OK;
 }
@@ -1000,97 +1032,6 @@ __global__ __launch_bounds__(256) void kg_trsv_update(const TrsvProb *__restrict
                       q.ldy, blockIdx.x - q.blk0, sh);
 }
 
-// Triangular solve of all 64-row blocks in ONE launch ("sync-free"):
-// workgroup b owns block row b (dispatch order = dependency order: forward
-// for L, reversed for U).  Its 4 waves take the finished blocks k it depends
-// on round-robin: wait for flag[k] == epoch (bounded spin, acquire), then
-// accumulate L(rows of b, cols of k) * y_k; the partial sums are reduced,
-// the 64x64 diagonal block is solved in LDS, y_b is stored and flag[b] is
-// released.  A workgroup only waits on lower-numbered workgroups, which the
-// in-order dispatcher has already placed or retired, so the chain always
-// progresses.  L unit lower (LOWER) or U upper with its diagonal.
-template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) void k_trsv_sf(int n, int nrhs, const cplx<T> *__restrict__ L, int64_t ldl,
-                                                 cplx<T> *X, int64_t ldx, unsigned *flags, unsigned epoch, int *err)
-{
-  __shared__ cplx<T> Ls[SV_KB][SV_KB + 1];
-  __shared__ cplx<T> Xs[SV_RHS][SV_KB];
-  __shared__ cplx<T> part[4][SV_RHS][64];
-  const int nb = (n + SV_KB - 1) / SV_KB;
-  const int bi = LOWER ? (int)blockIdx.x : nb - 1 - (int)blockIdx.x;
-  const int r0 = bi * SV_KB, kb = min(SV_KB, n - r0);
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  for (int idx = tid; idx < kb * kb; idx += blockDim.x) {
-    const int r = idx % kb, c = idx / kb;
-    Ls[r][c] = L[(size_t)(r0 + c) * ldl + r0 + r];
-  }
-  T ar[SV_RHS], ai[SV_RHS];
-#pragma unroll
-  for (int j = 0; j < SV_RHS; j++) { ar[j] = 0; ai[j] = 0; }
-  const int r = r0 + lane;
-  const int ndep = LOWER ? bi : nb - 1 - bi;
-  for (int q = wv; q < ndep; q += 4) {
-    const int k = LOWER ? q : bi + 1 + q;
-    if (lane == 0) {
-      unsigned spins = 0;
-      while (__hip_atomic_load(&flags[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-        if (++spins > SPIN_LIMIT) { atomicOr(err, 1); break; }
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    const int c0 = k * SV_KB, kc = min(SV_KB, n - c0);
-    if (r < r0 + kb) {
-      for (int c = 0; c < kc; c++) {
-        const cplx<T> l = L[(size_t)(c0 + c) * ldl + r];
-#pragma unroll
-        for (int j = 0; j < SV_RHS; j++) {
-          if (j < nrhs) {
-            const cplx<T> y = X[(size_t)j * ldx + c0 + c];
-            ar[j] += l.re * y.re - l.im * y.im;
-            ai[j] += l.re * y.im + l.im * y.re;
-          }
-        }
-      }
-    }
-  }
-  for (int j = 0; j < nrhs; j++) part[wv][j][lane] = {ar[j], ai[j]};
-  __syncthreads();
-  for (int idx = tid; idx < kb * nrhs; idx += blockDim.x) {
-    const int rr = idx % kb, j = idx / kb;
-    const cplx<T> b = X[(size_t)j * ldx + r0 + rr];
-    const T sr = part[0][j][rr].re + part[1][j][rr].re + part[2][j][rr].re + part[3][j][rr].re;
-    const T si = part[0][j][rr].im + part[1][j][rr].im + part[2][j][rr].im + part[3][j][rr].im;
-    Xs[j][rr] = {b.re - sr, b.im - si};
-  }
-  __syncthreads();
-  const int rl = tid % SV_KB, jg = tid / SV_KB;
-  if (LOWER) {
-    for (int i = 0; i < kb - 1; i++) {
-      if (rl > i && rl < kb)
-        for (int j = jg; j < nrhs; j += 4) Xs[j][rl] = c_sub(Xs[j][rl], c_mul(Ls[rl][i], Xs[j][i]));
-      __syncthreads();
-    }
-  } else {
-    for (int i = kb - 1; i >= 0; i--) {
-      if (rl == i)
-        for (int j = jg; j < nrhs; j += 4) Xs[j][i] = c_mul(Xs[j][i], c_inv(Ls[i][i]));
-      __syncthreads();
-      if (rl < i)
-        for (int j = jg; j < nrhs; j += 4) Xs[j][rl] = c_sub(Xs[j][rl], c_mul(Ls[rl][i], Xs[j][i]));
-      __syncthreads();
-    }
-  }
-  for (int idx = tid; idx < kb * nrhs; idx += blockDim.x) {
-    const int rr = idx % kb, j = idx / kb;
-    X[(size_t)j * ldx + r0 + rr] = Xs[j][rr];
-  }
-  __syncthreads();
-  if (tid == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __hip_atomic_store(&flags[bi], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
-
 template <typename T>
 int lu_solve(const cplx<T> *A, int n, int64_t lda, const int *ipiv, cplx<T> *B, int nrhs, int64_t ldb, hipStream_t st,
              void *scratch)
@@ -1111,35 +1052,33 @@ int lu_solve(const cplx<T> *A, int n, int64_t lda, const int *ipiv, cplx<T> *B, 
   }
   hipLaunchKernelGGL(k_gather_rows<T>, dim3((n + 255) / 256, nrhs), dim3(256), 0, st, B, ldb, X, (int64_t)n, n, nrhs,
                      (const int *)perm);
-  LUWork *lw = nullptr;
-  int rc = lu_work(lw);
-  if (rc) return rc;
-  const int nblk = (n + SV_KB - 1) / SV_KB;
-  if (nblk > lw->maxsv) return fail(BFA_ERR_ARG, "lu_solve: n = %d exceeds the solve flag pool", n);
+  const int nb = SV_KB;
+  const int nblk = (n + nb - 1) / nb;
   for (int j0 = 0; j0 < nrhs; j0 += SV_RHS) {
     const int nj = std::min(SV_RHS, nrhs - j0);
     cplx<T> *Xj = X + (size_t)j0 * n;
-    unsigned e1 = ++lw->epoch, e2 = ++lw->epoch;
-    hipLaunchKernelGGL((k_trsv_sf<T, true>), dim3(nblk), dim3(256), 0, st, n, nj, A, lda, Xj, (int64_t)n,
-                       lw->svflags, e1, lw->pw.flags);
-    hipLaunchKernelGGL((k_trsv_sf<T, false>), dim3(nblk), dim3(256), 0, st, n, nj, A, lda, Xj, (int64_t)n,
-                       lw->svflags + lw->maxsv, e2, lw->pw.flags);
+    for (int bi = 0; bi < nblk; bi++) {              // L y = P b
+      const int k = bi * nb, kb = std::min(nb, n - k);
+      hipLaunchKernelGGL((k_trsv_diag<T, true>), dim3(1), dim3(256), 0, st, kb, nj, A + (size_t)k * lda + k, lda,
+                         Xj + k, (int64_t)n);
+      const int m = n - k - kb;
+      if (m > 0)
+        hipLaunchKernelGGL(k_trsv_update<T>, dim3((m + 63) / 64), dim3(256), 0, st, m, kb, nj,
+                           A + (size_t)k * lda + k + kb, lda, (const cplx<T> *)(Xj + k), (int64_t)n, Xj + k + kb,
+                           (int64_t)n);
+    }
+    for (int bi = nblk - 1; bi >= 0; bi--) {         // U x = y
+      const int k = bi * nb, kb = std::min(nb, n - k);
+      hipLaunchKernelGGL((k_trsv_diag<T, false>), dim3(1), dim3(256), 0, st, kb, nj, A + (size_t)k * lda + k, lda,
+                         Xj + k, (int64_t)n);
+      if (k > 0)
+        hipLaunchKernelGGL(k_trsv_update<T>, dim3((k + 63) / 64), dim3(256), 0, st, k, kb, nj, A + (size_t)k * lda,
+                           lda, (const cplx<T> *)(Xj + k), (int64_t)n, Xj, (int64_t)n);
+    }
   }
   HIPCHECK(hipMemcpy2DAsync(B, ldb * sizeof(cplx<T>), X, n * sizeof(cplx<T>), n * sizeof(cplx<T>), nrhs,
                             hipMemcpyDeviceToDevice, st));
   HIPCHECK(hipGetLastError());
-  return BFA_OK;
-}
-
-// BFA_OK unless a one-launch triangular solve of this device gave up waiting
-int solve_status()
-{
-  LUWork *lw = nullptr;
-  int rc = lu_work(lw);
-  if (rc) return rc;
-  int f = 0;
-  HIPCHECK(hipMemcpy(&f, lw->pw.flags, 4, hipMemcpyDeviceToHost));
-  if (f) return fail(BFA_ERR_HIP, "triangular solve: a workgroup timed out waiting for its dependencies");
   return BFA_OK;
 }
 
@@ -1272,7 +1211,7 @@ extern "C" int bfa_lu_solve_z(const void *d_A, int n, int64_t lda, const int *d_
   hipFree(scratch);
   if (rc) return rc;
   if (e != hipSuccess) return fail(BFA_ERR_HIP, "bfa_lu_solve_z: %s", hipGetErrorString(e));
-  return bfa::solve_status();
+  return BFA_OK;
 }
 
 extern "C" int bfa_zgemm_sub(int m, int n, int k, const void *d_A, int64_t lda, const void *d_B, int64_t ldb,

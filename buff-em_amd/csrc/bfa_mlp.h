@@ -21,8 +21,9 @@ int mlp_forward(const MLPShape &s, int E, int B, const _Float16 *X16, int64_t sx
                 const float *bias32, float *Y, float *Zsave, _Float16 *Abuf, hipStream_t st);
 int mlp_backward(const MLPShape &s, int E, int B, const _Float16 *X16, int64_t sx, const _Float16 *W16,
                  const float *Zsave, const _Float16 *Abuf, const float *dY, float *grad, float *work, hipStream_t st);
-int mlp_mse(int E, int B, int nout, const float *Y, const float *T, float *dY, double *loss_parts, int nparts,
-            hipStream_t st);
+// w: per-sample weights [E][B] or NULL
+int mlp_mse(int E, int B, int nout, const float *Y, const float *T, const float *w, float *dY, double *loss_parts,
+            int nparts, hipStream_t st);
 int mlp_adam(int64_t n, float *w, const float *g, float *m, float *v, _Float16 *w16, float lr, int step,
              hipStream_t st);
 int to_half(const float *src, _Float16 *dst, int64_t n, hipStream_t st);

@@ -200,9 +200,10 @@ __global__ void k_colsum(const float *dz, int B, int N, int64_t sdz, float *db, 
   db[e * sdb + n] = acc;
 }
 
-// MSE over all outputs: loss = sum (Y - T)^2 / (B * nout); dY = 2 (Y - T) / (B * nout)
-__global__ __launch_bounds__(256) void k_mse(const float *Y, const float *T, int B, int nout, int64_t s, float *dY,
-                                             double *loss)
+// (Weighted) MSE over all outputs: loss = sum w_b (Y - T)^2 / (B * nout);
+// dY = 2 w_b (Y - T) / (B * nout); w = NULL means w_b = 1.
+__global__ __launch_bounds__(256) void k_mse(const float *Y, const float *T, const float *w, int B, int nout,
+                                             int64_t s, float *dY, double *loss)
 {
   __shared__ double red[256];
   const int e = blockIdx.y;
@@ -211,8 +212,9 @@ __global__ __launch_bounds__(256) void k_mse(const float *Y, const float *T, int
   double acc = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const float d = Y[e * s + i] - T[e * s + i];
-    dY[e * s + i] = scale * d;
-    acc += (double)d * d;
+    const float wb = w ? w[(int64_t)e * B + i / nout] : 1.0f;
+    dY[e * s + i] = scale * wb * d;
+    acc += (double)wb * d * d;
   }
   red[threadIdx.x] = acc;
   __syncthreads();
@@ -353,10 +355,10 @@ int mlp_backward(const MLPShape &s, int E, int B, const _Float16 *X16, int64_t s
   return BFA_OK;
 }
 
-int mlp_mse(int E, int B, int nout, const float *Y, const float *T, float *dY, double *loss_parts, int nparts,
-            hipStream_t st)
+int mlp_mse(int E, int B, int nout, const float *Y, const float *T, const float *w, float *dY, double *loss_parts,
+            int nparts, hipStream_t st)
 {
-  hipLaunchKernelGGL(k_mse, dim3(nparts, E), dim3(256), 0, st, Y, T, B, nout, (int64_t)B * nout, dY, loss_parts);
+  hipLaunchKernelGGL(k_mse, dim3(nparts, E), dim3(256), 0, st, Y, T, w, B, nout, (int64_t)B * nout, dY, loss_parts);
   HIPCHECK(hipGetLastError());
   return BFA_OK;
 }
@@ -436,7 +438,7 @@ extern "C" int bfa_mlp_grad(int nin, int hidden, int nout, int E, const float *p
     if (r) return r;
     r = mlp_forward(s, E, B, X16, (int64_t)B * s.kp, W16, W, Yd, Z, A, st);
     if (r) return r;
-    r = mlp_mse(E, B, nout, Yd, Td, dY, lp, nparts, st);
+    r = mlp_mse(E, B, nout, Yd, Td, nullptr, dY, lp, nparts, st);
     if (r) return r;
     r = mlp_backward(s, E, B, X16, (int64_t)B * s.kp, W16, Z, A, dY, G, work, st);
     if (r) return r;

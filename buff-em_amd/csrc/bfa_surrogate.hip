@@ -15,9 +15,10 @@
 //   surrogate  the 16-point product rule R16 plus the ensemble mean of an
 //              MLP that predicts R16's remaining error in units of the
 //              per-pair error estimate ||R16 - R4||, from the pair geometry
-//              canonicalised to the source frame and kr; cubature wherever
-//              the members disagree by more than a tolerance or the input
-//              lies outside the training range.
+//              canonicalised to the source frame, kr, and the direction and
+//              size of R16 - R4 (a learned extrapolation of the rule
+//              sequence); cubature wherever the members disagree by more
+//              than a tolerance or the input lies outside the training range.
 // The far-field assembly then reads the blocks instead of integrating.
 #include <hip/hip_runtime.h>
 #include <algorithm>
@@ -41,7 +42,8 @@ using namespace bfa;
   } while (0)
 
 #define FOURPI 12.566370614359172954
-#define NFEAT 19          // 12 target coordinates + 6 source shape + log(k h)
+#define NGEO 19           // 12 target coordinates + 6 source shape + log(k h)
+#define NFEAT 52          // NGEO + unit (R16 - R4) (32) + log10 ||R16 - R4|| / ||R16||
 #define NOUT 32           // 16 complex block entries
 #define MAXPTS 33
 
@@ -278,9 +280,14 @@ struct FeatJob {
   const double *V;
   const int *tetVI;
   int n;
+  const double *r4, *r16;   // rule blocks of the samples (NULL: geometric inputs only)
 };
 
-// features -> FP32 [n][NFEAT] and standardised FP16 [n][kp] (if x16)
+// Network inputs: the NGEO geometric ones, then the 4- and 16-point rules'
+// difference as a unit vector and its log10 size relative to R16 (the
+// network learns how R16's remaining error relates to that difference).
+// feat: raw inputs [n][nf] (nf = NGEO without rule blocks, else NFEAT);
+// x16: standardised FP16 inputs [n][kp].
 __global__ void k_features(FeatJob J, float *feat, _Float16 *x16, int kp, const float *mu, const float *sd)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -295,11 +302,23 @@ __global__ void k_features(FeatJob J, float *feat, _Float16 *x16, int kp, const 
   }
   float f[NFEAT];
   d_features(VS, VT, omega, f);
+  const int nf = J.r4 ? NFEAT : NGEO;
+  if (J.r4) {
+    const double *a = J.r4 + (size_t)i * NOUT, *b = J.r16 + (size_t)i * NOUT;
+    double d = 0.0, nb = 0.0;
+    for (int k = 0; k < NOUT; k++) {
+      d += (b[k] - a[k]) * (b[k] - a[k]);
+      nb += b[k] * b[k];
+    }
+    d = fmax(sqrt(d), 1e-300);
+    for (int k = 0; k < NOUT; k++) f[NGEO + k] = (float)((b[k] - a[k]) / d);
+    f[NGEO + NOUT] = (float)log10(d / sqrt(nb));
+  }
   if (feat)
-    for (int k = 0; k < NFEAT; k++) feat[(size_t)i * NFEAT + k] = f[k];
+    for (int k = 0; k < nf; k++) feat[(size_t)i * nf + k] = f[k];
   if (x16) {
     for (int k = 0; k < kp; k++) {
-      const float v = k < NFEAT ? (f[k] - mu[k]) / sd[k] : 0.f;
+      const float v = k < nf ? (f[k] - mu[k]) / sd[k] : 0.f;
       x16[(size_t)i * kp + k] = (_Float16)v;
     }
   }
@@ -336,13 +355,15 @@ __global__ void k_standardise(float *y, int n, int m, const float *mu, const flo
   y[i] = (y[i] - mu[k]) / sd[k];
 }
 
-// minibatch gather: X16[e][b] = src16[idx[e][b]], T[e][b] = srcT[idx[e][b]]
-__global__ void k_gather(const _Float16 *src16, int kp, const float *srcT, const int *idx, int B, _Float16 *X16,
-                         float *T)
+// minibatch gather: X16[e][b] = src16[idx[e][b]], T[e][b] = srcT[idx[e][b]],
+// Wt[e][b] = srcW[idx[e][b]]
+__global__ void k_gather(const _Float16 *src16, int kp, const float *srcT, const float *srcW, const int *idx, int B,
+                         _Float16 *X16, float *T, float *Wt)
 {
   const int b = blockIdx.x * blockDim.x + threadIdx.x, e = blockIdx.y;
   if (b >= B) return;
   const int s = idx[(size_t)e * B + b];
+  Wt[(size_t)e * B + b] = srcW[s];
   for (int k = 0; k < kp; k++) X16[((size_t)e * B + b) * kp + k] = src16[(size_t)s * kp + k];
   for (int k = 0; k < NOUT; k++) T[((size_t)e * B + b) * NOUT + k] = srcT[(size_t)s * NOUT + k];
 }
@@ -382,6 +403,7 @@ __global__ void k_combine(const float *Y, int E, int n, const float *ymu, const 
   }
   bool ood = false;
   for (int k = 0; k < NFEAT; k++) {
+    if (k >= NGEO && k < NGEO + NOUT) continue;   // unit-vector components are bounded anyway
     const float v = feat[(size_t)i * NFEAT + k];
     if (v < flo[k] || v > fhi[k]) ood = true;
   }
@@ -512,6 +534,7 @@ struct bfa_surrogate {
   _Float16 *d_X16 = nullptr;         // [nsamp][kp]
   float *d_Y = nullptr;              // [nsamp][32] standardised targets
   float *d_F = nullptr;              // [nsamp][NFEAT] raw features
+  float *d_W = nullptr;              // [nsamp] loss weights
   // last prediction statistics
   int last_flagged = 0;
   double last_disagree_max = 0.0;
@@ -521,7 +544,7 @@ struct bfa_surrogate {
 bfa_surrogate::~bfa_surrogate()
 {
   void *ptrs[] = {d_ps, d_pt, d_bfLoc, d_trow, d_tcol, d_tidx, d_blocks, W, m, v, G, W16, d_stats,
-                  d_spair, d_somega, d_X16, d_Y, d_F};
+                  d_spair, d_somega, d_X16, d_Y, d_F, d_W};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
 }
@@ -660,7 +683,7 @@ extern "C" int bfa_surr_create(bfa_geometry *g, double factor, int ensemble, int
   for (int e = 0; e < S->E; e++)
     for (int l = 0; l < 4; l++) {
       const int nout = l == 3 ? NOUT : hidden, kin = l == 0 ? S->shp.kp : hidden;
-      const int kreal = l == 0 ? NFEAT : hidden;
+      const int kreal = l == 0 ? NFEAT : hidden;   // padding columns stay zero
       const float sc = std::sqrt(2.0f / kreal) * (l == 3 ? 0.1f : 1.0f);
       for (int r = 0; r < nout; r++)
         for (int k = 0; k < kreal; k++) w[(size_t)e * np + S->shp.oW[l] + (size_t)r * kin + k] = sc * nd(rng);
@@ -723,7 +746,7 @@ extern "C" int bfa_surr_cubature(bfa_surrogate *S, const int *pair, const double
   return rc;
 }
 
-// Canonical features (host out [n][NFEAT]) of pair[n] at omega[n].
+// Geometric network inputs (host out [n][NGEO]) of pair[n] at omega[n].
 extern "C" int bfa_surr_features(bfa_surrogate *S, const int *pair, const double *omega, int n, float *feat)
 {
   if (!S || !pair || !omega || !feat || n <= 0) return fail(BFA_ERR_ARG, "bfa_surr_features: bad argument");
@@ -732,16 +755,17 @@ extern "C" int bfa_surr_features(bfa_surrogate *S, const int *pair, const double
   float *df = nullptr;
   HIPCHECK(hipMalloc(&dp, (size_t)n * 4));
   HIPCHECK(hipMalloc(&dw, (size_t)n * 8));
-  HIPCHECK(hipMalloc(&df, (size_t)n * NFEAT * 4));
+  HIPCHECK(hipMalloc(&df, (size_t)n * NGEO * 4));
   int rc = BFA_OK;
   if (hipMemcpy(dp, pair, (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(dw, omega, (size_t)n * 8, hipMemcpyHostToDevice) != hipSuccess)
     rc = fail(BFA_ERR_HIP, "bfa_surr_features: upload");
   if (!rc) {
-    FeatJob J{S->d_ps, S->d_pt, dp, dw, 0.0, (const double *)S->geo->d.V, (const int *)S->geo->d.tetVI, n};
+    FeatJob J{S->d_ps, S->d_pt, dp, dw, 0.0, (const double *)S->geo->d.V, (const int *)S->geo->d.tetVI, n,
+              nullptr, nullptr};
     hipLaunchKernelGGL(k_features, dim3((n + 255) / 256), dim3(256), 0, nullptr, J, df, (_Float16 *)nullptr, 0,
                        (const float *)nullptr, (const float *)nullptr);
-    if (hipGetLastError() != hipSuccess || hipMemcpy(feat, df, (size_t)n * NFEAT * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipGetLastError() != hipSuccess || hipMemcpy(feat, df, (size_t)n * NGEO * 4, hipMemcpyDeviceToHost) != hipSuccess)
       rc = fail(BFA_ERR_HIP, "bfa_surr_features: kernel");
   }
   (void)hipFree(dp);
@@ -766,9 +790,11 @@ extern "C" int bfa_surr_dataset(bfa_surrogate *S, int per_pair, double wmin, dou
   std::uniform_real_distribution<double> U(std::log(wmin), std::log(wmax));
   for (int p = 0, i = 0; p < S->npairs; p++)
     for (int r = 0; r < per_pair; r++, i++) { sp[i] = p; sw[i] = std::exp(U(rng)); }
-  for (void *p : {(void *)S->d_spair, (void *)S->d_somega, (void *)S->d_X16, (void *)S->d_Y, (void *)S->d_F})
+  for (void *p : {(void *)S->d_spair, (void *)S->d_somega, (void *)S->d_X16, (void *)S->d_Y, (void *)S->d_F,
+                  (void *)S->d_W})
     if (p) (void)hipFree(p);
   S->d_spair = nullptr; S->d_somega = nullptr; S->d_X16 = nullptr; S->d_Y = nullptr; S->d_F = nullptr;
+  S->d_W = nullptr;
   int rc;
   if ((rc = dev_copy(&S->d_spair, sp)) || (rc = dev_copy(&S->d_somega, sw))) return rc;
   const int kp = S->shp.kp;
@@ -788,7 +814,7 @@ extern "C" int bfa_surr_dataset(bfa_surrogate *S, int per_pair, double wmin, dou
     r = launch_blocks(S, S->d_spair, S->d_somega, 0.0, nullptr, n, label_npts, label_level, label, nullptr);
     if (r) return r;
     FeatJob J{S->d_ps, S->d_pt, S->d_spair, S->d_somega, 0.0, (const double *)S->geo->d.V,
-              (const int *)S->geo->d.tetVI, n};
+              (const int *)S->geo->d.tetVI, n, rule, r16};
     hipLaunchKernelGGL(k_features, dim3((n + 255) / 256), dim3(256), 0, nullptr, J, S->d_F, (_Float16 *)nullptr, 0,
                        (const float *)nullptr, (const float *)nullptr);
     hipLaunchKernelGGL(k_targets, dim3((n + 255) / 256), dim3(256), 0, nullptr, (const double *)label,
@@ -816,6 +842,29 @@ extern "C" int bfa_surr_dataset(bfa_surrogate *S, int per_pair, double wmin, dou
       for (int i = 0; i < n; i++) { const double y = Y[(size_t)i * NOUT + k]; s += y; s2 += y * y; }
       const double mu = s / n, var = std::max(1e-30, s2 / n - mu * mu);
       S->ymu[k] = (float)mu; S->ysd[k] = (float)std::sqrt(var);
+    }
+    // relative loss: sample weight 1 / (sum_k (y_ik / sd_k)^2 + 1e-2 of its
+    // median), normalised to mean 1, so typical pairs are fitted to the same
+    // relative accuracy as the large-error tail
+    {
+      std::vector<double> m2(n);
+      for (int i = 0; i < n; i++) {
+        double a = 0.0;
+        for (int k = 0; k < NOUT; k++) {
+          const double z = (double)Y[(size_t)i * NOUT + k] / S->ysd[k];
+          a += z * z;
+        }
+        m2[i] = a;
+      }
+      std::vector<double> tmp(m2);
+      std::nth_element(tmp.begin(), tmp.begin() + n / 2, tmp.end());
+      const double floor_ = 1e-2 * std::max(tmp[n / 2], 1e-30);
+      std::vector<float> wts(n);
+      double sw = 0.0;
+      for (int i = 0; i < n; i++) { wts[i] = (float)(1.0 / (m2[i] + floor_)); sw += wts[i]; }
+      for (int i = 0; i < n; i++) wts[i] = (float)(wts[i] * n / sw);
+      HIPCHECK(hipMalloc(&S->d_W, (size_t)n * 4));
+      HIPCHECK(hipMemcpy(S->d_W, wts.data(), (size_t)n * 4, hipMemcpyHostToDevice));
     }
     std::vector<float> st;
     st.insert(st.end(), S->xmu, S->xmu + NFEAT); st.insert(st.end(), S->xsd, S->xsd + NFEAT);
@@ -852,7 +901,7 @@ extern "C" int bfa_surr_train(bfa_surrogate *S, int epochs, int batch, double lr
   char *mem = nullptr;
   const size_t bytes = (size_t)E * B * kp * 2 + (size_t)E * B * NOUT * 4 * 3 + (size_t)E * B * S->hidden * 3 * 4 +
                        (size_t)E * mlp_work_halfs(S->shp, B) * 2 + (size_t)E * mlp_work_floats(S->shp, B) * 4 +
-                       (size_t)E * B * 4 + (size_t)E * nparts * 8 + 16 * 256;
+                       (size_t)E * B * 4 * 2 + (size_t)E * nparts * 8 + 16 * 256;
   HIPCHECK(hipMalloc(&mem, bytes));
   size_t o = 0;
   auto take = [&](size_t b) { size_t r = (o + 255) / 256 * 256; o = r + b; return mem + r; };
@@ -864,6 +913,7 @@ extern "C" int bfa_surr_train(bfa_surrogate *S, int epochs, int batch, double lr
   _Float16 *A = (_Float16 *)take((size_t)E * mlp_work_halfs(S->shp, B) * 2);
   float *work = (float *)take((size_t)E * mlp_work_floats(S->shp, B) * 4);
   int *idx = (int *)take((size_t)E * B * 4);
+  float *Wt = (float *)take((size_t)E * B * 4);
   double *lp = (double *)take((size_t)E * nparts * 8);
   std::mt19937_64 rng(S->seed * 7919 + S->adam_step);
   std::vector<std::vector<int>> perm(E, std::vector<int>(S->nsamp));
@@ -884,9 +934,9 @@ extern "C" int bfa_surr_train(bfa_surrogate *S, int epochs, int batch, double lr
         break;
       }
       hipLaunchKernelGGL(k_gather, dim3((B + 255) / 256, E), dim3(256), 0, nullptr, (const _Float16 *)S->d_X16, kp,
-                         (const float *)S->d_Y, (const int *)idx, B, X16, T);
+                         (const float *)S->d_Y, (const float *)S->d_W, (const int *)idx, B, X16, T, Wt);
       if ((rc = mlp_forward(S->shp, E, B, X16, (int64_t)B * kp, S->W16, S->W, Yp, Z, A, nullptr))) break;
-      if ((rc = mlp_mse(E, B, NOUT, Yp, T, dY, lp, nparts, nullptr))) break;
+      if ((rc = mlp_mse(E, B, NOUT, Yp, T, Wt, dY, lp, nparts, nullptr))) break;
       if ((rc = mlp_backward(S->shp, E, B, X16, (int64_t)B * kp, S->W16, Z, A, dY, S->G, work, nullptr))) break;
       S->adam_step++;
       // cosine decay of the step size over the run
@@ -954,7 +1004,7 @@ extern "C" int bfa_surr_table(bfa_surrogate *S, double omega, int mode, double t
     const int m = std::min(chunk, n - c0);
     // features of pairs c0..c0+m at omega (sample index = pair index offset)
     FeatJob J{S->d_ps + c0, S->d_pt + c0, nullptr, nullptr, omega, (const double *)S->geo->d.V,
-              (const int *)S->geo->d.tetVI, m};
+              (const int *)S->geo->d.tetVI, m, rule + (size_t)c0 * NOUT, r16 + (size_t)c0 * NOUT};
     hipLaunchKernelGGL(k_features, dim3((m + 255) / 256), dim3(256), 0, nullptr, J, F, X16, kp, mu, sd);
     if ((rc = mlp_forward(S->shp, E, m, X16, 0, S->W16, S->W, Y, nullptr, A, nullptr))) break;
     hipLaunchKernelGGL(k_combine, dim3((m + 255) / 256), dim3(256), 0, nullptr, (const float *)Y, E, m, ym, ys,

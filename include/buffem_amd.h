@@ -211,10 +211,11 @@ int  bfa_mlp_grad(int nin, int hidden, int nout, int E, const float *params, int
  * (face i of S, face j of T) feeds the far-field assembly.  The blocks come
  * from the reference's 4-point rule, from cubature (product TetCR rule of
  * npts in {4,16,33} on tetrahedra subdivided `level` in {0,1,2} times), or
- * from an ensemble of MLPs (19 inputs: the target's 4 vertices and the
- * source's shape in the source frame over the source's mean edge, log(k h);
- * 32 outputs: the 16-point product rule's remaining error in units of the
- * per-pair estimate ||R16 - R4||) added to R16, with cubature fallback.    */
+ * from an ensemble of MLPs (52 inputs: the target's 4 vertices and the
+ * source's shape in the source frame over the source's mean edge, log(k h),
+ * the unit vector and log10 relative size of R16 - R4 (16- and 4-point
+ * product rules); 32 outputs: R16's remaining error in units of
+ * ||R16 - R4||) added to R16, with cubature fallback.                       */
 typedef struct bfa_surrogate bfa_surrogate;
 int  bfa_surr_create(bfa_geometry *g, double factor, int ensemble, int hidden, uint64_t seed,
                      bfa_surrogate **out);
@@ -225,7 +226,7 @@ int  bfa_surr_pairs(const bfa_surrogate *s, int *npairs, int *ps, int *pt);
  * G_00, G_01, ..., G_33).                                                  */
 int  bfa_surr_cubature(bfa_surrogate *s, const int *pair, const double *omega, int n, int npts,
                        int level, double *blocks);
-/* Network inputs of pair[n] at omega[n]: feat[n][19] (host).               */
+/* Geometric network inputs of pair[n] at omega[n]: feat[n][19] (host).     */
 int  bfa_surr_features(bfa_surrogate *s, const int *pair, const double *omega, int n, float *feat);
 /* Training set: every pair at per_pair seeded log-uniform frequencies in
  * [wmin, wmax], labelled by cubature (label_npts, label_level).             */
@@ -242,7 +243,7 @@ int  bfa_surr_table_download(const bfa_surrogate *s, double *blocks /* npairs*32
 /* bfa_assemble_vie with the far field read from the block table.           */
 int  bfa_assemble_vie_table(bfa_surrogate *s, double omega, const bfa_material *mat, void *d_M,
                             int64_t ld, void *stream);
-/* ensemble size, parameters per member, training samples; stats[4*19+2*32]
+/* ensemble size, parameters per member, training samples; stats[4*52+2*32]
  * = input mean, sd, lower, upper bound; target mean, sd.                   */
 int  bfa_surr_info(const bfa_surrogate *s, int *ensemble, int64_t *nparam, int *nsamples, float *stats);
 /* Get (out) and/or set (in) the FP32 parameters [ensemble][nparam].        */
