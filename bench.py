@@ -28,6 +28,31 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes per launch of the roofline kernel at N = 19,872, measured by two
+# rocprofv3 --pmc passes (FETCH_SIZE x2, WRITE_SIZE x1; the scale factors are
+# read off a streaming kernel of known size in the same run)
+MEASURED_TRAFFIC = {("k_gemm_sub2<float>", 19872): 2.172e9}
+
+
+def trailing_gemm_schedule(N, nb=256, elem=8):
+    """(launches, algorithmic bytes, flops) of the 128x128-tile trailing-update
+    GEMMs of the look-ahead LU: per block step the next block column and the
+    bulk rest, each C(m x n) -= A(m x nb) B(nb x n) with m, n >= 256."""
+    cnt, byt, flo = 0, 0.0, 0.0
+    for j in range(0, N, nb):
+        nxt = min(j + nb, N)
+        if nxt >= N:
+            break
+        nb2 = min(nb, N - nxt)
+        m = N - nxt
+        for n in (nb2, N - nxt - nb2):
+            if m >= 256 and n >= 256:
+                k = min(nb, N - j)
+                byt += elem * (m * k + k * n + m * n) + elem * m * n
+                flo += 8.0 * m * n * k
+                cnt += 1
+    return cnt, byt, flo
+
 
 def load_bfa():
     import importlib.util
@@ -320,7 +345,9 @@ def main():
         "bound": "mfma", "unit": "TFLOP/s",
         "achieved": gemm_tflops, "peak": peak,
         "frac": gemm_tflops / peak if gemm_tflops else None,
-        "traffic": None,
+        "traffic": MEASURED_TRAFFIC.get(("k_gemm_sub2<float>" if best != "fp64" else "k_gemm_sub2<double>", N)),
+        "algorithmic_bytes_per_launch": (lambda c: c[1] / c[0] if c[0] else None)(
+            trailing_gemm_schedule(N, elem=16 if best == "fp64" else 8)),
         "per_launch_flops": gemm_flops / max(gemm_launches, 1),
         "avg_launch_ms": gemm_ms / max(gemm_launches, 1),
         "launches_per_step": gemm_launches / args.steps,

