@@ -733,7 +733,6 @@ struct LUWork {
   void *mem = nullptr;
   int num_cu = 0;
   hipStream_t side = nullptr;    // bulk trailing updates of the look-ahead LU
-  hipStream_t side_m = nullptr;  // same, restricted to 3/4 of the CUs (complex64 LU)
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
 };
 static LUWork g_lu[16];
@@ -845,20 +844,8 @@ int lu_factor(cplx<T> *A, int n, int64_t lda, int *ipiv, int *info, hipStream_t 
   if (!lw->side) {
     HIPCHECK(hipStreamCreateWithFlags(&lw->side, hipStreamNonBlocking));
     for (auto &e : lw->ev) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    // In the complex64 LU the panel chain, not the trailing GEMM, is the
-    // critical path: the bulk update is kept off 8 CUs in every 32 (bits
-    // 24..31 of each mask word, an even share of every XCD whether the mask
-    // numbers CUs XCD-major or round-robin) so the panel's workgroups find
-    // free CUs at once.  Unavailable masks fall back to the full stream.
-    std::vector<uint32_t> mask((lw->num_cu + 31) / 32, 0u);
-    for (int c = 0; c < lw->num_cu; c++)
-      if ((c & 31) < 24) mask[c / 32] |= 1u << (c & 31);
-    if (hipExtStreamCreateWithCUMask(&lw->side_m, (uint32_t)mask.size(), mask.data()) != hipSuccess) {
-      (void)hipGetLastError();
-      lw->side_m = lw->side;
-    }
   }
-  hipStream_t sb = sizeof(T) == 4 ? lw->side_m : lw->side;
+  hipStream_t sb = lw->side;
   hipEvent_t ev_p = lw->ev[0], ev_b = lw->ev[1], ev_s = lw->ev[2];
   HIPCHECK(hipMemsetAsync(lw->pw.flags, 0, 8, st));
   HIPCHECK(hipEventRecord(ev_s, st));
