@@ -67,7 +67,7 @@ static int dev_upload(void **dst, const void *src, size_t bytes)
 static void dev_free_all(bfa::DevBuffers &d)
 {
   void **ptrs[] = {&d.V, &d.tetVI, &d.tetFI, &d.tetVol, &d.q4, &d.q16, &d.q33, &d.bfV, &d.bfT,
-                   &d.bfPF, &d.bfQ, &d.bfRad, &d.rowptr, &d.cols, &d.pairRow, &d.tdPairs,
+                   &d.bfPF, &d.bfQ, &d.bfRad, &d.rowptr, &d.cols, &d.pairRow, &d.ftTab, &d.tdPairs,
                    &d.tdVerts, &d.tdMom, &d.pairTD, &d.table};
   for (void **p : ptrs)
     if (*p) { hipFree(*p); *p = nullptr; }
@@ -124,6 +124,24 @@ extern "C" int bfa_geometry_upload(bfa_geometry *g, void *stream)
     }
     for (int j = g->rowptr[a]; j < g->rowptr[a + 1]; j++) pairRow[j] = a;
   }
+  // far-field tiles: distinct tetrahedra of each FT2-BF tile, in order of
+  // first appearance, and the slot of every (BF, side) in that list
+  const int nft = (N + FT2 - 1) / FT2;
+  std::vector<int> ftTab((size_t)nft * FTAB, -1);
+  for (int tile = 0; tile < nft; tile++) {
+    int *row = &ftTab[(size_t)tile * FTAB];
+    int n = 0;
+    for (int k = 0; k < 2 * FT2; k++) {
+      const int a = tile * FT2 + k / 2;
+      if (a >= N) { row[2 * FT2 + k] = 0; continue; }
+      const int t = bfT[2 * a + (k & 1)];
+      int u = 0;
+      while (u < n && row[u] != t) u++;
+      if (u == n) row[n++] = t;
+      row[2 * FT2 + k] = u;
+    }
+    row[4 * FT2] = n;
+  }
   for (int t = 0; t < nt; t++) {
     volcen[t] = g->tetVol[t];
     for (int d = 0; d < 3; d++) volcen[nt + 3 * t + d] = g->tetCen[3 * t + d];
@@ -144,7 +162,8 @@ extern "C" int bfa_geometry_upload(bfa_geometry *g, void *stream)
       (rc = dev_upload(&d.bfRad, g->fRad.data(), (size_t)N * 8)) ||
       (rc = dev_upload(&d.rowptr, g->rowptr.data(), g->rowptr.size() * 4)) ||
       (rc = dev_upload(&d.cols, g->cols.data(), g->cols.size() * 4)) ||
-      (rc = dev_upload(&d.pairRow, pairRow.data(), pairRow.size() * 4))) {
+      (rc = dev_upload(&d.pairRow, pairRow.data(), pairRow.size() * 4)) ||
+      (rc = dev_upload(&d.ftTab, ftTab.data(), ftTab.size() * 4))) {
     dev_free_all(d);
     return rc;
   }
@@ -156,6 +175,7 @@ extern "C" int bfa_geometry_upload(bfa_geometry *g, void *stream)
   dg.bfV = (const int *)d.bfV; dg.bfT = (const int *)d.bfT; dg.bfPF = (const double *)d.bfPF;
   dg.bfQ = (const double *)d.bfQ; dg.bfRad = (const double *)d.bfRad;
   dg.rowptr = (const int *)d.rowptr; dg.cols = (const int *)d.cols; dg.pairRow = (const int *)d.pairRow;
+  dg.ftTab = (const int *)d.ftTab;
   g->uploaded = true;
   g->static_built = false;
   return BFA_OK;
@@ -590,59 +610,151 @@ __device__ __forceinline__ void tile_coords(int64_t bid, int ntiles, int &ti, in
   tj = r + (int)(bid - start(r));
 }
 
+// Tetrahedron-pair form of the far field.  Consecutive BFs share
+// tetrahedra, so a 32 x 32 BF tile touches ~31 x 31 distinct tetrahedron
+// pairs instead of 4 x 1024.  Per tetrahedron pair (S, T) the 4-point rule's
+// moments about the centroids cS, cT (signed weights w = 6 V w_rule):
+//   I0 = sum w w' g, Ix = sum w w' g (x - cS), Iy = sum w w' g (y - cT),
+//   Ixy = sum w w' g (x - cS).(y - cT),   g = exp(ikr) / r,
+// then per BF pair and side combination, with a = Q_a - cS, b = Q_b - cT:
+//   sum w w' g b_a.b_b - div div/k^2 = pf_a pf_b [Ixy - b.Ix - a.Iy + (a.b) I0 - 9 I0/k^2]
+// -- the same 64-point sum as GetGME_BFBFInt's fixed rule, regrouped.
+#define FTT_MAX 1024   // tetrahedron pairs whose moments fit in LDS (larger tiles: direct path)
+#define FT_THREADS 512
+
+// Structure-of-arrays so that lanes on consecutive tetrahedra hit
+// consecutive LDS banks.
+struct FarTpShared {
+  double mom[16][FTT_MAX];       // per tetrahedron pair: I0, Ix, Iy, Ixy (re, im)
+  double pts[2][16][2 * FT2];    // [row/col][point*4 + {x,y,z,w}][distinct tet]
+  double cen[2][3][2 * FT2];
+  int slot[2][2 * FT2];          // (BF local * 2 + side) -> distinct tet
+};
+
+// 4-point product-rule moments of one tetrahedron pair (signed weights);
+// out[i * stride] for i < 16
 template <bool REALK>
-__device__ __forceinline__ void far_tile(const DevGeom &g, double kre, double kim, cd *M, int64_t ld, int ti, int tj,
-                                         double (&cpts)[TILE][8][7], int (&cverts)[TILE][5])
+__device__ __forceinline__ void tp_moments(const double *xs, const double *cS, const double *ys, const double *cT,
+                                           double kre, double kim, double *out, int stride)
 {
-  const int tid = threadIdx.x, N = g.nbf;
-  for (int idx = tid; idx < TILE * 8; idx += blockDim.x) {
-    const int bl = idx >> 3, pt = idx & 7, b = tj * TILE + bl;
-    if (b < N) bf_points4(g, b, pt, &cpts[bl][pt][0]);
+  double m[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) m[i] = 0.0;
+#pragma unroll 1
+  for (int p = 0; p < 4; p++) {
+    const double *x = xs + 4 * p;
+    const double xi0 = x[0] - cS[0], xi1 = x[1] - cS[1], xi2 = x[2] - cS[2];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const double *y = ys + 4 * q;
+      const double R0 = x[0] - y[0], R1 = x[1] - y[1], R2 = x[2] - y[2];
+      const double r = sqrt(R0 * R0 + R1 * R1 + R2 * R2);
+      double sn, co;
+      sincos(kre * r, &sn, &co);
+      double amp = x[3] * y[3] / r;
+      if (!REALK) amp *= exp(-kim * r);
+      const double gr = amp * co, gi = amp * sn;
+      const double e0 = y[0] - cT[0], e1 = y[1] - cT[1], e2 = y[2] - cT[2];
+      const double xy = xi0 * e0 + xi1 * e1 + xi2 * e2;
+      m[0] += gr; m[1] += gi;
+      m[2] += gr * xi0; m[3] += gi * xi0; m[4] += gr * xi1; m[5] += gi * xi1; m[6] += gr * xi2; m[7] += gi * xi2;
+      m[8] += gr * e0; m[9] += gi * e0; m[10] += gr * e1; m[11] += gi * e1; m[12] += gr * e2; m[13] += gi * e2;
+      m[14] += gr * xy; m[15] += gi * xy;
+    }
   }
-  for (int idx = tid; idx < TILE * 5; idx += blockDim.x) {
-    const int bl = idx / 5, b = tj * TILE + bl;
-    cverts[bl][idx % 5] = b < N ? g.bfV[5 * b + idx % 5] : -1;
+#pragma unroll
+  for (int i = 0; i < 16; i++) out[i * stride] = m[i];
+}
+
+// points and centroid of distinct tetrahedron u of side `which` into registers
+__device__ __forceinline__ void tp_load(const FarTpShared &sh, int which, int u, double *x, double *c)
+{
+#pragma unroll
+  for (int e = 0; e < 16; e++) x[e] = sh.pts[which][e][u];
+#pragma unroll
+  for (int d = 0; d < 3; d++) c[d] = sh.cen[which][d][u];
+}
+
+template <bool REALK>
+__device__ void far_tp_tile(const DevGeom &g, double kre, double kim, cd *M, int64_t ld, int ti, int tj,
+                            FarTpShared &sh)
+{
+  const int tid = threadIdx.x, nth = blockDim.x, N = g.nbf;
+  const int *tab[2] = {g.ftTab + (size_t)ti * FTAB, g.ftTab + (size_t)tj * FTAB};
+  const int nr = tab[0][4 * FT2], nc = tab[1][4 * FT2];
+  for (int idx = tid; idx < 2 * 2 * FT2; idx += nth) {
+    const int which = idx / (2 * FT2), k = idx % (2 * FT2);
+    sh.slot[which][k] = tab[which][2 * FT2 + k];
+  }
+  // quadrature points of the distinct tetrahedra
+  for (int idx = tid; idx < 2 * 2 * FT2 * 16; idx += nth) {
+    const int which = idx / (2 * FT2 * 16), rem = idx % (2 * FT2 * 16);
+    const int u = rem / 16, e = rem % 16;
+    if (u < (which ? nc : nr)) sh.pts[which][e][u] = g.q4[(size_t)tab[which][u] * 16 + e];
   }
   __syncthreads();
-  const int a = ti * TILE + (tid & 63);
-  if (a >= N) return;
-  double rp[8][7];
-#pragma unroll
-  for (int pt = 0; pt < 8; pt++) bf_points4(g, a, pt, rp[pt]);
-  int va[5];
-#pragma unroll
-  for (int i = 0; i < 5; i++) va[i] = g.bfV[5 * a + i];
-  // k^2 and 1/k^2
+  for (int idx = tid; idx < 2 * 2 * FT2; idx += nth) {
+    const int which = idx / (2 * FT2), u = idx % (2 * FT2);
+    if (u < (which ? nc : nr))
+      for (int d = 0; d < 3; d++)
+        sh.cen[which][d][u] = 0.25 * (sh.pts[which][d][u] + sh.pts[which][4 + d][u] + sh.pts[which][8 + d][u] +
+                                      sh.pts[which][12 + d][u]);
+  }
+  __syncthreads();
+  // moments of every (row tet, col tet) pair; a tile with more pairs than
+  // fit in LDS computes each needed pair on the fly in the combination below
+  const bool direct = nr * nc > FTT_MAX;
+  if (!direct) {
+    for (int pidx = tid; pidx < nr * nc; pidx += nth) {
+      const int us = pidx / nc, ut = pidx % nc;
+      double xs[16], cS[3], ys[16], cT[3];
+      tp_load(sh, 0, us, xs, cS);
+      tp_load(sh, 1, ut, ys, cT);
+      tp_moments<REALK>(xs, cS, ys, cT, kre, kim, &sh.mom[0][pidx], FTT_MAX);
+    }
+  }
+  __syncthreads();
+  // 1/k^2
   const double k2re = kre * kre - kim * kim, k2im = 2.0 * kre * kim;
   const double ik2d = k2re * k2re + k2im * k2im;
   const double ik2re = k2re / ik2d, ik2im = -k2im / ik2d;
-  const int b0 = (tid >> 6) * 16;
-  for (int jj = 0; jj < 16; jj++) {
-    const int bl = b0 + jj, b = tj * TILE + bl;
-    if (b >= N || b < a) continue;
-    if (bf_ncv(va, cverts[bl]) > 0) continue;
+  // lanes run along the column BF so that moment reads are consecutive
+  for (int pidx = tid; pidx < FT2 * FT2; pidx += nth) {
+    const int lb = pidx % FT2, la = pidx / FT2;
+    const int a = ti * FT2 + la, b = tj * FT2 + lb;
+    if (a >= N || b >= N || b < a) continue;
+    if (bf_ncv(g.bfV + 5 * a, g.bfV + 5 * b) > 0) continue;
     double accr = 0.0, acci = 0.0;
-#pragma unroll 2
-    for (int ip = 0; ip < 8; ip++) {
-#pragma unroll
-      for (int jp = 0; jp < 8; jp++) {
-        const double *c = cpts[bl][jp];
-        const double R0 = rp[ip][0] - c[0], R1 = rp[ip][1] - c[1], R2 = rp[ip][2] - c[2];
-        const double r = sqrt(R0 * R0 + R1 * R1 + R2 * R2);
-        const double inv = 1.0 / r;
-        const double dot = rp[ip][3] * c[3] + rp[ip][4] * c[4] + rp[ip][5] * c[5];
-        const double dd = rp[ip][6] * c[6];
-        double s, co;
-        sincos(kre * r, &s, &co);
-        if (REALK) {
-          const double pe = (dot - dd * ik2re) * inv;
-          accr += pe * co; acci += pe * s;
+    for (int sa = 0; sa < 2; sa++) {
+      const int us = sh.slot[0][2 * la + sa];
+      const double pfa = g.bfPF[2 * a + sa];
+      const double *Qa = g.bfQ + 6 * a + 3 * sa;
+      const double al0 = Qa[0] - sh.cen[0][0][us], al1 = Qa[1] - sh.cen[0][1][us], al2 = Qa[2] - sh.cen[0][2][us];
+      for (int sb = 0; sb < 2; sb++) {
+        const int ut = sh.slot[1][2 * lb + sb];
+        const double pfb = g.bfPF[2 * b + sb];
+        const double *Qb = g.bfQ + 6 * b + 3 * sb;
+        const double be0 = Qb[0] - sh.cen[1][0][ut], be1 = Qb[1] - sh.cen[1][1][ut], be2 = Qb[2] - sh.cen[1][2][ut];
+        double m[16];
+        if (direct) {
+          double xs[16], cS[3], ys[16], cT[3];
+          tp_load(sh, 0, us, xs, cS);
+          tp_load(sh, 1, ut, ys, cT);
+          tp_moments<REALK>(xs, cS, ys, cT, kre, kim, m, 1);
         } else {
-          const double amp = exp(-kim * r) * inv;
-          const double per = dot - dd * ik2re, pei = -dd * ik2im;
-          const double er = amp * co, ei = amp * s;
-          accr += per * er - pei * ei; acci += per * ei + pei * er;
+#pragma unroll
+          for (int i = 0; i < 16; i++) m[i] = sh.mom[i][us * nc + ut];
         }
+        const double ab = al0 * be0 + al1 * be1 + al2 * be2;
+        // Ixy - b.Ix - a.Iy + (a.b) I0
+        double dr = m[14] - (be0 * m[2] + be1 * m[4] + be2 * m[6]) - (al0 * m[8] + al1 * m[10] + al2 * m[12]) + ab * m[0];
+        double di = m[15] - (be0 * m[3] + be1 * m[5] + be2 * m[7]) - (al0 * m[9] + al1 * m[11] + al2 * m[13]) + ab * m[1];
+        // - 9 I0 / k^2
+        dr -= 9.0 * (m[0] * ik2re - m[1] * ik2im);
+        di -= 9.0 * (m[0] * ik2im + m[1] * ik2re);
+        const double f = pfa * pfb;
+        accr += f * dr;
+        acci += f * di;
       }
     }
     const cd v = {accr / FOURPI, acci / FOURPI};
@@ -652,13 +764,13 @@ __device__ __forceinline__ void far_tile(const DevGeom &g, double kre, double ki
 }
 
 template <bool REALK>
-__global__ __launch_bounds__(256) void k_far(DevGeom g, double kre, double kim, cd *M, int64_t ld, int ntiles)
+__global__ __launch_bounds__(FT_THREADS) void k_far_tp(DevGeom g, double kre, double kim, cd *M, int64_t ld,
+                                                       int ntiles)
 {
-  __shared__ double cpts[TILE][8][7];
-  __shared__ int cverts[TILE][5];
+  __shared__ FarTpShared sh;
   int ti, tj;
   tile_coords(blockIdx.x, ntiles, ti, tj);
-  far_tile<REALK>(g, kre, kim, M, ld, ti, tj, cpts, cverts);
+  far_tp_tile<REALK>(g, kre, kim, M, ld, ti, tj, sh);
 }
 
 // FP32 variant of the far-field tile (the precision form's far field):
@@ -887,11 +999,14 @@ static int assemble_vie(bfa_geometry *g, double omega_re, double omega_im, const
     else
       hipLaunchKernelGGL(k_far32<false>, dim3((unsigned)nblk), dim3(256), 0, st, g->dg, omega_re, omega_im, M, ld,
                          ntiles);
-  } else if (omega_im == 0.0) {
-    hipLaunchKernelGGL(k_far<true>, dim3((unsigned)nblk), dim3(256), 0, st, g->dg, omega_re, 0.0, M, ld, ntiles);
   } else {
-    hipLaunchKernelGGL(k_far<false>, dim3((unsigned)nblk), dim3(256), 0, st, g->dg, omega_re, omega_im, M, ld,
-                       ntiles);
+    const int nt2 = (N + FT2 - 1) / FT2;
+    const int64_t nblk2 = (int64_t)nt2 * (nt2 + 1) / 2;
+    if (omega_im == 0.0)
+      hipLaunchKernelGGL(k_far_tp<true>, dim3((unsigned)nblk2), dim3(FT_THREADS), 0, st, g->dg, omega_re, 0.0, M, ld, nt2);
+    else
+      hipLaunchKernelGGL(k_far_tp<false>, dim3((unsigned)nblk2), dim3(FT_THREADS), 0, st, g->dg, omega_re, omega_im, M, ld,
+                         nt2);
   }
   const int64_t nnz = g->dg.nnz;
   hipLaunchKernelGGL(k_near, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st, g->dg, omega_re, omega_im,
@@ -1012,16 +1127,127 @@ __device__ double emt_tetpair(const DevGeom &g, int ta, double pfa, const double
   return acc;
 }
 
-__global__ __launch_bounds__(256) void k_emt_q(DevGeom g, double omega, double *Q, int64_t ldq)
+// Tetrahedron-pair form of Q for the pairs on the 4-point rule (no shared
+// vertex, |omega| max(r_a, r_b) <= 5): the same regrouping as the far field
+// with the real kernel k Im g / 4pi = k sin(kr) / (4 pi r) (series below
+// |kr| = 0.02); other pairs run emt_tetpair's rule directly.
+__device__ __forceinline__ void emt_moments(const double *xs, const double *cS, const double *ys, const double *cT,
+                                            double k, double *out, int stride)
 {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int N = g.nbf;
-  const int b = (int)(idx / N), a = (int)(idx % N);
-  if (b >= N || a > b) return;
-  const int ncv = bf_ncv(g.bfV + 5 * a, g.bfV + 5 * b);
-  const double kR = fabs(omega) * fmax(g.bfRad[a], g.bfRad[b]);
-  const bool hf = kR > 5.0;
-  const int np = hf ? (ncv > 0 ? 33 : 16) : (ncv > 0 ? 16 : 4);
+  double m[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) m[i] = 0.0;
+#pragma unroll 1
+  for (int p = 0; p < 4; p++) {
+    const double *x = xs + 4 * p;
+    const double xi0 = x[0] - cS[0], xi1 = x[1] - cS[1], xi2 = x[2] - cS[2];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const double *y = ys + 4 * q;
+      const double R0 = x[0] - y[0], R1 = x[1] - y[1], R2 = x[2] - y[2];
+      const double r = sqrt(R0 * R0 + R1 * R1 + R2 * R2), kr = k * r;
+      double im;
+      if (fabs(kr) < 2.0e-2) im = (1.0 - kr * kr / 6.0) * k / FOURPI;
+      else im = sin(kr) / (FOURPI * r);
+      const double gv = x[3] * y[3] * k * im;
+      const double e0 = y[0] - cT[0], e1 = y[1] - cT[1], e2 = y[2] - cT[2];
+      m[0] += gv;
+      m[1] += gv * xi0; m[2] += gv * xi1; m[3] += gv * xi2;
+      m[4] += gv * e0; m[5] += gv * e1; m[6] += gv * e2;
+      m[7] += gv * (xi0 * e0 + xi1 * e1 + xi2 * e2);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 8; i++) out[i * stride] = m[i];
+}
+
+__device__ void emt_tp_tile(const DevGeom &g, double omega, double *Q, int64_t ldq, int ti, int tj, FarTpShared &sh)
+{
+  const int tid = threadIdx.x, nth = blockDim.x, N = g.nbf;
+  const int *tab[2] = {g.ftTab + (size_t)ti * FTAB, g.ftTab + (size_t)tj * FTAB};
+  const int nr = tab[0][4 * FT2], nc = tab[1][4 * FT2];
+  for (int idx = tid; idx < 2 * 2 * FT2; idx += nth) {
+    const int which = idx / (2 * FT2), k = idx % (2 * FT2);
+    sh.slot[which][k] = tab[which][2 * FT2 + k];
+  }
+  for (int idx = tid; idx < 2 * 2 * FT2 * 16; idx += nth) {
+    const int which = idx / (2 * FT2 * 16), rem = idx % (2 * FT2 * 16);
+    const int u = rem / 16, e = rem % 16;
+    if (u < (which ? nc : nr)) sh.pts[which][e][u] = g.q4[(size_t)tab[which][u] * 16 + e];
+  }
+  __syncthreads();
+  for (int idx = tid; idx < 2 * 2 * FT2; idx += nth) {
+    const int which = idx / (2 * FT2), u = idx % (2 * FT2);
+    if (u < (which ? nc : nr))
+      for (int d = 0; d < 3; d++)
+        sh.cen[which][d][u] = 0.25 * (sh.pts[which][d][u] + sh.pts[which][4 + d][u] + sh.pts[which][8 + d][u] +
+                                      sh.pts[which][12 + d][u]);
+  }
+  __syncthreads();
+  const bool direct = nr * nc > FTT_MAX;
+  if (!direct) {
+    for (int pidx = tid; pidx < nr * nc; pidx += nth) {
+      const int us = pidx / nc, ut = pidx % nc;
+      double xs[16], cS[3], ys[16], cT[3];
+      tp_load(sh, 0, us, xs, cS);
+      tp_load(sh, 1, ut, ys, cT);
+      emt_moments(xs, cS, ys, cT, omega, &sh.mom[0][pidx], FTT_MAX);
+    }
+  }
+  __syncthreads();
+  const double ik2 = 9.0 / (omega * omega);
+  for (int pidx = tid; pidx < FT2 * FT2; pidx += nth) {
+    const int lb = pidx % FT2, la = pidx / FT2;
+    const int a = ti * FT2 + la, b = tj * FT2 + lb;
+    if (a >= N || b >= N || b < a) continue;
+    // pairs sharing a vertex: emt_near_pair
+    if (bf_ncv(g.bfV + 5 * a, g.bfV + 5 * b) > 0) continue;
+    double q = 0.0;
+    if (fabs(omega) * fmax(g.bfRad[a], g.bfRad[b]) > 5.0) {
+      for (int sa = 0; sa < 2; sa++)
+        for (int sb = 0; sb < 2; sb++)
+          q += emt_tetpair(g, g.bfT[2 * a + sa], g.bfPF[2 * a + sa], g.bfQ + 6 * a + 3 * sa, g.bfT[2 * b + sb],
+                           g.bfPF[2 * b + sb], g.bfQ + 6 * b + 3 * sb, 16, omega);
+    } else {
+      for (int sa = 0; sa < 2; sa++) {
+        const int us = sh.slot[0][2 * la + sa];
+        const double pfa = g.bfPF[2 * a + sa];
+        const double *Qa = g.bfQ + 6 * a + 3 * sa;
+        const double al0 = Qa[0] - sh.cen[0][0][us], al1 = Qa[1] - sh.cen[0][1][us], al2 = Qa[2] - sh.cen[0][2][us];
+        for (int sb = 0; sb < 2; sb++) {
+          const int ut = sh.slot[1][2 * lb + sb];
+          const double pfb = g.bfPF[2 * b + sb];
+          const double *Qb = g.bfQ + 6 * b + 3 * sb;
+          const double be0 = Qb[0] - sh.cen[1][0][ut], be1 = Qb[1] - sh.cen[1][1][ut],
+                       be2 = Qb[2] - sh.cen[1][2][ut];
+          double m[8];
+          if (direct) {
+            double xs[16], cS[3], ys[16], cT[3];
+            tp_load(sh, 0, us, xs, cS);
+            tp_load(sh, 1, ut, ys, cT);
+            emt_moments(xs, cS, ys, cT, omega, m, 1);
+          } else {
+#pragma unroll
+            for (int i = 0; i < 8; i++) m[i] = sh.mom[i][us * nc + ut];
+          }
+          const double ab = al0 * be0 + al1 * be1 + al2 * be2;
+          q += pfa * pfb *
+               (m[7] - (be0 * m[1] + be1 * m[2] + be2 * m[3]) - (al0 * m[4] + al1 * m[5] + al2 * m[6]) + ab * m[0] -
+                ik2 * m[0]);
+        }
+      }
+    }
+    Q[(size_t)b * ldq + a] = q;
+    Q[(size_t)a * ldq + b] = q;
+  }
+}
+
+// Q for touching pair j of the CSR list (16-point rule, 33 at high kR)
+__device__ __forceinline__ void emt_near_pair(const DevGeom &g, double omega, double *Q, int64_t ldq, int64_t j)
+{
+  const int a = g.pairRow[j], b = g.cols[j];
+  const bool hf = fabs(omega) * fmax(g.bfRad[a], g.bfRad[b]) > 5.0;
+  const int np = hf ? 33 : 16;
   double q = 0.0;
   for (int sa = 0; sa < 2; sa++)
     for (int sb = 0; sb < 2; sb++)
@@ -1029,6 +1255,20 @@ __global__ __launch_bounds__(256) void k_emt_q(DevGeom g, double omega, double *
                        g.bfPF[2 * b + sb], g.bfQ + 6 * b + 3 * sb, np, omega);
   Q[(size_t)b * ldq + a] = q;
   Q[(size_t)a * ldq + b] = q;
+}
+
+__global__ __launch_bounds__(256) void k_emt_near(DevGeom g, double omega, double *Q, int64_t ldq)
+{
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < g.nnz) emt_near_pair(g, omega, Q, ldq, j);
+}
+
+__global__ __launch_bounds__(FT_THREADS) void k_emt_q(DevGeom g, double omega, double *Q, int64_t ldq, int ntiles)
+{
+  __shared__ FarTpShared sh;
+  int ti, tj;
+  tile_coords(blockIdx.x, ntiles, ti, tj);
+  emt_tp_tile(g, omega, Q, ldq, ti, tj, sh);
 }
 
 // per (a, j): y = sum_b Q_ab J_bj ; part = Re(conj(J_aj) y) + |J_aj|^2 Q_aa ; ext = Re(conj(J) <b,E>)
@@ -1080,8 +1320,12 @@ extern "C" int bfa_emt_power(bfa_geometry *g, double omega, const bfa_planewave 
   PWList l;
   memset(&l, 0, sizeof l);
   for (int j = 0; j < npw; j++) l.pw[j] = pw[j];
-  const int64_t tot = (int64_t)N * N;
-  hipLaunchKernelGGL(k_emt_q, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, g->dg, omega, Q, (int64_t)N);
+  const int nt2 = (N + FT2 - 1) / FT2;
+  hipLaunchKernelGGL(k_emt_q, dim3((unsigned)((int64_t)nt2 * (nt2 + 1) / 2)), dim3(FT_THREADS), 0, st, g->dg, omega,
+                     Q, (int64_t)N, nt2);
+  if (g->dg.nnz > 0)
+    hipLaunchKernelGGL(k_emt_near, dim3((unsigned)((g->dg.nnz + 255) / 256)), dim3(256), 0, st, g->dg, omega, Q,
+                       (int64_t)N);
   hipLaunchKernelGGL(k_emt_rows, dim3(nb, npw), dim3(256), 0, st, g->dg, omega, (const double *)Q, (int64_t)N,
                      (const cd *)d_J, ldj, l, npw, parts, parts + (size_t)nb * npw);
   std::vector<double> h((size_t)2 * nb * npw);
@@ -1124,16 +1368,15 @@ __device__ __forceinline__ int find_inst(const AsmInst *__restrict__ d, int n, i
   return lo;
 }
 
-__global__ __launch_bounds__(256) void kg_far(const AsmInst *__restrict__ d, int ni, const FreqState *fs)
+__global__ __launch_bounds__(FT_THREADS) void kg_far(const AsmInst *__restrict__ d, int ni, const FreqState *fs)
 {
-  __shared__ double cpts[TILE][8][7];
-  __shared__ int cverts[TILE][5];
+  __shared__ FarTpShared sh;
   const AsmInst &q = d[find_inst(d, ni, blockIdx.x, GF_FAR)];
   int ti, tj;
   tile_coords(blockIdx.x - q.far0, q.ntiles, ti, tj);
   const double kre = fs->omega_re, kim = fs->omega_im;
-  if (kim == 0.0) far_tile<true>(q.g, kre, 0.0, (cd *)q.M, q.ldm, ti, tj, cpts, cverts);
-  else far_tile<false>(q.g, kre, kim, (cd *)q.M, q.ldm, ti, tj, cpts, cverts);
+  if (kim == 0.0) far_tp_tile<true>(q.g, kre, 0.0, (cd *)q.M, q.ldm, ti, tj, sh);
+  else far_tp_tile<false>(q.g, kre, kim, (cd *)q.M, q.ldm, ti, tj, sh);
 }
 
 __global__ __launch_bounds__(256) void kg_near(const AsmInst *__restrict__ d, int ni, const FreqState *fs)
@@ -1156,26 +1399,20 @@ __global__ __launch_bounds__(256) void kg_rhs(const AsmInst *__restrict__ d, int
   ((cd *)q.B)[(size_t)j * q.ldb + a] = {-v.im * s, v.re * s};
 }
 
-__global__ __launch_bounds__(256) void kg_emt_q(const AsmInst *__restrict__ d, int ni, const FreqState *fs)
+__global__ __launch_bounds__(FT_THREADS) void kg_emt_q(const AsmInst *__restrict__ d, int ni, const FreqState *fs)
 {
+  __shared__ FarTpShared sh;
   const AsmInst &q = d[find_inst(d, ni, blockIdx.x, GF_EMT)];
-  const int N = q.g.nbf;
-  const int64_t idx = (int64_t)(blockIdx.x - q.emt0) * blockDim.x + threadIdx.x;
-  const int b = (int)(idx / N), a = (int)(idx % N);
-  if (b >= N || a > b) return;
-  const DevGeom &g = q.g;
-  const double omega = fs->omega_re;
-  const int ncv = bf_ncv(g.bfV + 5 * a, g.bfV + 5 * b);
-  const double kR = fabs(omega) * fmax(g.bfRad[a], g.bfRad[b]);
-  const bool hf = kR > 5.0;
-  const int np = hf ? (ncv > 0 ? 33 : 16) : (ncv > 0 ? 16 : 4);
-  double v = 0.0;
-  for (int sa = 0; sa < 2; sa++)
-    for (int sb = 0; sb < 2; sb++)
-      v += emt_tetpair(g, g.bfT[2 * a + sa], g.bfPF[2 * a + sa], g.bfQ + 6 * a + 3 * sa, g.bfT[2 * b + sb],
-                       g.bfPF[2 * b + sb], g.bfQ + 6 * b + 3 * sb, np, omega);
-  q.Q[(size_t)b * N + a] = v;
-  q.Q[(size_t)a * N + b] = v;
+  int ti, tj;
+  tile_coords(blockIdx.x - q.emt0, q.ntiles, ti, tj);
+  emt_tp_tile(q.g, fs->omega_re, q.Q, q.g.nbf, ti, tj, sh);
+}
+
+__global__ __launch_bounds__(256) void kg_emt_near(const AsmInst *__restrict__ d, int ni, const FreqState *fs)
+{
+  const AsmInst &q = d[find_inst(d, ni, blockIdx.x, GF_NEAR)];
+  const int64_t j = (int64_t)(blockIdx.x - q.near0) * blockDim.x + threadIdx.x;
+  if (j < q.g.nnz) emt_near_pair(q.g, fs->omega_re, q.Q, q.g.nbf, j);
 }
 
 __global__ __launch_bounds__(256) void kg_emt_rows(const AsmInst *__restrict__ d, int ni, const FreqState *fs,
@@ -1233,7 +1470,7 @@ __global__ __launch_bounds__(64) void kg_emt_final(const AsmInst *__restrict__ d
 namespace bfa {
 void launch_far_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, hipStream_t st)
 {
-  if (nblk > 0) hipLaunchKernelGGL(kg_far, dim3(nblk), dim3(256), 0, st, d, ni, fs);
+  if (nblk > 0) hipLaunchKernelGGL(kg_far, dim3(nblk), dim3(FT_THREADS), 0, st, d, ni, fs);
 }
 void launch_near_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, hipStream_t st)
 {
@@ -1245,7 +1482,11 @@ void launch_rhs_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, const
 }
 void launch_emt_q_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, hipStream_t st)
 {
-  if (nblk > 0) hipLaunchKernelGGL(kg_emt_q, dim3(nblk), dim3(256), 0, st, d, ni, fs);
+  if (nblk > 0) hipLaunchKernelGGL(kg_emt_q, dim3(nblk), dim3(FT_THREADS), 0, st, d, ni, fs);
+}
+void launch_emt_near_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, hipStream_t st)
+{
+  if (nblk > 0) hipLaunchKernelGGL(kg_emt_near, dim3(nblk), dim3(256), 0, st, d, ni, fs);
 }
 void launch_emt_rows_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, const PWBatch &pw, int npw,
                        hipStream_t st)

@@ -704,7 +704,7 @@ std::vector<AsmInst> asm_descs(const bfa_batch *B, bool solution, int &nfar, int
     q.Q = (double *)B->LU32[i];   // the complex64 LU is dead once the solve is done
     q.part = B->at<double>(B->off[i].part);
     const int N = B->N[i];
-    q.ntiles = (N + 63) / 64;
+    q.ntiles = (N + FT2 - 1) / FT2;
     q.far0 = nfar;
     nfar += q.ntiles * (q.ntiles + 1) / 2;
     q.near0 = nnear;
@@ -713,7 +713,7 @@ std::vector<AsmInst> asm_descs(const bfa_batch *B, bool solution, int &nfar, int
     q.nrowblk = (N + 255) / 256;
     nrow += q.nrowblk;
     q.emt0 = nemt;
-    nemt += (int)(((int64_t)N * N + 255) / 256);
+    nemt += q.ntiles * (q.ntiles + 1) / 2;
   }
   return v;
 }
@@ -781,9 +781,10 @@ int build_plan(bfa_batch *B, double tol, int maxit)
   }
   // --- power (the complex64 LU storage becomes the EMT scratch)
   P.phase = 3;
-  P.op([B, o_emt, ni, nemt, nrow, fs, pws, npw](hipStream_t st) {
+  P.op([B, o_emt, ni, nemt, nnear2, nrow, fs, pws, npw](hipStream_t st) {
     const AsmInst *d = dp<AsmInst>(B, o_emt);
     launch_emt_q_g(d, ni, nemt, fs, st);
+    launch_emt_near_g(d, ni, nnear2, fs, st);
     launch_emt_rows_g(d, ni, nrow, fs, pws, npw, st);
     launch_emt_final_g(d, ni, fs, npw, B->power, st);
   });

@@ -97,7 +97,7 @@ struct AsmInst {
   double *Q;             // EMT scratch, N x N doubles, ld = N
   double *part;          // EMT partial sums [2][npw][row blocks]
   int64_t ldm, ldb;
-  int far0, ntiles;      // far-field tiles (upper triangle of 64x64) from far0
+  int far0, ntiles;      // far-field tiles (upper triangle of FT2 x FT2 BF tiles) from far0
   int near0, row0;       // touching-pair blocks (256) / row blocks (256)
   int emt0, nrowblk;     // EMT Q blocks (256 entries of N x N) from emt0
 };
@@ -117,6 +117,7 @@ void launch_near_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, hipS
 void launch_rhs_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, const PWBatch &pw, int npw,
                   hipStream_t st);
 void launch_emt_q_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, hipStream_t st);
+void launch_emt_near_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, hipStream_t st);
 void launch_emt_rows_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, const PWBatch &pw, int npw,
                        hipStream_t st);
 // out[(fidx * ni + i) * npw + j][2] = {P_abs, P_scat}

@@ -12,6 +12,12 @@ namespace bfa {
 // Thread-local error message, set by fail().
 int fail(int status, const char *fmt, ...);
 
+// BF tile edge of the tetrahedron-pair far-field kernel and the stride of
+// its per-tile table: [2*FT2] distinct tetrahedra (-1 padded), [2*FT2] slot
+// of BF (local l, side s) at 2l+s in that list, then the count.
+#define FT2 32
+#define FTAB (4 * FT2 + 4)
+
 // Device-side view of the geometry (all pointers are device pointers).
 // Cubature tables hold, per tetrahedron and point, {x, y, z, w} with
 // w = 6 * Vol_signed * w_rule (src/libs/libbuff/Cubature.cc:201-218).
@@ -33,6 +39,7 @@ struct DevGeom {
   const int *rowptr;      // [nbf+1]   touching pairs CSR
   const int *cols;        // [nnz]
   const int *pairRow;     // [nnz]     row of each CSR entry
+  const int *ftTab;       // [ceil(nbf/FT2)][FTAB] far-field tile tetrahedra
 };
 
 // Device buffers owned by a geometry handle.
@@ -40,7 +47,7 @@ struct DevBuffers {
   void *V = nullptr, *tetVI = nullptr, *tetFI = nullptr, *tetVol = nullptr;
   void *q4 = nullptr, *q16 = nullptr, *q33 = nullptr;
   void *bfV = nullptr, *bfT = nullptr, *bfPF = nullptr, *bfQ = nullptr, *bfRad = nullptr;
-  void *rowptr = nullptr, *cols = nullptr, *pairRow = nullptr;
+  void *rowptr = nullptr, *cols = nullptr, *pairRow = nullptr, *ftTab = nullptr;
   // static (cold) data
   void *tdPairs = nullptr;     // [ntd] int4 {tA, tB, ncv, 0}
   void *tdVerts = nullptr;     // [ntd][24] double: VA(4x3), VB(4x3) ordered

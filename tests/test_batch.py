@@ -76,9 +76,12 @@ def test_batch_matches_single_fp64(bfa, torch_cuda, batch_setup):
             rel = np.abs(r["power"][f, i] - P) / np.abs(P)
             assert rel.max() < 1e-9, (f, i, rel, r["power"][f, i], P)
             if f == len(OMEGAS) - 1:
-                # the batch's complex128 matrix is the single path's, entry for entry
+                # the batch's complex128 matrix is the single path's: same rule and
+                # summation order; the two kernels' code may contract multiply-adds
+                # differently, so entries agree to the last bits
                 Mb = b.matrix(i)
-                assert np.array_equal(Mb, Mh), np.abs(Mb - Mh).max()
+                d = np.abs(Mb - Mh).max()
+                assert d <= 1e-14 * np.abs(Mh).max(), d
                 Jb = b.solution(i)
                 assert np.linalg.norm(Jb - J) / np.linalg.norm(J) < 1e-9
 
