@@ -145,6 +145,7 @@ extern "C" int bfa_lu_solve_c(const void *d_A, int n, int64_t lda, const int *d_
   void *scratch = nullptr;
   HIPCHECK(hipMalloc(&scratch, bfa::lu_solve_scratch_bytes(n, nrhs, 8)));
   int rc = bfa::lu_solve_c(d_A, n, lda, d_ipiv, d_B, nrhs, ldb, (hipStream_t)stream, scratch);
+  if (rc == BFA_OK) rc = bfa::lu_solve_check(scratch, n, (hipStream_t)stream);
   hipError_t e = hipStreamSynchronize((hipStream_t)stream);
   (void)hipFree(scratch);
   if (rc) return rc;
@@ -193,7 +194,8 @@ extern "C" int bfa_solve_refine(const void *d_M64, int64_t ldm, const void *d_LU
     (void)hipFree(w);
     return fail(BFA_ERR_HIP, "bfa_solve_refine setup: %s", hipGetErrorString(e));
   }
-  for (it = 0; it <= maxit; it++) {
+  rc = bfa::lu_solve_prep_c(d_LU32, n, ldlu, d_ipiv, st, scratch);
+  for (it = 0; rc == BFA_OK && it <= maxit; it++) {
     if (it > 0) {
       // r = b - M64 x, in chunks of at most RES_MAXRHS right-hand sides
       for (int j0 = 0; j0 < nrhs; j0 += RES_MAXRHS) {
@@ -220,11 +222,12 @@ extern "C" int bfa_solve_refine(const void *d_M64, int64_t ldm, const void *d_LU
     }
     // d = LU32^-1 r (complex64), x += d
     hipLaunchKernelGGL(k_z2c, dim3(nblk, nrhs), dim3(256), 0, st, (const z2 *)R, (int64_t)n, Dd, (int64_t)n, n, nrhs);
-    rc = bfa::lu_solve_c(d_LU32, n, ldlu, d_ipiv, Dd, nrhs, n, st, scratch);
+    rc = bfa::lu_solve_apply_c(d_LU32, n, ldlu, Dd, nrhs, n, st, scratch);
     if (rc) break;
     hipLaunchKernelGGL(k_c2z_add, dim3(nblk, nrhs), dim3(256), 0, st, (const c2 *)Dd, (int64_t)n, X, (int64_t)n, n,
                        nrhs);
   }
+  if (rc == BFA_OK) rc = bfa::lu_solve_check(scratch, n, st);
   if (rc == BFA_OK) {
     e = hipMemcpy2DAsync(d_B, ldb * sizeof(z2), X, n * sizeof(z2), n * sizeof(z2), nrhs, hipMemcpyDeviceToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);

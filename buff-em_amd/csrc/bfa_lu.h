@@ -18,4 +18,10 @@ int lu_solve_z(const void *A, int n, int64_t lda, const int *ipiv, void *B, int 
                void *scratch);
 int lu_solve_c(const void *A, int n, int64_t lda, const int *ipiv, void *B, int nrhs, int64_t ldb, hipStream_t st,
                void *scratch);
+// the same solve in two steps: permutation and inverted diagonal blocks
+// once per factorisation, then any number of applications (complex64)
+int lu_solve_prep_c(const void *A, int n, int64_t lda, const int *ipiv, hipStream_t st, void *scratch);
+int lu_solve_apply_c(const void *A, int n, int64_t lda, void *B, int nrhs, int64_t ldb, hipStream_t st, void *scratch);
+// nonzero timeout flag of the streaming solves that used scratch -> error
+int lu_solve_check(const void *scratch, int n, hipStream_t st);
 }  // namespace bfa
