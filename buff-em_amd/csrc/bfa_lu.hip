@@ -1041,11 +1041,13 @@ __global__ __launch_bounds__(256) void kg_trsv_update(const TrsvProb *__restrict
 // (unit) and U are inverted once per factorisation (k_trsv_inv); one launch
 // per triangle then solves every row block: a workgroup takes a ticket t
 // (row block t of L, or nblk-1-t of U), accumulates the updates of the
-// blocks published before it as they appear, applies its inverted diagonal
-// block and publishes.  Solved blocks and the published-block counter are
-// exchanged with agent-scope atomic stores and loads (coherent across the
-// XCDs, no cache maintenance); a ticket only ever waits on lower tickets,
-// which belong to workgroups already running.
+// blocks solved before it as they appear, applies its inverted diagonal
+// block and stores its solution.  The output starts filled with a
+// signalling-NaN tag that arithmetic never produces; solved values are
+// stored and polled with agent-scope atomics (coherent across the XCDs, no
+// cache maintenance, no separate flag), so a block is visible as soon as
+// its values land.  A ticket only ever waits on lower tickets, which belong
+// to workgroups already running.
 template <typename T>
 __global__ __launch_bounds__(256) void k_trsv_inv(int n, const cplx<T> *__restrict__ A, int64_t lda, cplx<T> *inv)
 {
@@ -1080,34 +1082,91 @@ __global__ __launch_bounds__(256) void k_trsv_inv(int n, const cplx<T> *__restri
   for (int idx = tid; idx < SV_KB * SV_KB; idx += blockDim.x) o[idx] = X[idx % SV_KB][idx / SV_KB];
 }
 
+template <typename T> struct SvTag;
+template <> struct SvTag<float> {
+  typedef unsigned long long W;   // re and im stored together
+  static constexpr unsigned bits = 0x7FA5A5A5u;
+};
+template <> struct SvTag<double> {
+  typedef unsigned long long W;
+  static constexpr unsigned long long bits = 0x7FF4A5A5A5A5A5A5ull;
+};
+
+// agent-scope store / load of one complex value with tag detection
+__device__ __forceinline__ void sv_store(cplx<float> *p, cplx<float> v)
+{
+  const unsigned long long w =
+      (unsigned long long)__float_as_uint(v.re) | ((unsigned long long)__float_as_uint(v.im) << 32);
+  st_agent((unsigned long long *)p, w);
+}
+__device__ __forceinline__ void sv_store(cplx<double> *p, cplx<double> v)
+{
+  st_agent((unsigned long long *)p, (unsigned long long)__double_as_longlong(v.re));
+  st_agent((unsigned long long *)p + 1, (unsigned long long)__double_as_longlong(v.im));
+}
+__device__ __forceinline__ bool sv_load(const cplx<float> *p, cplx<float> &v)
+{
+  const unsigned long long w = ld_agent((const unsigned long long *)p);
+  const unsigned lo = (unsigned)w, hi = (unsigned)(w >> 32);
+  v = {__uint_as_float(lo), __uint_as_float(hi)};
+  return lo != SvTag<float>::bits && hi != SvTag<float>::bits;
+}
+__device__ __forceinline__ bool sv_load(const cplx<double> *p, cplx<double> &v)
+{
+  const unsigned long long a = ld_agent((const unsigned long long *)p);
+  const unsigned long long b = ld_agent((const unsigned long long *)p + 1);
+  v = {__longlong_as_double((long long)a), __longlong_as_double((long long)b)};
+  return a != SvTag<double>::bits && b != SvTag<double>::bits;
+}
+
+// fill two n x nrhs outputs with the tag, clear the tickets
+template <typename T>
+__global__ void k_sv_reset(cplx<T> *Y, cplx<T> *X, size_t count, unsigned *tickets, int nt)
+{
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) {
+    if (sizeof(T) == 4) {
+      const unsigned long long w = (unsigned long long)SvTag<float>::bits * 0x100000001ull;
+      ((unsigned long long *)Y)[i] = w;
+      ((unsigned long long *)X)[i] = w;
+    } else {
+      ((unsigned long long *)Y)[2 * i] = ((unsigned long long *)Y)[2 * i + 1] = (unsigned long long)SvTag<double>::bits;
+      ((unsigned long long *)X)[2 * i] = ((unsigned long long *)X)[2 * i + 1] = (unsigned long long)SvTag<double>::bits;
+    }
+  }
+  if (i < (size_t)nt) tickets[i] = 0u;
+}
+
 template <typename T> struct StreamShared {
   cplx<T> xs[SV_RHS][SV_KB];
   cplx<T> part[4][SV_RHS][SV_KB];
-  int avail, ticket;
+  int ticket;
 };
 
-// ctr[0]: ticket counter, ctr[1]: blocks published, ctr[2]: timeout flag
+// Bin: right-hand sides (n x nrhs, read-only here); Xo: tag-filled output.
+// ticket: this launch's counter; flag: set when a wait gives up.
 template <typename T, bool LOWER>
 __global__ __launch_bounds__(256) void k_trsv_stream(int n, int nrhs, const cplx<T> *__restrict__ A, int64_t lda,
-                                                     const cplx<T> *__restrict__ inv, cplx<T> *X, int64_t ldx,
-                                                     unsigned *ctr)
+                                                     const cplx<T> *__restrict__ inv, const cplx<T> *Bin,
+                                                     cplx<T> *Xo, int64_t ldx, unsigned *ticket, unsigned *flag)
 {
   __shared__ StreamShared<T> sh;
   const int nblk = (n + SV_KB - 1) / SV_KB;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  if (tid == 0) {
-    sh.ticket = (int)__hip_atomic_fetch_add(&ctr[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    sh.avail = 0;
-  }
+  if (tid == 0) sh.ticket = (int)__hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
   const int t = sh.ticket;
   const int bi = LOWER ? t : nblk - 1 - t;
   const int r0 = bi * SV_KB, kb = min(SV_KB, n - r0);
   const int r = r0 + lane;
+  // this wave's columns of the inverted diagonal block, loaded up front
+  const cplx<T> *Di = inv + ((size_t)(LOWER ? 0 : nblk) + bi) * SV_KB * SV_KB;
+  cplx<T> dcol[SV_KB / 4];
+#pragma unroll
+  for (int q = 0; q < SV_KB / 4; q++) dcol[q] = Di[(size_t)(wv + 4 * q) * SV_KB + lane];
   T ar[SV_RHS], ai[SV_RHS];
 #pragma unroll
   for (int j = 0; j < SV_RHS; j++) { ar[j] = 0; ai[j] = 0; }
-  int avail = 0;
   for (int s = 0; s < t; s++) {
     const int k = LOWER ? s : nblk - 1 - s;
     const int c0 = k * SV_KB, cb = min(SV_KB, n - c0);
@@ -1118,21 +1177,17 @@ __global__ __launch_bounds__(256) void k_trsv_stream(int n, int nrhs, const cplx
       const int c = wv + 4 * q;
       l[q] = (lane < kb && c < cb) ? A[(size_t)(c0 + c) * lda + r] : cplx<T>{0, 0};
     }
-    if (s >= avail) {
-      if (tid == 0) {
-        unsigned v, spins = 0;
-        while ((v = ld_agent(&ctr[1])) <= (unsigned)s) {
-          if (++spins > SPIN_LIMIT) { atomicOr(&ctr[2], 1u); v = (unsigned)nblk; break; }
+    // wave 0 polls x_k until every value has landed
+    if (wv == 0) {
+      unsigned spins = 0;
+      for (int idx = lane; idx < cb * nrhs; idx += 64) {
+        const int c = idx % cb, j = idx / cb;
+        cplx<T> v;
+        while (!sv_load(&Xo[(size_t)j * ldx + c0 + c], v)) {
+          if (++spins > SPIN_LIMIT) { atomicOr(flag, 1u); break; }
         }
-        sh.avail = (int)v;
+        sh.xs[j][c] = v;
       }
-      __syncthreads();
-      avail = sh.avail;
-    }
-    for (int idx = tid; idx < cb * nrhs; idx += blockDim.x) {
-      const int c = idx % cb, j = idx / cb;
-      const cplx<T> *src = &X[(size_t)j * ldx + c0 + c];
-      sh.xs[j][c] = {ld_agent(&src->re), ld_agent(&src->im)};
     }
     __syncthreads();
 #pragma unroll
@@ -1151,13 +1206,12 @@ __global__ __launch_bounds__(256) void k_trsv_stream(int n, int nrhs, const cplx
   }
   for (int j = 0; j < nrhs; j++) sh.part[wv][j][lane] = {ar[j], ai[j]};
   __syncthreads();
-  // raw right-hand side of this block: b - sum of updates (rows of X
-  // owned by this workgroup alone, written by an earlier launch)
+  // raw right-hand side of this block: b - sum of updates
   if (wv == 0) {
     for (int j = 0; j < nrhs; j++) {
       cplx<T> v = {0, 0};
       if (lane < kb) {
-        const cplx<T> b = X[(size_t)j * ldx + r];
+        const cplx<T> b = Bin[(size_t)j * ldx + r];
         v.re = b.re - (sh.part[0][j][lane].re + sh.part[1][j][lane].re + sh.part[2][j][lane].re + sh.part[3][j][lane].re);
         v.im = b.im - (sh.part[0][j][lane].im + sh.part[1][j][lane].im + sh.part[2][j][lane].im + sh.part[3][j][lane].im);
       }
@@ -1166,13 +1220,12 @@ __global__ __launch_bounds__(256) void k_trsv_stream(int n, int nrhs, const cplx
   }
   __syncthreads();
   // x = inv(D) * raw, D the diagonal block
-  const cplx<T> *Di = inv + ((size_t)(LOWER ? 0 : nblk) + bi) * SV_KB * SV_KB;
 #pragma unroll
   for (int j = 0; j < SV_RHS; j++) { ar[j] = 0; ai[j] = 0; }
-#pragma unroll 4
+#pragma unroll
   for (int q = 0; q < SV_KB / 4; q++) {
     const int c = wv + 4 * q;
-    const cplx<T> d = Di[(size_t)c * SV_KB + lane];
+    const cplx<T> d = dcol[q];
 #pragma unroll
     for (int j = 0; j < SV_RHS; j++) {
       if (j < nrhs) {
@@ -1184,22 +1237,17 @@ __global__ __launch_bounds__(256) void k_trsv_stream(int n, int nrhs, const cplx
   }
   for (int j = 0; j < nrhs; j++) sh.part[wv][j][lane] = {ar[j], ai[j]};
   __syncthreads();
-  if (wv == 0) {
-    if (lane < kb) {
-      for (int j = 0; j < nrhs; j++) {
-        const T xr = sh.part[0][j][lane].re + sh.part[1][j][lane].re + sh.part[2][j][lane].re + sh.part[3][j][lane].re;
-        const T xi = sh.part[0][j][lane].im + sh.part[1][j][lane].im + sh.part[2][j][lane].im + sh.part[3][j][lane].im;
-        cplx<T> *d = &X[(size_t)j * ldx + r];
-        st_agent(&d->re, xr);
-        st_agent(&d->im, xi);
-      }
+  if (wv == 0 && lane < kb) {
+    for (int j = 0; j < nrhs; j++) {
+      const T xr = sh.part[0][j][lane].re + sh.part[1][j][lane].re + sh.part[2][j][lane].re + sh.part[3][j][lane].re;
+      const T xi = sh.part[0][j][lane].im + sh.part[1][j][lane].im + sh.part[2][j][lane].im + sh.part[3][j][lane].im;
+      sv_store(&Xo[(size_t)j * ldx + r], cplx<T>{xr, xi});
     }
-    __builtin_amdgcn_s_waitcnt(0);   // the block's stores (all by this wave) have completed
-    if (lane == 0) st_agent(&ctr[1], (unsigned)(t + 1));
   }
 }
 
-// scratch: [perm n ints][counters 256 B][inverted diagonal blocks][X n x nrhs]
+// scratch: [perm n ints][flag 256 B][inverted diagonal blocks]
+//          [P b, Y, X: n x nrhs each][tickets]
 static size_t sv_align(size_t x) { return (x + 255) / 256 * 256; }
 static size_t sv_inv_off(int n) { return sv_align((size_t)n * 4) + 256; }
 static size_t sv_x_off(int n, size_t elem)
@@ -1207,6 +1255,7 @@ static size_t sv_x_off(int n, size_t elem)
   const size_t nblk = (size_t)(n + SV_KB - 1) / SV_KB;
   return sv_inv_off(n) + sv_align(2 * nblk * SV_KB * SV_KB * elem);
 }
+static int sv_nticket(int nrhs) { return 2 * ((nrhs + SV_RHS - 1) / SV_RHS); }
 
 // Row permutation from the pivots and the inverted diagonal blocks: once
 // per factorisation.
@@ -1228,7 +1277,7 @@ int lu_solve_prep(const cplx<T> *A, int n, int64_t lda, const int *ipiv, hipStre
   }
   const int nblk = (n + SV_KB - 1) / SV_KB;
   cplx<T> *inv = (cplx<T> *)((char *)scratch + sv_inv_off(n));
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, (unsigned *)((char *)scratch + sv_align((size_t)n * 4)), 3);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, (unsigned *)((char *)scratch + sv_align((size_t)n * 4)), 1);
   hipLaunchKernelGGL(k_trsv_inv<T>, dim3(nblk, 2), dim3(256), 0, st, n, A, lda, inv);
   HIPCHECK(hipGetLastError());
   return BFA_OK;
@@ -1240,21 +1289,27 @@ int lu_solve_apply(const cplx<T> *A, int n, int64_t lda, cplx<T> *B, int nrhs, i
                    void *scratch)
 {
   const int *perm = (const int *)scratch;
-  unsigned *ctr = (unsigned *)((char *)scratch + sv_align((size_t)n * 4));
+  unsigned *flag = (unsigned *)((char *)scratch + sv_align((size_t)n * 4));
   const cplx<T> *inv = (const cplx<T> *)((char *)scratch + sv_inv_off(n));
-  cplx<T> *X = (cplx<T> *)((char *)scratch + sv_x_off(n, sizeof(cplx<T>)));
-  hipLaunchKernelGGL(k_gather_rows<T>, dim3((n + 255) / 256, nrhs), dim3(256), 0, st, B, ldb, X, (int64_t)n, n, nrhs,
+  const size_t vec = sv_align((size_t)n * nrhs * sizeof(cplx<T>));
+  cplx<T> *Pb = (cplx<T> *)((char *)scratch + sv_x_off(n, sizeof(cplx<T>)));
+  cplx<T> *Y = (cplx<T> *)((char *)Pb + vec);
+  cplx<T> *X = (cplx<T> *)((char *)Y + vec);
+  unsigned *tickets = (unsigned *)((char *)X + vec);
+  const int nt = sv_nticket(nrhs);
+  const size_t count = (size_t)n * nrhs;
+  hipLaunchKernelGGL(k_gather_rows<T>, dim3((n + 255) / 256, nrhs), dim3(256), 0, st, B, ldb, Pb, (int64_t)n, n, nrhs,
                      perm);
+  hipLaunchKernelGGL(k_sv_reset<T>, dim3((unsigned)((std::max(count, (size_t)nt) + 255) / 256)), dim3(256), 0, st, Y,
+                     X, count, tickets, nt);
   const int nblk = (n + SV_KB - 1) / SV_KB;
-  for (int j0 = 0; j0 < nrhs; j0 += SV_RHS) {
+  for (int j0 = 0, c = 0; j0 < nrhs; j0 += SV_RHS, c++) {
     const int nj = std::min(SV_RHS, nrhs - j0);
-    cplx<T> *Xj = X + (size_t)j0 * n;
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, ctr, 2);
-    hipLaunchKernelGGL((k_trsv_stream<T, true>), dim3(nblk), dim3(256), 0, st, n, nj, A, lda, inv, Xj, (int64_t)n,
-                       ctr);
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, ctr, 2);
-    hipLaunchKernelGGL((k_trsv_stream<T, false>), dim3(nblk), dim3(256), 0, st, n, nj, A, lda, inv, Xj, (int64_t)n,
-                       ctr);
+    const size_t o = (size_t)j0 * n;
+    hipLaunchKernelGGL((k_trsv_stream<T, true>), dim3(nblk), dim3(256), 0, st, n, nj, A, lda, inv,
+                       (const cplx<T> *)(Pb + o), Y + o, (int64_t)n, tickets + 2 * c, flag);
+    hipLaunchKernelGGL((k_trsv_stream<T, false>), dim3(nblk), dim3(256), 0, st, n, nj, A, lda, inv,
+                       (const cplx<T> *)(Y + o), X + o, (int64_t)n, tickets + 2 * c + 1, flag);
   }
   HIPCHECK(hipMemcpy2DAsync(B, ldb * sizeof(cplx<T>), X, n * sizeof(cplx<T>), n * sizeof(cplx<T>), nrhs,
                             hipMemcpyDeviceToDevice, st));
@@ -1273,14 +1328,14 @@ int lu_solve(const cplx<T> *A, int n, int64_t lda, const int *ipiv, cplx<T> *B, 
 
 size_t lu_solve_scratch_bytes(int n, int nrhs, size_t elem)
 {
-  return sv_x_off(n, elem) + (size_t)n * nrhs * elem;
+  return sv_x_off(n, elem) + 3 * sv_align((size_t)n * nrhs * elem) + sv_align((size_t)sv_nticket(nrhs) * 4);
 }
 
 // timeout flag of the streaming solves (nonzero: a wait gave up)
 int lu_solve_check(const void *scratch, int n, hipStream_t st)
 {
   unsigned f = 0;
-  HIPCHECK(hipMemcpyAsync(&f, (const char *)scratch + sv_align((size_t)n * 4) + 8, 4, hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipMemcpyAsync(&f, (const char *)scratch + sv_align((size_t)n * 4), 4, hipMemcpyDeviceToHost, st));
   HIPCHECK(hipStreamSynchronize(st));
   return f ? fail(BFA_ERR_HIP, "triangular solve: dependency wait timed out") : BFA_OK;
 }
