@@ -73,6 +73,9 @@ struct SolveInst {
   const int *ipiv;
   int *permw;           // writable perm
   int *flags;           // LU flags [2]
+  c2 *Y, *Xs;           // streaming-sweep outputs (tag-filled each round)
+  unsigned *tick;       // [2] sweep tickets
+  unsigned long long tag;
 };
 
 __device__ __forceinline__ int sfirst(const SolveInst &q, int which)
@@ -150,21 +153,25 @@ __global__ __launch_bounds__(256) void kg_refine_init(const SolveInst *__restric
   }
 }
 
-// D(i, j) = complex64(R(perm[i], j))
+// D(i, j) = complex64(R(perm[i], j)); the sweep outputs Y, Xs are filled
+// with the tag and the sweep tickets cleared
 __global__ __launch_bounds__(256) void kg_gather(const SolveInst *__restrict__ d, int ni, int nrhs)
 {
   const SolveInst &q = d[find_solve(d, ni, blockIdx.x, 0)];
   if (*q.conv) return;
   const int r = (blockIdx.x - q.row0) * blockDim.x + threadIdx.x;
+  if (r == 0) { q.tick[0] = 0u; q.tick[1] = 0u; }
   if (r >= q.Np) return;
   const int s = q.perm[r];
   for (int j = 0; j < nrhs; j++) {
     const z2 v = q.R[(size_t)j * q.ld + s];
     q.D[(size_t)j * q.ld + r] = {(float)v.re, (float)v.im};
+    ((unsigned long long *)q.Y)[(size_t)j * q.ld + r] = q.tag;
+    ((unsigned long long *)q.Xs)[(size_t)j * q.ld + r] = q.tag;
   }
 }
 
-// X += D on rows < N
+// X += (LU)^-1 D (the backward sweep's output) on rows < N
 __global__ __launch_bounds__(256) void kg_add(const SolveInst *__restrict__ d, int ni, int nrhs)
 {
   const SolveInst &q = d[find_solve(d, ni, blockIdx.x, 0)];
@@ -172,7 +179,7 @@ __global__ __launch_bounds__(256) void kg_add(const SolveInst *__restrict__ d, i
   const int r = (blockIdx.x - q.row0) * blockDim.x + threadIdx.x;
   if (r >= q.N) return;
   for (int j = 0; j < nrhs; j++) {
-    const c2 v = q.D[(size_t)j * q.ld + r];
+    const c2 v = q.Xs[(size_t)j * q.ld + r];
     z2 &x = q.X[(size_t)j * q.ld + r];
     x.re += (double)v.re;
     x.im += (double)v.im;
@@ -280,7 +287,7 @@ struct bfa_batch {
   std::vector<void *> M64, LU32;
   // everything else in one allocation; per-instance byte offsets
   void *small = nullptr;
-  struct Off { size_t ipiv, perm, B, X, R, D, P, norms, part, sval, srow, sdata, sj; };
+  struct Off { size_t ipiv, perm, B, X, R, D, Y, Xs, inv, tick, P, norms, part, sval, srow, sdata, sj; };
   std::vector<Off> off;
   size_t o_fs = 0, o_conv = 0, o_iters = 0, o_resid = 0, o_counters = 0, o_flags = 0;
   size_t o_hist_it = 0, o_hist_res = 0, o_hist_info = 0;
@@ -435,11 +442,14 @@ struct Planner {
 
   int panel_leaf(const std::vector<Pan> &ps, int w)
   {
-    const size_t cap = panel_lds_cap(false);
-    const int maxres = panel_max_resident(false, cap);
-    if (maxres <= 0) return fail(BFA_ERR_HIP, "batch: panel occupancy query failed");
+    // at most ~31 KB of rows per workgroup: more workgroups per panel are
+    // faster even alone (m = 19,872: 256 us per 32 columns at 124 rows per
+    // workgroup, 313 us at 600)
+    const size_t cap = std::min(panel_lds_cap(false), (size_t)31 * 1024);
     const int row_bytes = w * (int)sizeof(c2);
-    const int rmax = (int)(cap / row_bytes);
+    const int rmax = std::max(16, (int)(cap / row_bytes));
+    const int maxres = panel_max_resident(false, (size_t)rmax * row_bytes);
+    if (maxres <= 0) return fail(BFA_ERR_HIP, "batch: panel occupancy query failed");
     std::vector<PanelProb> chunk;
     int used = 0, rpw_max = 0;
     auto flush = [&]() {
@@ -584,54 +594,54 @@ struct Planner {
     return BFA_OK;
   }
 
-  // forward and backward sweeps of the complex64 triangular solves on D
-  void solve_sweeps(int nrhs)
+  // streaming triangular solves of the complex64 factors: D -> Y -> Xs
+  std::vector<SvProb> sv_descs(int nrhs, int &nblk)
   {
-    int maxNp = 0;
-    for (int i = 0; i < B->count; i++) maxNp = std::max(maxNp, B->Np[i]);
-    const int nb = maxNp / SVB;
-    auto emit = [&](int bi, bool lower) {
-      std::vector<TrsvProb> dg, up;
-      int blk = 0;
-      for (int i = 0; i < B->count; i++) {
-        const int Np = B->Np[i];
-        if (bi * SVB >= Np) continue;
-        const c2 *LU = (const c2 *)B->LU32[i];
-        c2 *D = B->Dptr(i);
-        const int64_t ld = Np, ldx = B->Np[i];
-        const int k = bi * SVB;
-        TrsvProb t{};
-        t.L = LU + (size_t)k * ld + k; t.ldl = ld; t.Y = D + k; t.ldy = ldx; t.kb = SVB; t.nrhs = nrhs;
-        t.skip = B->conv() + i;
-        dg.push_back(t);
-        TrsvProb u{};
-        u.skip = B->conv() + i; u.kb = SVB; u.nrhs = nrhs; u.ldl = ld; u.ldx = ldx; u.ldy = ldx;
-        u.Xk = D + k;
-        if (lower) {
-          u.m = Np - k - SVB; u.L = LU + (size_t)k * ld + k + SVB; u.Y = D + k + SVB;
-        } else {
-          u.m = k; u.L = LU + (size_t)k * ld; u.Y = D;
-        }
-        if (u.m > 0) {
-          u.blk0 = blk;
-          blk += (u.m + 63) / 64;
-          up.push_back(u);
-        }
-      }
-      if (!dg.empty()) {
-        const size_t off = ar.put(dg);
-        const int np = (int)dg.size();
-        op([Bp = B, off, np, lower](hipStream_t st) { launch_trsv_diag_g(false, lower, dp<TrsvProb>(Bp, off), np, st); });
-      }
-      if (!up.empty()) {
-        const size_t off = ar.put(up);
-        const int np = (int)up.size(), nblk = blk;
-        op([Bp = B, off, np, nblk](hipStream_t st) { launch_trsv_update_g(false, dp<TrsvProb>(Bp, off), np, nblk, st); });
-      }
-    };
-    for (int bi = 0; bi < nb; bi++) emit(bi, true);
-    for (int bi = nb - 1; bi >= 0; bi--) emit(bi, false);
+    std::vector<SvProb> v;
+    nblk = 0;
+    for (int i = 0; i < B->count; i++) {
+      const bfa_batch::Off &o = B->off[i];
+      SvProb q{};
+      q.LU = B->LU32[i];
+      q.inv = B->at<void>(o.inv);
+      q.Bin = B->Dptr(i);
+      q.Y = B->at<void>(o.Y);
+      q.X = B->at<void>(o.Xs);
+      q.tick = B->at<unsigned>(o.tick);
+      q.flag = B->flags() + 2 * i;
+      q.skip = B->conv() + i;
+      q.ld = B->Np[i];
+      q.ldx = B->Np[i];
+      q.n = B->Np[i];
+      q.nrhs = nrhs;
+      q.blk0 = nblk;
+      nblk += B->Np[i] / SVB;
+      v.push_back(q);
+    }
+    return v;
   }
+  // once per factorisation: inverted diagonal blocks of every instance
+  void solve_inverses(int nrhs)
+  {
+    int nblk = 0;
+    const size_t off = ar.put(sv_descs(nrhs, nblk));
+    const int np = B->count;
+    op([Bp = B, off, np, nblk](hipStream_t st) { launch_sv_inv_g(dp<SvProb>(Bp, off), np, nblk, st); });
+    sv_off = off;
+    sv_nblk = nblk;
+  }
+  // forward and backward sweeps (idle for converged instances)
+  void solve_sweeps()
+  {
+    const size_t off = sv_off;
+    const int np = B->count, nblk = sv_nblk;
+    op([Bp = B, off, np, nblk](hipStream_t st) {
+      launch_sv_stream_g(true, dp<SvProb>(Bp, off), np, nblk, st);
+      launch_sv_stream_g(false, dp<SvProb>(Bp, off), np, nblk, st);
+    });
+  }
+  size_t sv_off = 0;
+  int sv_nblk = 0;
 };
 
 }  // namespace
@@ -683,6 +693,10 @@ std::vector<SolveInst> solve_descs(const bfa_batch *B, int &nrow, int &nres, int
     q.LU = (c2 *)B->LU32[i];
     q.ipiv = B->ipiv(i);
     q.flags = B->flags() + 2 * i;
+    q.Y = B->at<c2>(o.Y);
+    q.Xs = B->at<c2>(o.Xs);
+    q.tick = B->at<unsigned>(o.tick);
+    q.tag = sv_tag_c64();
   }
   return v;
 }
@@ -766,11 +780,12 @@ int build_plan(bfa_batch *B, double tol, int maxit)
   P.op([B, o_sol, ni, srow, npw](hipStream_t st) {
     hipLaunchKernelGGL(kg_refine_init, dim3(srow), dim3(256), 0, st, dp<SolveInst>(B, o_sol), ni, npw);
   });
+  P.solve_inverses(npw);
   for (int it = 0; it < maxit; it++) {
     P.op([B, o_sol, ni, srow, npw](hipStream_t st) {
       hipLaunchKernelGGL(kg_gather, dim3(srow), dim3(256), 0, st, dp<SolveInst>(B, o_sol), ni, npw);
     });
-    P.solve_sweeps(npw);
+    P.solve_sweeps();
     P.op([B, o_sol, ni, srow, sres, npw, tol](hipStream_t st) {
       const SolveInst *d = dp<SolveInst>(B, o_sol);
       hipLaunchKernelGGL(kg_add, dim3(srow), dim3(256), 0, st, d, ni, npw);
@@ -855,6 +870,10 @@ extern "C" int bfa_batch_create(int count, bfa_geometry *const *geoms, const bfa
     f.X = take((size_t)npw * Np * 16);
     f.R = take((size_t)npw * Np * 16);
     f.D = take((size_t)npw * Np * 8);
+    f.Y = take((size_t)npw * Np * 8);
+    f.Xs = take((size_t)npw * Np * 8);
+    f.inv = take((size_t)2 * (Np / SVB) * SVB * SVB * 8);
+    f.tick = take(8);
     f.P = take((size_t)nchunk * npw * N * 16);
     f.norms = take((size_t)npw * nblk * 16);
     f.part = take((size_t)2 * npw * nblk * 8);

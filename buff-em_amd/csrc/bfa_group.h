@@ -59,17 +59,21 @@ struct PanelProb {
   PanelSlots slots;
 };
 
-// Triangular solve pieces of the multi-RHS solve, nrhs <= 8:
-//   diag:   X(kb x nrhs) := L^-1 X or U^-1 X, kb <= 64, one workgroup
-//   update: Y(m x nrhs) -= L(m x kb) Xk, 64 rows per workgroup from blk0
-// skip (optional): the problem is idle when *skip != 0.
-struct TrsvProb {
-  const void *L;
-  const void *Xk;
-  void *Y;
+
+// Streaming triangular solve of one problem (see k_trsv_stream): the
+// diagonal-block inverses inv [2][n/64][64*64] are built once per
+// factorisation (blk0 also indexes them); each refinement round reads Bin,
+// writes the tag-filled Y (forward sweep) and X (backward sweep).
+struct SvProb {
+  const void *LU;
+  void *inv;
+  const void *Bin;
+  void *Y, *X;
+  unsigned *tick;        // [2] tickets of the two sweeps
+  int *flag;             // set when a wait gives up
   const int *skip;
-  int64_t ldl, ldx, ldy;
-  int m, kb, nrhs, blk0;
+  int64_t ld, ldx;
+  int n, nrhs, blk0;
 };
 
 // launchers (f64 selects complex128, else complex64); nblk = total workgroups
@@ -77,8 +81,10 @@ void launch_gemm_g(bool f64, int tile, const GemmProb *d, int np, int nblk, hipS
 void launch_trsm_g(bool f64, bool lower, const TrsmProb *d, int np, int nblk, hipStream_t st);
 void launch_swap_g(bool f64, const SwapProb *d, int np, int nblk, hipStream_t st);
 void launch_panel_g(bool f64, const PanelProb *d, int np, int nblk, size_t lds, hipStream_t st);
-void launch_trsv_diag_g(bool f64, bool lower, const TrsvProb *d, int np, hipStream_t st);
-void launch_trsv_update_g(bool f64, const TrsvProb *d, int np, int nblk, hipStream_t st);
+void launch_sv_inv_g(const SvProb *d, int np, int nblk, hipStream_t st);            // complex64
+void launch_sv_stream_g(bool lower, const SvProb *d, int np, int nblk, hipStream_t st);
+// tag value that fills Y and X before a streaming sweep (complex64 word)
+unsigned long long sv_tag_c64();
 // panel capacity: LDS bytes available per workgroup and the co-resident
 // workgroup count of the grouped panel kernel (0 on error)
 size_t panel_lds_cap(bool f64);

@@ -905,136 +905,9 @@ __global__ void k_gather_rows(const cplx<T> *B, int64_t ldb, cplx<T> *X, int64_t
   if (i < n && j < nrhs) X[(size_t)j * ldx + i] = B[(size_t)j * ldb + perm[i]];
 }
 
-// Diagonal block (kb <= SV_KB rows) of a triangular solve with up to SV_RHS
-// right-hand sides: block and RHS slice held in LDS, one workgroup.
+// triangular solves: SV_KB-row blocks, up to SV_RHS right-hand sides per sweep
 #define SV_KB 64
 #define SV_RHS 8
-template <typename T, bool LOWER>
-__device__ __forceinline__ void trsv_diag_body(int kb, int nrhs, const cplx<T> *L, int64_t ldl, cplx<T> *X,
-                                               int64_t ldx, cplx<T> (&Ls)[SV_KB][SV_KB + 1],
-                                               cplx<T> (&Xs)[SV_RHS][SV_KB])
-{
-  const int tid = threadIdx.x;
-  for (int idx = tid; idx < kb * kb; idx += blockDim.x) {
-    const int r = idx % kb, c = idx / kb;
-    Ls[r][c] = L[(size_t)c * ldl + r];
-  }
-  for (int idx = tid; idx < kb * nrhs; idx += blockDim.x) {
-    const int r = idx % kb, j = idx / kb;
-    Xs[j][r] = X[(size_t)j * ldx + r];
-  }
-  __syncthreads();
-  const int r = tid % SV_KB, jg = tid / SV_KB;     // 4 RHS groups of 64 rows
-  if (LOWER) {
-    for (int i = 0; i < kb - 1; i++) {
-      if (r > i && r < kb)
-        for (int j = jg; j < nrhs; j += 4) Xs[j][r] = c_sub(Xs[j][r], c_mul(Ls[r][i], Xs[j][i]));
-      __syncthreads();
-    }
-  } else {
-    for (int i = kb - 1; i >= 0; i--) {
-      if (r == i)
-        for (int j = jg; j < nrhs; j += 4) Xs[j][i] = c_mul(Xs[j][i], c_inv(Ls[i][i]));
-      __syncthreads();
-      if (r < i)
-        for (int j = jg; j < nrhs; j += 4) Xs[j][r] = c_sub(Xs[j][r], c_mul(Ls[r][i], Xs[j][i]));
-      __syncthreads();
-    }
-  }
-  for (int idx = tid; idx < kb * nrhs; idx += blockDim.x) {
-    const int rr = idx % kb, j = idx / kb;
-    X[(size_t)j * ldx + rr] = Xs[j][rr];
-  }
-}
-
-template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) void k_trsv_diag(int kb, int nrhs, const cplx<T> *L, int64_t ldl, cplx<T> *X,
-                                                   int64_t ldx)
-{
-  __shared__ cplx<T> Ls[SV_KB][SV_KB + 1];
-  __shared__ cplx<T> Xs[SV_RHS][SV_KB];
-  trsv_diag_body<T, LOWER>(kb, nrhs, L, ldl, X, ldx, Ls, Xs);
-}
-
-// Y(m x nrhs) -= L(m x kb) * Xk(kb x nrhs), kb <= SV_KB: a workgroup owns 64
-// rows; its 4 waves take interleaved quarters of the columns (coalesced
-// column walks of L, Xk broadcast from LDS) and are reduced through LDS.
-template <typename T> struct TrsvShared {
-  cplx<T> xs[SV_RHS][SV_KB];
-  cplx<T> part[4][SV_RHS][64];
-};
-template <typename T>
-__device__ __forceinline__ void trsv_update_body(int m, int kb, int nrhs, const cplx<T> *__restrict__ L, int64_t ldl,
-                                                 const cplx<T> *__restrict__ Xk, int64_t ldx, cplx<T> *Y, int64_t ldy,
-                                                 int rblk, TrsvShared<T> &sh)
-{
-  cplx<T>(&xs)[SV_RHS][SV_KB] = sh.xs;
-  cplx<T>(&part)[4][SV_RHS][64] = sh.part;
-  for (int idx = threadIdx.x; idx < kb * nrhs; idx += blockDim.x) {
-    const int c = idx % kb, j = idx / kb;
-    xs[j][c] = Xk[(size_t)j * ldx + c];
-  }
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int r = rblk * 64 + lane;
-  T ar[SV_RHS], ai[SV_RHS];
-#pragma unroll
-  for (int j = 0; j < SV_RHS; j++) { ar[j] = 0; ai[j] = 0; }
-  if (r < m) {
-#pragma unroll 4
-    for (int c = wv; c < kb; c += 4) {
-      const cplx<T> l = L[(size_t)c * ldl + r];
-#pragma unroll
-      for (int j = 0; j < SV_RHS; j++) {
-        if (j < nrhs) {
-          const cplx<T> x = xs[j][c];
-          ar[j] += l.re * x.re - l.im * x.im;
-          ai[j] += l.re * x.im + l.im * x.re;
-        }
-      }
-    }
-  }
-  for (int j = 0; j < nrhs; j++) part[wv][j][lane] = {ar[j], ai[j]};
-  __syncthreads();
-  if (wv == 0 && r < m) {
-    for (int j = 0; j < nrhs; j++) {
-      const T sr = part[0][j][lane].re + part[1][j][lane].re + part[2][j][lane].re + part[3][j][lane].re;
-      const T si = part[0][j][lane].im + part[1][j][lane].im + part[2][j][lane].im + part[3][j][lane].im;
-      cplx<T> &y = Y[(size_t)j * ldy + r];
-      y.re -= sr;
-      y.im -= si;
-    }
-  }
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void k_trsv_update(int m, int kb, int nrhs, const cplx<T> *__restrict__ L,
-                                                     int64_t ldl, const cplx<T> *__restrict__ Xk, int64_t ldx,
-                                                     cplx<T> *Y, int64_t ldy)
-{
-  __shared__ TrsvShared<T> sh;
-  trsv_update_body<T>(m, kb, nrhs, L, ldl, Xk, ldx, Y, ldy, blockIdx.x, sh);
-}
-
-template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) void kg_trsv_diag(const TrsvProb *__restrict__ d, int np)
-{
-  __shared__ cplx<T> Ls[SV_KB][SV_KB + 1];
-  __shared__ cplx<T> Xs[SV_RHS][SV_KB];
-  const TrsvProb q = d[blockIdx.x];
-  if (q.skip && *q.skip) return;
-  trsv_diag_body<T, LOWER>(q.kb, q.nrhs, (const cplx<T> *)q.L, q.ldl, (cplx<T> *)q.Y, q.ldy, Ls, Xs);
-}
-
-template <typename T>
-__global__ __launch_bounds__(256) void kg_trsv_update(const TrsvProb *__restrict__ d, int np)
-{
-  __shared__ TrsvShared<T> sh;
-  const TrsvProb q = d[find_prob(d, np, blockIdx.x)];
-  if (q.skip && *q.skip) return;
-  trsv_update_body<T>(q.m, q.kb, q.nrhs, (const cplx<T> *)q.L, q.ldl, (const cplx<T> *)q.Xk, q.ldx, (cplx<T> *)q.Y,
-                      q.ldy, blockIdx.x - q.blk0, sh);
-}
 
 // ---------------------------------------------------------------------------
 // Streaming block substitution.  The diagonal SV_KB x SV_KB blocks of L
@@ -1048,13 +921,17 @@ __global__ __launch_bounds__(256) void kg_trsv_update(const TrsvProb *__restrict
 // cache maintenance, no separate flag), so a block is visible as soon as
 // its values land.  A ticket only ever waits on lower tickets, which belong
 // to workgroups already running.
+template <typename T> struct InvShared {
+  cplx<T> Ls[SV_KB][SV_KB + 1];
+  cplx<T> X[SV_KB][SV_KB + 1];
+};
 template <typename T>
-__global__ __launch_bounds__(256) void k_trsv_inv(int n, const cplx<T> *__restrict__ A, int64_t lda, cplx<T> *inv)
+__device__ __forceinline__ void trsv_inv_body(int n, const cplx<T> *__restrict__ A, int64_t lda, cplx<T> *inv, int bi,
+                                              bool lower, InvShared<T> &sh)
 {
-  __shared__ cplx<T> Ls[SV_KB][SV_KB + 1];
-  __shared__ cplx<T> X[SV_KB][SV_KB + 1];
-  const int nblk = (n + SV_KB - 1) / SV_KB, bi = blockIdx.x;
-  const bool lower = blockIdx.y == 0;
+  cplx<T>(&Ls)[SV_KB][SV_KB + 1] = sh.Ls;
+  cplx<T>(&X)[SV_KB][SV_KB + 1] = sh.X;
+  const int nblk = (n + SV_KB - 1) / SV_KB;
   const int k0 = bi * SV_KB, kb = min(SV_KB, n - k0);
   const int tid = threadIdx.x, c = tid % SV_KB, rg = tid / SV_KB;
   for (int idx = tid; idx < SV_KB * SV_KB; idx += blockDim.x) {
@@ -1080,6 +957,13 @@ __global__ __launch_bounds__(256) void k_trsv_inv(int n, const cplx<T> *__restri
   }
   cplx<T> *o = inv + ((size_t)(lower ? 0 : nblk) + bi) * SV_KB * SV_KB;
   for (int idx = tid; idx < SV_KB * SV_KB; idx += blockDim.x) o[idx] = X[idx % SV_KB][idx / SV_KB];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsv_inv(int n, const cplx<T> *__restrict__ A, int64_t lda, cplx<T> *inv)
+{
+  __shared__ InvShared<T> sh;
+  trsv_inv_body<T>(n, A, lda, inv, blockIdx.x, blockIdx.y == 0, sh);
 }
 
 template <typename T> struct SvTag;
@@ -1144,13 +1028,12 @@ template <typename T> struct StreamShared {
 };
 
 // Bin: right-hand sides (n x nrhs, read-only here); Xo: tag-filled output.
-// ticket: this launch's counter; flag: set when a wait gives up.
+// ticket: this sweep's counter; flag: set when a wait gives up.
 template <typename T, bool LOWER>
-__global__ __launch_bounds__(256) void k_trsv_stream(int n, int nrhs, const cplx<T> *__restrict__ A, int64_t lda,
-                                                     const cplx<T> *__restrict__ inv, const cplx<T> *Bin,
-                                                     cplx<T> *Xo, int64_t ldx, unsigned *ticket, unsigned *flag)
+__device__ __forceinline__ void trsv_stream_body(int n, int nrhs, const cplx<T> *__restrict__ A, int64_t lda,
+                                                 const cplx<T> *__restrict__ inv, const cplx<T> *Bin, cplx<T> *Xo,
+                                                 int64_t ldx, unsigned *ticket, unsigned *flag, StreamShared<T> &sh)
 {
-  __shared__ StreamShared<T> sh;
   const int nblk = (n + SV_KB - 1) / SV_KB;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   if (tid == 0) sh.ticket = (int)__hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1244,6 +1127,35 @@ __global__ __launch_bounds__(256) void k_trsv_stream(int n, int nrhs, const cplx
       sv_store(&Xo[(size_t)j * ldx + r], cplx<T>{xr, xi});
     }
   }
+}
+
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(256) void k_trsv_stream(int n, int nrhs, const cplx<T> *__restrict__ A, int64_t lda,
+                                                     const cplx<T> *__restrict__ inv, const cplx<T> *Bin,
+                                                     cplx<T> *Xo, int64_t ldx, unsigned *ticket, unsigned *flag)
+{
+  __shared__ StreamShared<T> sh;
+  trsv_stream_body<T, LOWER>(n, nrhs, A, lda, inv, Bin, Xo, ldx, ticket, flag, sh);
+}
+
+// grouped forms over the problems of a batch (complex64)
+__global__ __launch_bounds__(256) void kg_sv_inv(const SvProb *__restrict__ d, int np)
+{
+  __shared__ InvShared<float> sh;
+  const SvProb q = d[find_prob(d, np, blockIdx.x)];
+  trsv_inv_body<float>(q.n, (const cplx<float> *)q.LU, q.ld, (cplx<float> *)q.inv, blockIdx.x - q.blk0,
+                       blockIdx.y == 0, sh);
+}
+
+template <bool LOWER>
+__global__ __launch_bounds__(256) void kg_sv_stream(const SvProb *__restrict__ d, int np)
+{
+  __shared__ StreamShared<float> sh;
+  const SvProb q = d[find_prob(d, np, blockIdx.x)];
+  if (q.skip && *q.skip) return;
+  trsv_stream_body<float, LOWER>(q.n, q.nrhs, (const cplx<float> *)q.LU, q.ld, (const cplx<float> *)q.inv,
+                                 (const cplx<float> *)(LOWER ? q.Bin : q.Y), (cplx<float> *)(LOWER ? q.Y : q.X), q.ldx,
+                                 q.tick + (LOWER ? 0 : 1), (unsigned *)q.flag, sh);
 }
 
 // scratch: [perm n ints][flag 256 B][inverted diagonal blocks]
@@ -1364,6 +1276,18 @@ int lu_solve_prep_c(const void *A, int n, int64_t lda, const int *ipiv, hipStrea
 int lu_solve_apply_c(const void *A, int n, int64_t lda, void *B, int nrhs, int64_t ldb, hipStream_t st, void *s)
 { return lu_solve_apply<float>((const cplx<float> *)A, n, lda, (cplx<float> *)B, nrhs, ldb, st, s); }
 
+void launch_sv_inv_g(const SvProb *d, int np, int nblk, hipStream_t st)
+{
+  if (np > 0 && nblk > 0) hipLaunchKernelGGL(kg_sv_inv, dim3(nblk, 2), dim3(256), 0, st, d, np);
+}
+void launch_sv_stream_g(bool lower, const SvProb *d, int np, int nblk, hipStream_t st)
+{
+  if (np <= 0 || nblk <= 0) return;
+  if (lower) hipLaunchKernelGGL(kg_sv_stream<true>, dim3(nblk), dim3(256), 0, st, d, np);
+  else hipLaunchKernelGGL(kg_sv_stream<false>, dim3(nblk), dim3(256), 0, st, d, np);
+}
+unsigned long long sv_tag_c64() { return (unsigned long long)SvTag<float>::bits * 0x100000001ull; }
+
 void launch_gemm_g(bool f64, int tile, const GemmProb *d, int np, int nblk, hipStream_t st)
 {
   if (np <= 0 || nblk <= 0) return;
@@ -1428,24 +1352,6 @@ void launch_panel_g(bool f64, const PanelProb *d, int np, int nblk, size_t lds, 
   else hipLaunchKernelGGL(kg_panel<float>, dim3(nblk), dim3(256), lds, st, d, np);
 }
 
-void launch_trsv_diag_g(bool f64, bool lower, const TrsvProb *d, int np, hipStream_t st)
-{
-  if (np <= 0) return;
-  if (f64) {
-    if (lower) hipLaunchKernelGGL((kg_trsv_diag<double, true>), dim3(np), dim3(256), 0, st, d, np);
-    else hipLaunchKernelGGL((kg_trsv_diag<double, false>), dim3(np), dim3(256), 0, st, d, np);
-  } else {
-    if (lower) hipLaunchKernelGGL((kg_trsv_diag<float, true>), dim3(np), dim3(256), 0, st, d, np);
-    else hipLaunchKernelGGL((kg_trsv_diag<float, false>), dim3(np), dim3(256), 0, st, d, np);
-  }
-}
-
-void launch_trsv_update_g(bool f64, const TrsvProb *d, int np, int nblk, hipStream_t st)
-{
-  if (np <= 0 || nblk <= 0) return;
-  if (f64) hipLaunchKernelGGL(kg_trsv_update<double>, dim3(nblk), dim3(256), 0, st, d, np);
-  else hipLaunchKernelGGL(kg_trsv_update<float>, dim3(nblk), dim3(256), 0, st, d, np);
-}
 
 }  // namespace bfa
 
