@@ -53,6 +53,7 @@ struct c2 { float re, im; };
 #define SVB 64          // triangular-solve block
 #define RCH 256         // residual column chunk
 #define NBUCKET 16
+#define CSTRIDE (16 + 2 * PWID)   // u32 per panel barrier: counter, pad, 32 keys
 
 // per-instance solve state on the device
 struct SolveInst {
@@ -287,7 +288,7 @@ struct bfa_batch {
   std::vector<void *> M64, LU32;
   // everything else in one allocation; per-instance byte offsets
   void *small = nullptr;
-  struct Off { size_t ipiv, perm, B, X, R, D, Y, Xs, inv, tick, P, norms, part, sval, srow, sdata, sj; };
+  struct Off { size_t ipiv, perm, B, X, R, D, Y, Xs, inv, tick, P, norms, part, sdata, sj; };
   std::vector<Off> off;
   size_t o_fs = 0, o_conv = 0, o_iters = 0, o_resid = 0, o_counters = 0, o_flags = 0;
   size_t o_hist_it = 0, o_hist_res = 0, o_hist_info = 0;
@@ -473,7 +474,10 @@ struct Planner {
       q.piv_off = p.piv_off;
       q.slots = B->slots[p.inst];
       if (B->ncounters >= B->ncounters_max) return fail(BFA_ERR_ARG, "batch: barrier counter pool exhausted");
-      q.slots.bar = B->counters() + B->ncounters++;
+      // per panel: barrier counter, then the 32 column keys (64-bit aligned)
+      unsigned *c = B->counters() + (size_t)CSTRIDE * B->ncounters++;
+      q.slots.bar = c;
+      q.slots.key = (unsigned long long *)(c + 16);
       chunk.push_back(q);
       used += nwg;
       rpw_max = std::max(rpw_max, rpw);
@@ -754,7 +758,7 @@ int build_plan(bfa_batch *B, double tol, int maxit)
   });
   // --- convert + factor
   P.phase = 1;
-  const size_t ncnt = B->ncounters_max;
+  const size_t ncnt = B->ncounters_max * CSTRIDE;
   unsigned *cnt = B->counters();
   int *flags0 = B->flags();
   const size_t nflags = (size_t)2 * ni;
@@ -877,8 +881,6 @@ extern "C" int bfa_batch_create(int count, bfa_geometry *const *geoms, const bfa
     f.P = take((size_t)nchunk * npw * N * 16);
     f.norms = take((size_t)npw * nblk * 16);
     f.part = take((size_t)2 * npw * nblk * 8);
-    f.sval = take(2 * 256 * 8);
-    f.srow = take(2 * 256 * 4);
     f.sdata = take((size_t)2 * 256 * PWID * 16);
     f.sj = take(2 * PWID * 16);
     B->off.push_back(f);
@@ -889,7 +891,7 @@ extern "C" int bfa_batch_create(int count, bfa_geometry *const *geoms, const bfa
   B->o_iters = take((size_t)count * 4);
   B->o_resid = take((size_t)count * 8);
   B->o_flags = take((size_t)count * 8);
-  B->o_counters = take(B->ncounters_max * 4);
+  B->o_counters = take(B->ncounters_max * CSTRIDE * 4);
   B->o_hist_it = take((size_t)BATCH_MAXFREQ * count * 4);
   B->o_hist_res = take((size_t)BATCH_MAXFREQ * count * 8);
   B->o_hist_info = take((size_t)BATCH_MAXFREQ * count * 4);
@@ -908,8 +910,6 @@ extern "C" int bfa_batch_create(int count, bfa_geometry *const *geoms, const bfa
       return bail(fail(BFA_ERR_OOM, "bfa_batch_create: instance %d complex64 LU", i));
     B->LU32.push_back(l);
     PanelSlots ps{};
-    ps.val = B->at<double>(B->off[i].sval);
-    ps.row = B->at<int>(B->off[i].srow);
     ps.rowdata = B->at<void>(B->off[i].sdata);
     ps.rowj = B->at<void>(B->off[i].sj);
     ps.flags = B->flags() + 2 * i;

@@ -42,11 +42,10 @@ struct SwapProb {
 
 // Panel workspace of one problem (cross-workgroup slots and barrier)
 struct PanelSlots {
-  double *val;      // [2][maxwg]
-  int *row;         // [2][maxwg]
-  void *rowdata;    // [2][maxwg][32] complex
+  void *rowdata;    // [2][maxwg][32] complex: each workgroup's pivot candidate row
   void *rowj;       // [2][32] complex
   unsigned *bar;    // barrier counter, zero at launch
+  unsigned long long *key;  // [32] per-column packed argmax keys, zero at launch
   int *flags;       // [0] spin timeout, [1] first zero pivot (1-based)
 };
 

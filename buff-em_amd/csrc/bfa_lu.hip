@@ -516,13 +516,23 @@ template <typename T> __device__ __forceinline__ void wave_argmax(T &v, int &r, 
   }
 }
 
+// Pivot search key: |x|_1 rounded to float (monotone, non-negative: its
+// bits order as integers) above the inverted row, so the 64-bit maximum is
+// the largest magnitude and, among equal keys, the first row (LAPACK's
+// izamax up to the float rounding of near-ties).
+__device__ __forceinline__ unsigned long long panel_key(float v, int row)
+{
+  return ((unsigned long long)__float_as_uint(v) << 32) | (unsigned long long)(0xFFFFFFFFu - (unsigned)row);
+}
+
 // Body shared by the single-matrix and the grouped panel kernels: workgroup
 // g of the nwg that factor this panel; rows = the workgroup's LDS block.
 // Per column: the workgroup's candidate (found while updating the previous
-// column) is reduced across waves, published with its row data, one grid
-// barrier, every workgroup reduces the slots to the same pivot, then each
-// thread swaps/scales/updates its own rows and tracks the next column's
-// candidate.  Five workgroup barriers per column.
+// column) is reduced across waves, its row data published and its key
+// merged into the column's atomic maximum, one grid barrier, every
+// workgroup reads the same pivot from the key (the winner's rows belong to
+// workgroup p / rpw), then each thread swaps/scales/updates its own rows and
+// tracks the next column's candidate.
 template <typename T>
 __device__ __forceinline__ void panel_body(cplx<T> *A, int m, int w, int64_t lda, int *ipiv, int rpw,
                                            const PanelWork &pw, int piv_off, int g, int nwg, cplx<T> *rows,
@@ -557,10 +567,8 @@ __device__ __forceinline__ void panel_body(cplx<T> *A, int m, int w, int64_t lda
     int lrow = sh.wrow[0], ld = 0;
     for (int q = 1; q < nwave; q++) argmax_merge(lbest, lrow, ld, sh.wval[q], sh.wrow[q], q);
     // (w <= PW = 32: every store below is made by wave 0)
-    if (tid == 0) {
-      st_agent(&pw.val[buf * nwg + g], (double)lbest);
-      st_agent(&pw.row[buf * nwg + g], lrow);
-    }
+    if (tid == 0 && lrow != INT_MAX)
+      __hip_atomic_fetch_max(&pw.key[j], panel_key((float)lbest, lrow), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (lrow != INT_MAX && tid < w) {
       const cplx<T> v = RW(lrow - r0, tid);
       cplx<T> *d = &rdata[((size_t)buf * nwg + g) * PW + tid];
@@ -573,37 +581,25 @@ __device__ __forceinline__ void panel_body(cplx<T> *A, int m, int w, int64_t lda
       st_agent(&rowjg[buf * PW + tid].im, v.im);
     }
     grid_barrier(pw.bar, (unsigned)((j + 1) * nwg), pw.flags);
-    // global pivot (same in every workgroup): first index of the maximum
-    {
-      T bv = -1;
-      int br = INT_MAX, bw = -1;
-      for (int q = tid; q < nwg; q += blockDim.x) {
-        const int rr = ld_agent(&pw.row[buf * nwg + q]);
-        if (rr == INT_MAX) continue;
-        argmax_merge(bv, br, bw, (T)ld_agent(&pw.val[buf * nwg + q]), rr, q);
-      }
-      wave_argmax(bv, br, bw);
-      if (lane == 0) { sh.wval[wave] = bv; sh.wrow[wave] = br; sh.wwin[wave] = bw; }
-    }
+    // global pivot (same in every workgroup): the largest key
+    if (tid == 0) sh.wrow[0] = (int)(0xFFFFFFFFu - (unsigned)ld_agent(&pw.key[j]));
     __syncthreads();
-    T pval = sh.wval[0];
-    int p = sh.wrow[0], win = sh.wwin[0];
-    for (int q = 1; q < nwave; q++) argmax_merge(pval, p, win, sh.wval[q], sh.wrow[q], sh.wwin[q]);
+    const int p = sh.wrow[0], win = p / rpw;
     if (tid < w) {
       const cplx<T> *d = &rdata[((size_t)buf * nwg + win) * PW + tid];
       sh.prow[tid] = {ld_agent(&d->re), ld_agent(&d->im)};
       sh.rj[tid] = {ld_agent(&rowjg[buf * PW + tid].re), ld_agent(&rowjg[buf * PW + tid].im)};
     }
+    __syncthreads();
+    const bool scale = c_abs1(sh.prow[j]) != (T)0;
     if (g == 0 && tid == 0) {
       ipiv[j] = p + piv_off;
-      if (pval == (T)0 && pw.flags[1] == 0) pw.flags[1] = piv_off + j + 1;
+      if (!scale && pw.flags[1] == 0) pw.flags[1] = piv_off + j + 1;
     }
-    __syncthreads();
     // swap rows j and p, scale column j below the diagonal, update the rest;
     // each row belongs to one thread, which also tracks column j+1's candidate
     best = -1;
     brow = INT_MAX;
-    const bool scale = pval != (T)0;
     const cplx<T> inv = scale ? c_inv(sh.prow[j]) : cplx<T>{0, 0};
     for (int r = tid; r < nr; r += blockDim.x) {
       const int gr = r0 + r;
@@ -748,15 +744,14 @@ static int lu_work(LUWork *&w)
   HIPCHECK(hipGetDeviceProperties(&prop, dev));
   w->num_cu = prop.multiProcessorCount;
   w->maxwg = 1024;
-  const size_t bytes = 2 * w->maxwg * 8 + 2 * w->maxwg * 4 + 2 * (size_t)w->maxwg * PW * 16 + 2 * PW * 16 + 64 + 64;
+  const size_t bytes = 2 * (size_t)w->maxwg * PW * 16 + 2 * PW * 16 + 64 + PW * 8 + 64;
   HIPCHECK(hipMalloc(&w->mem, bytes));
   HIPCHECK(hipMemset(w->mem, 0, bytes));
   char *p = (char *)w->mem;
-  w->pw.val = (double *)p; p += 2 * w->maxwg * 8;
-  w->pw.row = (int *)p; p += 2 * w->maxwg * 4;
   w->pw.rowdata = p; p += 2 * (size_t)w->maxwg * PW * 16;
   w->pw.rowj = p; p += 2 * PW * 16;
   w->pw.bar = (unsigned *)p; p += 64;
+  w->pw.key = (unsigned long long *)p; p += PW * 8;
   w->pw.flags = (int *)p;
   w->device = dev;
   return BFA_OK;
@@ -803,7 +798,7 @@ static int panel_base(cplx<T> *A, int m, int w, int64_t lda, int *ipiv, int piv_
   }
   if (nwg > resident[ti])
     return fail(BFA_ERR_HIP, "panel needs %d co-resident workgroups, device fits %d", nwg, resident[ti]);
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, lw->pw.bar, 16);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(128), 0, st, lw->pw.bar, 16 + 2 * PW);   // barrier and keys
   hipLaunchKernelGGL(k_panel<T>, dim3(nwg), dim3(256), lds, st, A, m, w, lda, ipiv, rpw, lw->pw, piv_off);
   HIPCHECK(hipGetLastError());
   return BFA_OK;
