@@ -53,7 +53,7 @@ struct c2 { float re, im; };
 #define SVB 64          // triangular-solve block
 #define RCH 256         // residual column chunk
 #define NBUCKET 16
-#define CSTRIDE (16 + 2 * PWID)   // u32 per panel barrier: counter, pad, 32 keys
+#define CSTRIDE 768   // u32 per panel barrier: 8 counters 128 B apart, then 8 x 32 keys
 
 // per-instance solve state on the device
 struct SolveInst {
@@ -474,10 +474,10 @@ struct Planner {
       q.piv_off = p.piv_off;
       q.slots = B->slots[p.inst];
       if (B->ncounters >= B->ncounters_max) return fail(BFA_ERR_ARG, "batch: barrier counter pool exhausted");
-      // per panel: barrier counter, then the 32 column keys (64-bit aligned)
+      // per panel: barrier counters, then the column keys (64-bit aligned)
       unsigned *c = B->counters() + (size_t)CSTRIDE * B->ncounters++;
       q.slots.bar = c;
-      q.slots.key = (unsigned long long *)(c + 16);
+      q.slots.key = (unsigned long long *)(c + 8 * 32);
       chunk.push_back(q);
       used += nwg;
       rpw_max = std::max(rpw_max, rpw);
@@ -854,6 +854,7 @@ extern "C" int bfa_batch_create(int count, bfa_geometry *const *geoms, const bfa
     for (int t = 0; t < NUM_TILES; t++) B->tune_ms[b][t] = 0.0;
   }
   if (hipGetDevice(&B->device) != hipSuccess) return bail(fail(BFA_ERR_HIP, "bfa_batch_create: no device"));
+  if (panel_sync_words() != CSTRIDE) return bail(fail(BFA_ERR_STATE, "bfa_batch_create: panel barrier layout mismatch"));
   size_t o = 0;
   auto take = [&o](size_t bytes) { size_t r = (o + 255) / 256 * 256; o = r + bytes; return r; };
   for (int i = 0; i < count; i++) {

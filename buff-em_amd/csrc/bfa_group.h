@@ -44,8 +44,8 @@ struct SwapProb {
 struct PanelSlots {
   void *rowdata;    // [2][maxwg][32] complex: each workgroup's pivot candidate row
   void *rowj;       // [2][32] complex
-  unsigned *bar;    // barrier counter, zero at launch
-  unsigned long long *key;  // [32] per-column packed argmax keys, zero at launch
+  unsigned *bar;    // barrier counters, zero at launch (panel_sync_words() u32 with key)
+  unsigned long long *key;  // per-column packed argmax keys, zero at launch
   int *flags;       // [0] spin timeout, [1] first zero pivot (1-based)
 };
 
@@ -87,6 +87,8 @@ unsigned long long sv_tag_c64();
 // panel capacity: LDS bytes available per workgroup and the co-resident
 // workgroup count of the grouped panel kernel (0 on error)
 size_t panel_lds_cap(bool f64);
+// u32 words of one panel's barrier counters + keys (key = bar + 8 * 32)
+int panel_sync_words();
 int panel_max_resident(bool f64, size_t lds);
 
 // ---------------------------------------------------------------------------

@@ -480,15 +480,46 @@ template <typename V> __device__ __forceinline__ V ld_agent(const V *p)
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ void grid_barrier(unsigned *bar, unsigned target, int *flags)
+// The panel's barrier counters and pivot keys are spread over PB_LANES
+// cache lines (workgroup g uses lane g % PB_LANES, which follows the XCD
+// round-robin), so arrivals on one address are ~nwg / 8 instead of nwg.
+#define PB_LANES 8
+#define PB_BAR_STRIDE 32    // u32 between counters (128 bytes)
+#define PB_WORDS (PB_LANES * PB_BAR_STRIDE + PB_LANES * PW * 2)   // u32: counters, then keys [lane][PW]
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned lo = __shfl_xor((unsigned)v, off, 64), hi = __shfl_xor((unsigned)(v >> 32), off, 64);
+    const unsigned long long o = ((unsigned long long)hi << 32) | lo;
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// Wave 0 arrives (thread 0, after its wave's stores completed) and polls the
+// sum of the counters (lanes < PB_LANES); every wave waits at the workgroup
+// barrier that follows.
+__device__ __forceinline__ void grid_barrier(unsigned *bar, int g, unsigned target, int *flags)
 {
   __syncthreads();
-  if (threadIdx.x == 0) {
-    __builtin_amdgcn_s_waitcnt(0);   // this wave's slot stores have completed
-    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    if (lane == 0) {
+      __builtin_amdgcn_s_waitcnt(0);   // this wave's slot stores and key update have completed
+      __hip_atomic_fetch_add(&bar[(g % PB_LANES) * PB_BAR_STRIDE], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     unsigned spins = 0;
-    while (ld_agent(bar) < target) {
-      if (++spins > SPIN_LIMIT) { atomicOr(&flags[0], 1); break; }
+    for (;;) {
+      unsigned c = lane < PB_LANES ? ld_agent(&bar[lane * PB_BAR_STRIDE]) : 0u;
+#pragma unroll
+      for (int off = 4; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
+      if (__shfl(c, 0, 64) >= target) break;
+      if (++spins > SPIN_LIMIT) {
+        if (lane == 0) atomicOr(&flags[0], 1);
+        break;
+      }
     }
   }
   __syncthreads();
@@ -496,8 +527,8 @@ __device__ __forceinline__ void grid_barrier(unsigned *bar, unsigned target, int
 
 template <typename T> struct PanelShared {
   cplx<T> prow[PW], rj[PW];
-  T wval[4];
-  int wrow[4], wwin[4];
+  unsigned long long wkey[4];
+  int wrow[4];
 };
 
 // first index of the maximum: (v, r) wins over (ov, orr) unless ov is larger
@@ -505,15 +536,6 @@ template <typename T> struct PanelShared {
 template <typename T> __device__ __forceinline__ void argmax_merge(T &v, int &r, int &w, T ov, int orr, int ow)
 {
   if (ov > v || (ov == v && orr < r)) { v = ov; r = orr; w = ow; }
-}
-template <typename T> __device__ __forceinline__ void wave_argmax(T &v, int &r, int &w)
-{
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const T ov = __shfl_xor(v, off, 64);
-    const int orr = __shfl_xor(r, off, 64), ow = __shfl_xor(w, off, 64);
-    argmax_merge(v, r, w, ov, orr, ow);
-  }
 }
 
 // Pivot search key: |x|_1 rounded to float (monotone, non-negative: its
@@ -558,17 +580,16 @@ __device__ __forceinline__ void panel_body(cplx<T> *A, int m, int w, int64_t lda
   for (int j = 0; j < w; j++) {
     const int buf = j & 1;
     {
-      int wdummy = 0;
-      wave_argmax(best, brow, wdummy);
-      if (lane == 0) { sh.wval[wave] = best; sh.wrow[wave] = brow; }
+      const unsigned long long k = wave_max_u64(brow == INT_MAX ? 0ull : panel_key((float)best, brow));
+      if (lane == 0) sh.wkey[wave] = k;
     }
     __syncthreads();
-    T lbest = sh.wval[0];
-    int lrow = sh.wrow[0], ld = 0;
-    for (int q = 1; q < nwave; q++) argmax_merge(lbest, lrow, ld, sh.wval[q], sh.wrow[q], q);
+    unsigned long long lkey = sh.wkey[0];
+    for (int q = 1; q < nwave; q++) lkey = sh.wkey[q] > lkey ? sh.wkey[q] : lkey;
+    const int lrow = lkey ? (int)(0xFFFFFFFFu - (unsigned)lkey) : INT_MAX;
     // (w <= PW = 32: every store below is made by wave 0)
-    if (tid == 0 && lrow != INT_MAX)
-      __hip_atomic_fetch_max(&pw.key[j], panel_key((float)lbest, lrow), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0 && lkey)
+      __hip_atomic_fetch_max(&pw.key[(g % PB_LANES) * PW + j], lkey, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (lrow != INT_MAX && tid < w) {
       const cplx<T> v = RW(lrow - r0, tid);
       cplx<T> *d = &rdata[((size_t)buf * nwg + g) * PW + tid];
@@ -580,15 +601,19 @@ __device__ __forceinline__ void panel_body(cplx<T> *A, int m, int w, int64_t lda
       st_agent(&rowjg[buf * PW + tid].re, v.re);
       st_agent(&rowjg[buf * PW + tid].im, v.im);
     }
-    grid_barrier(pw.bar, (unsigned)((j + 1) * nwg), pw.flags);
+    grid_barrier(pw.bar, g, (unsigned)((j + 1) * nwg), pw.flags);
     // global pivot (same in every workgroup): the largest key
-    if (tid == 0) sh.wrow[0] = (int)(0xFFFFFFFFu - (unsigned)ld_agent(&pw.key[j]));
+    if (tid < 64) {
+      const unsigned long long k = wave_max_u64(tid < PB_LANES ? ld_agent(&pw.key[tid * PW + j]) : 0ull);
+      if (tid == 0) sh.wrow[0] = (int)(0xFFFFFFFFu - (unsigned)k);
+    }
     __syncthreads();
     const int p = sh.wrow[0], win = p / rpw;
-    if (tid < w) {
+    if (tid < PW) {
       const cplx<T> *d = &rdata[((size_t)buf * nwg + win) * PW + tid];
-      sh.prow[tid] = {ld_agent(&d->re), ld_agent(&d->im)};
-      sh.rj[tid] = {ld_agent(&rowjg[buf * PW + tid].re), ld_agent(&rowjg[buf * PW + tid].im)};
+      sh.prow[tid] = tid < w ? cplx<T>{ld_agent(&d->re), ld_agent(&d->im)} : cplx<T>{0, 0};
+      sh.rj[tid] = tid < w ? cplx<T>{ld_agent(&rowjg[buf * PW + tid].re), ld_agent(&rowjg[buf * PW + tid].im)}
+                           : cplx<T>{0, 0};
     }
     __syncthreads();
     const bool scale = c_abs1(sh.prow[j]) != (T)0;
@@ -744,14 +769,15 @@ static int lu_work(LUWork *&w)
   HIPCHECK(hipGetDeviceProperties(&prop, dev));
   w->num_cu = prop.multiProcessorCount;
   w->maxwg = 1024;
-  const size_t bytes = 2 * (size_t)w->maxwg * PW * 16 + 2 * PW * 16 + 64 + PW * 8 + 64;
+  const size_t bytes = 2 * (size_t)w->maxwg * PW * 16 + 2 * PW * 16 + PB_WORDS * 4 + 64;
   HIPCHECK(hipMalloc(&w->mem, bytes));
   HIPCHECK(hipMemset(w->mem, 0, bytes));
   char *p = (char *)w->mem;
   w->pw.rowdata = p; p += 2 * (size_t)w->maxwg * PW * 16;
   w->pw.rowj = p; p += 2 * PW * 16;
-  w->pw.bar = (unsigned *)p; p += 64;
-  w->pw.key = (unsigned long long *)p; p += PW * 8;
+  w->pw.bar = (unsigned *)p;
+  w->pw.key = (unsigned long long *)(w->pw.bar + PB_LANES * PB_BAR_STRIDE);
+  p += PB_WORDS * 4;
   w->pw.flags = (int *)p;
   w->device = dev;
   return BFA_OK;
@@ -798,7 +824,7 @@ static int panel_base(cplx<T> *A, int m, int w, int64_t lda, int *ipiv, int piv_
   }
   if (nwg > resident[ti])
     return fail(BFA_ERR_HIP, "panel needs %d co-resident workgroups, device fits %d", nwg, resident[ti]);
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(128), 0, st, lw->pw.bar, 16 + 2 * PW);   // barrier and keys
+  hipLaunchKernelGGL(k_zero_u32, dim3((PB_WORDS + 255) / 256), dim3(256), 0, st, lw->pw.bar, PB_WORDS);   // barrier and keys
   hipLaunchKernelGGL(k_panel<T>, dim3(nwg), dim3(256), lds, st, A, m, w, lda, ipiv, rpw, lw->pw, piv_off);
   HIPCHECK(hipGetLastError());
   return BFA_OK;
@@ -1281,6 +1307,7 @@ void launch_sv_stream_g(bool lower, const SvProb *d, int np, int nblk, hipStream
   if (lower) hipLaunchKernelGGL(kg_sv_stream<true>, dim3(nblk), dim3(256), 0, st, d, np);
   else hipLaunchKernelGGL(kg_sv_stream<false>, dim3(nblk), dim3(256), 0, st, d, np);
 }
+int panel_sync_words() { return PB_WORDS; }
 unsigned long long sv_tag_c64() { return (unsigned long long)SvTag<float>::bits * 0x100000001ull; }
 
 void launch_gemm_g(bool f64, int tile, const GemmProb *d, int np, int nblk, hipStream_t st)
