@@ -424,6 +424,77 @@ static void trsm(int kb, int n, const cplx<T> *L, int64_t ldl, cplx<T> *B, int64
   }
 }
 
+// B(kb x n) := L^-1 B, L unit lower, kb <= TF_MAXK, in one launch.  A
+// workgroup owns a strip of TF_SC columns of B (column-major in LDS, 32 KB
+// so that it fits beside the trailing-update GEMM's tiles on a CU); per
+// 32-row block, the diagonal solve runs in registers (a wave takes two
+// columns, lane = row, the solved value passed along by shuffles, no
+// workgroup barrier), then every row below subtracts the block's
+// contribution from register accumulators.
+#define TF_MAXK 256
+template <typename T> struct TfShape { static constexpr int SC = 32768 / (TF_MAXK * (int)sizeof(cplx<T>)); };
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsm_lf(int kb, int n, const cplx<T> *__restrict__ L, int64_t ldl, cplx<T> *B,
+                                                 int64_t ldb)
+{
+  constexpr int SC = TfShape<T>::SC;
+  __shared__ cplx<T> X[SC][TF_MAXK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int c0 = blockIdx.x * SC, nc = min(SC, n - c0);
+  for (int idx = tid; idx < kb * SC; idx += blockDim.x) {
+    const int r = idx % kb, c = idx / kb;
+    X[c][r] = c < nc ? B[(size_t)(c0 + c) * ldb + r] : cplx<T>{0, 0};
+  }
+  __syncthreads();
+  const int row = lane & 31, half = lane >> 5;
+  for (int b0 = 0; b0 < kb; b0 += 32) {
+    const int bk = min(32, kb - b0);
+    cplx<T> lrow[32];
+#pragma unroll
+    for (int k = 0; k < 32; k++)
+      lrow[k] = (k < row && row < bk) ? L[(size_t)(b0 + k) * ldl + b0 + row] : cplx<T>{0, 0};
+    for (int pr = wave; pr < SC / 2; pr += 4) {
+      const int c = 2 * pr + half;
+      cplx<T> x = row < bk ? X[c][b0 + row] : cplx<T>{0, 0};
+#pragma unroll
+      for (int k = 0; k < 31; k++) {
+        const cplx<T> xk = {__shfl(x.re, k + 32 * half, 64), __shfl(x.im, k + 32 * half, 64)};
+        x = c_sub(x, c_mul(lrow[k], xk));   // lrow[k] = 0 unless k < row
+      }
+      if (row < bk) X[c][b0 + row] = x;
+    }
+    __syncthreads();
+    for (int r = b0 + bk + tid; r < kb; r += blockDim.x) {
+      cplx<T> acc[SC];
+#pragma unroll
+      for (int c = 0; c < SC; c++) acc[c] = X[c][r];
+      for (int k = 0; k < bk; k++) {
+        const cplx<T> l = L[(size_t)(b0 + k) * ldl + r];
+#pragma unroll
+        for (int c = 0; c < SC; c++) acc[c] = c_sub(acc[c], c_mul(l, X[c][b0 + k]));
+      }
+#pragma unroll
+      for (int c = 0; c < SC; c++) X[c][r] = acc[c];
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < kb * SC; idx += blockDim.x) {
+    const int r = idx % kb, c = idx / kb;
+    if (c < nc) B[(size_t)(c0 + c) * ldb + r] = X[c][r];
+  }
+}
+
+// B(kb x n) := L^-1 B, L unit lower: one launch up to TF_MAXK rows
+template <typename T>
+static void trsm_lower(int kb, int n, const cplx<T> *L, int64_t ldl, cplx<T> *B, int64_t ldb, hipStream_t st)
+{
+  if (kb <= 0 || n <= 0) return;
+  if (kb > TF_MAXK) { trsm<T, true>(kb, n, L, ldl, B, ldb, st); return; }
+  constexpr int SC = TfShape<T>::SC;
+  hipLaunchKernelGGL(k_trsm_lf<T>, dim3((n + SC - 1) / SC), dim3(256), 0, st, kb, n, L, ldl, B, ldb);
+}
+
 // ---------------------------------------------------------------------------
 // Row interchanges rows k0..k0+kb-1 <-> ipiv[.] (absolute, 0-based) on the
 // columns [c0, c1) of A; one thread per column, swaps applied in order.
@@ -842,7 +913,7 @@ static int panel_rec(cplx<T> *A, int m, int w, int64_t lda, int *ipiv, int piv_o
   cplx<T> *Ar = A + (size_t)w1 * lda;
   hipLaunchKernelGGL(k_laswp<T>, dim3((w2 + 127) / 128), dim3(128), 0, st, Ar - (ptrdiff_t)piv_off, lda, 0, w2, piv_off,
                      w1, (const int *)(ipiv - piv_off));
-  trsm<T, true>(w1, w2, A, lda, Ar, lda, st);
+  trsm_lower<T>(w1, w2, A, lda, Ar, lda, st);
   gemm_sub<T>(m - w1, w2, w1, A + w1, lda, Ar, lda, Ar + w1, lda, st);
   rc = panel_rec<T>(A + (size_t)w1 * lda + w1, m - w1, w2, lda, ipiv + w1, piv_off + w1, lw, st);
   if (rc) return rc;
@@ -889,14 +960,14 @@ int lu_factor(cplx<T> *A, int n, int64_t lda, int *ipiv, int *info, hipStream_t 
     cplx<T> *Anext = A + (size_t)next * lda;
     // critical path: block column next, then its panel
     laswp<T>(A, lda, next, next + nb2, j, jb, ipiv, st);
-    trsm<T, true>(jb, nb2, Ajj, lda, Anext + j, lda, st);
+    trsm_lower<T>(jb, nb2, Ajj, lda, Anext + j, lda, st);
     gemm_sub<T>(n - next, nb2, jb, Ajj + jb, lda, Anext + j, lda, Anext + next, lda, st);
     // bulk: the columns right of block column next
     const int rest = n - next - nb2;
     if (rest > 0) {
       cplx<T> *Arest = A + (size_t)(next + nb2) * lda;
       laswp<T>(A, lda, next + nb2, n, j, jb, ipiv, sb);
-      trsm<T, true>(jb, rest, Ajj, lda, Arest + j, lda, sb);
+      trsm_lower<T>(jb, rest, Ajj, lda, Arest + j, lda, sb);
       gemm_sub<T>(n - next, rest, jb, Ajj + jb, lda, Arest + j, lda, Arest + next, lda, sb);
     }
     HIPCHECK(hipEventRecord(ev_b, sb));
