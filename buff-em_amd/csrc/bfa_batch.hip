@@ -489,6 +489,23 @@ struct Planner {
   void trsm_lower_blocked(const std::vector<Pan> &ps, const std::vector<c2 *> &Bm, const std::vector<int> &ncols, int kb)
   {
     // B(kb x n) := L^-1 B, L unit lower (kb x kb at the top of the panel)
+    if (kb <= 256) {
+      const int sc = trsm_lf_cols(false);
+      std::vector<TrsmProb> v;
+      int blk = 0;
+      for (size_t k = 0; k < ps.size(); k++) {
+        if (ncols[k] <= 0) continue;
+        TrsmProb t{};
+        t.L = ps[k].A; t.B = Bm[k]; t.ldl = ps[k].lda; t.ldb = ps[k].lda; t.kb = kb; t.n = ncols[k]; t.blk0 = blk;
+        blk += (ncols[k] + sc - 1) / sc;
+        v.push_back(t);
+      }
+      if (v.empty()) return;
+      const size_t off = ar.put(v);
+      const int np = (int)v.size(), nblk = blk;
+      op([Bp = B, off, np, nblk](hipStream_t st) { launch_trsm_lf_g(false, dp<TrsmProb>(Bp, off), np, nblk, st); });
+      return;
+    }
     for (int i0 = 0; i0 < kb; i0 += 32) {
       const int ib = std::min(32, kb - i0);
       std::vector<TrsmProb> tp;

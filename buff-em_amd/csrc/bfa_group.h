@@ -79,6 +79,10 @@ struct SvProb {
 void launch_gemm_g(bool f64, int tile, const GemmProb *d, int np, int nblk, hipStream_t st);
 void launch_trsm_g(bool f64, bool lower, const TrsmProb *d, int np, int nblk, hipStream_t st);
 void launch_swap_g(bool f64, const SwapProb *d, int np, int nblk, hipStream_t st);
+// B(kb x n) := L^-1 B, L unit lower, kb <= 256, one launch: trsm_lf_cols()
+// columns of B per workgroup from blk0
+int trsm_lf_cols(bool f64);
+void launch_trsm_lf_g(bool f64, const TrsmProb *d, int np, int nblk, hipStream_t st);
 void launch_panel_g(bool f64, const PanelProb *d, int np, int nblk, size_t lds, hipStream_t st);
 void launch_sv_inv_g(const SvProb *d, int np, int nblk, hipStream_t st);            // complex64
 void launch_sv_stream_g(bool lower, const SvProb *d, int np, int nblk, hipStream_t st);

@@ -435,13 +435,12 @@ static void trsm(int kb, int n, const cplx<T> *L, int64_t ldl, cplx<T> *B, int64
 template <typename T> struct TfShape { static constexpr int SC = 32768 / (TF_MAXK * (int)sizeof(cplx<T>)); };
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_trsm_lf(int kb, int n, const cplx<T> *__restrict__ L, int64_t ldl, cplx<T> *B,
-                                                 int64_t ldb)
+__device__ __forceinline__ void trsm_lf_body(int kb, int n, const cplx<T> *__restrict__ L, int64_t ldl, cplx<T> *B,
+                                             int64_t ldb, int c0, cplx<T> (&X)[TfShape<T>::SC][TF_MAXK])
 {
   constexpr int SC = TfShape<T>::SC;
-  __shared__ cplx<T> X[SC][TF_MAXK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int c0 = blockIdx.x * SC, nc = min(SC, n - c0);
+  const int nc = min(SC, n - c0);
   for (int idx = tid; idx < kb * SC; idx += blockDim.x) {
     const int r = idx % kb, c = idx / kb;
     X[c][r] = c < nc ? B[(size_t)(c0 + c) * ldb + r] : cplx<T>{0, 0};
@@ -483,6 +482,14 @@ __global__ __launch_bounds__(256) void k_trsm_lf(int kb, int n, const cplx<T> *_
     const int r = idx % kb, c = idx / kb;
     if (c < nc) B[(size_t)(c0 + c) * ldb + r] = X[c][r];
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsm_lf(int kb, int n, const cplx<T> *__restrict__ L, int64_t ldl, cplx<T> *B,
+                                                 int64_t ldb)
+{
+  __shared__ cplx<T> X[TfShape<T>::SC][TF_MAXK];
+  trsm_lf_body<T>(kb, n, L, ldl, B, ldb, blockIdx.x * TfShape<T>::SC, X);
 }
 
 // B(kb x n) := L^-1 B, L unit lower: one launch up to TF_MAXK rows
@@ -793,6 +800,15 @@ __global__ __launch_bounds__(256) void kg_trsm(const TrsmProb *__restrict__ d, i
   const TrsmProb q = d[find_prob(d, np, blockIdx.x)];
   trsm_body<T, LOWER>(q.kb, q.n, (const cplx<T> *)q.L, q.ldl, (cplx<T> *)q.B, q.ldb,
                       (blockIdx.x - q.blk0) * TRSM_COLS, X, Ls);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kg_trsm_lf(const TrsmProb *__restrict__ d, int np)
+{
+  __shared__ cplx<T> X[TfShape<T>::SC][TF_MAXK];
+  const TrsmProb q = d[find_prob(d, np, blockIdx.x)];
+  trsm_lf_body<T>(q.kb, q.n, (const cplx<T> *)q.L, q.ldl, (cplx<T> *)q.B, q.ldb,
+                  (blockIdx.x - q.blk0) * TfShape<T>::SC, X);
 }
 
 template <typename T>
@@ -1403,6 +1419,14 @@ void launch_trsm_g(bool f64, bool lower, const TrsmProb *d, int np, int nblk, hi
     if (lower) hipLaunchKernelGGL((kg_trsm<float, true>), dim3(nblk), dim3(256), 0, st, d, np);
     else hipLaunchKernelGGL((kg_trsm<float, false>), dim3(nblk), dim3(256), 0, st, d, np);
   }
+}
+
+int trsm_lf_cols(bool f64) { return f64 ? TfShape<double>::SC : TfShape<float>::SC; }
+void launch_trsm_lf_g(bool f64, const TrsmProb *d, int np, int nblk, hipStream_t st)
+{
+  if (np <= 0 || nblk <= 0) return;
+  if (f64) hipLaunchKernelGGL(kg_trsm_lf<double>, dim3(nblk), dim3(256), 0, st, d, np);
+  else hipLaunchKernelGGL(kg_trsm_lf<float>, dim3(nblk), dim3(256), 0, st, d, np);
 }
 
 void launch_swap_g(bool f64, const SwapProb *d, int np, int nblk, hipStream_t st)
