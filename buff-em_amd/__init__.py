@@ -61,6 +61,12 @@ _SIGS = {
     "bfa_geometry_upload": (I, [P, P]),
     "bfa_static_build": (I, [P, I, P]),
     "bfa_static_download": (I, [P, P]),
+    "bfa_static_upload": (I, [P, P]),
+    "bfa_fibbi_keys": (I, [P, P]),
+    "bfa_fibbi_read": (I, [ctypes.c_char_p, P, P, P]),
+    "bfa_fibbi_write": (I, [ctypes.c_char_p, I64, P, P]),
+    "bfa_fibbi_store": (I, [P, ctypes.c_char_p, P]),
+    "bfa_fibbi_preload": (I, [P, ctypes.c_char_p, I, P, P, P]),
     "bfa_assemble_vie": (I, [P, D, D, P, P, I64, P]),
     "bfa_assemble_rhs": (I, [P, D, P, I, P, I64, P]),
     "bfa_assemble_vie_far32": (I, [P, D, D, P, P, I64, P]),
@@ -274,6 +280,32 @@ class Geometry:
         _check(lib().bfa_static_download(self.h, t.ctypes.data), "bfa_static_download")
         return t
 
+    def static_upload(self, table):
+        """Static records (nnz x 6, touching-pair CSR order) from the caller."""
+        t = np.ascontiguousarray(table, dtype=np.float64)
+        _check(lib().bfa_static_upload(self.h, t.ctypes.data), "bfa_static_upload")
+
+    def fibbi_keys(self):
+        """FIBBI cache search keys (nnz x 27 float32) of the touching pairs."""
+        nnz = I64()
+        _check(lib().bfa_geometry_touch_count(self.h, ctypes.byref(nnz)), "bfa_geometry_touch_count")
+        k = np.zeros((nnz.value, 27), dtype=np.float32)
+        _check(lib().bfa_fibbi_keys(self.h, k.ctypes.data), "bfa_fibbi_keys")
+        return k
+
+    def fibbi_store(self, path):
+        """Write the static table as a FIBBI cache file; returns the record count."""
+        n = I64()
+        _check(lib().bfa_fibbi_store(self.h, os.fsencode(path), ctypes.byref(n)), "bfa_fibbi_store")
+        return n.value
+
+    def fibbi_preload(self, path, td_order=0, stream=None):
+        """Static table from a FIBBI cache file (misses computed); returns (hits, misses)."""
+        h, m = I64(), I64()
+        _check(lib().bfa_fibbi_preload(self.h, os.fsencode(path), int(td_order), ctypes.byref(h), ctypes.byref(m),
+                                       _stream_ptr(stream)), "bfa_fibbi_preload")
+        return h.value, m.value
+
     def assemble_vie(self, omega, mat, d_M, ld=None, stream=None, far32=False):
         """VIE matrix at omega; far32 evaluates the far-field cubature in FP32."""
         omega = complex(omega)
@@ -419,6 +451,25 @@ class Batch:
         T = np.zeros((n, n), np.complex128)
         _check(lib().bfa_batch_matrix(self._h, int(i), T.ctypes.data), "bfa_batch_matrix")
         return T.T
+
+
+def fibbi_read(path):
+    """Records of a FIBBI cache file: (keys [n, 27] float32, data [n, 6] float64)."""
+    n = I64(0)
+    _check(lib().bfa_fibbi_read(os.fsencode(path), ctypes.byref(n), None, None), "bfa_fibbi_read")
+    k = np.zeros((n.value, 27), dtype=np.float32)
+    d = np.zeros((n.value, 6))
+    _check(lib().bfa_fibbi_read(os.fsencode(path), ctypes.byref(n), k.ctypes.data, d.ctypes.data), "bfa_fibbi_read")
+    return k, d
+
+
+def fibbi_write(path, keys, data):
+    """Write records as a FIBBI cache file (the reference's format)."""
+    k = np.ascontiguousarray(keys, dtype=np.float32).reshape(-1, 27)
+    d = np.ascontiguousarray(data, dtype=np.float64).reshape(-1, 6)
+    if len(k) != len(d):
+        raise ValueError("keys and data differ in length")
+    _check(lib().bfa_fibbi_write(os.fsencode(path), len(k), k.ctypes.data, d.ctypes.data), "bfa_fibbi_write")
 
 
 def mlp_param_count(nin, hidden=256, nout=32):

@@ -557,6 +557,18 @@ extern "C" int bfa_static_build(bfa_geometry *g, int td_order, void *stream)
   return BFA_OK;
 }
 
+// static records supplied by the caller (nnz x 6, CSR order), e.g. from a
+// FIBBI cache file; replaces any table built before
+extern "C" int bfa_static_upload(bfa_geometry *g, const double *table)
+{
+  if (!g || !g->uploaded) return fail(BFA_ERR_STATE, "bfa_static_upload: geometry not uploaded");
+  if (!table) return fail(BFA_ERR_ARG, "bfa_static_upload: null table");
+  int rc = dev_upload(&g->d.table, table, (size_t)std::max<int64_t>(g->dg.nnz, 1) * 6 * 8);
+  if (rc) return rc;
+  g->static_built = true;
+  return BFA_OK;
+}
+
 extern "C" int bfa_static_download(const bfa_geometry *g, double *table)
 {
   if (!g || !g->static_built) return fail(BFA_ERR_STATE, "static table not built");
@@ -566,10 +578,10 @@ extern "C" int bfa_static_download(const bfa_geometry *g, double *table)
 }
 
 // ===========================================================================
-// Warm step: dense far field.  Workgroup = 64 x 64 tile of the upper
-// triangle; lane = row a, each wave sweeps 16 columns b (uniform across the
-// wave: column point data broadcast from LDS).  Touching pairs are skipped
-// (written by the sparse near-field pass).
+// Warm step: dense far field over the upper triangle (touching pairs are
+// skipped: the sparse near-field pass writes them).  FP64: tetrahedron-pair
+// tiles (k_far_tp); FP32 (precision form): 64 x 64 BF tiles, lane = row a,
+// each wave sweeping 16 columns b whose points are broadcast from LDS.
 // ===========================================================================
 #define TILE 64
 

@@ -21,6 +21,8 @@
 #pragma once
 #include <complex>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -128,8 +130,13 @@ class SWGGeometry {
  public:
   int TotalBFs = 0;
 
-  // GMSH .msh/.vmsh mesh, one object, the given material
-  SWGGeometry(const char *MeshFile, const bfa_material &mat, int td_order = 0) : mat_(mat)
+  // GMSH .msh/.vmsh mesh, one object, the given material.  As the
+  // reference's per-mesh FIBBI cache (FIBBICache.cc:108-143, VIEMatrix.cc:
+  // 382-424): the static records are preloaded from
+  // ${BUFF_CACHE_PATH}/<mesh base>.cache, else ./<mesh base>.cache, records
+  // missing there are computed, and the cache is written back after the
+  // next AssembleVIEMatrix when anything was computed.
+  SWGGeometry(const char *MeshFile, const bfa_material &mat, int td_order = 0) : mat_(mat), mesh_(MeshFile)
   {
     int nv = 0, nt = 0;
     check(bfa_mesh_read_vmsh(MeshFile, &nv, &nt, nullptr, nullptr), "mesh size");
@@ -167,6 +174,7 @@ class SWGGeometry {
     check(bfa_assemble_vie(g_, Omega.real(), Omega.imag(), &mat_, M->data(), TotalBFs, nullptr),
           "AssembleVIEMatrix");
     check(bfa_stream_synchronize(nullptr), "AssembleVIEMatrix sync");
+    store_cache();   // GCache->Store(OA->MeshFileName), VIEMatrix.cc:420-424
     return M;
   }
 
@@ -222,10 +230,56 @@ class SWGGeometry {
     check(bfa_geometry_upload(g_, nullptr), "geometry upload");
     // the frequency-independent static table plays the role of the
     // reference's per-mesh FIBBI cache (FIBBICache.cc:108-143)
+    if (!mesh_.empty()) {
+      for (const std::string &path : cache_candidates()) {
+        FILE *f = fopen(path.c_str(), "rb");
+        if (!f) continue;
+        fclose(f);
+        int64_t hits = 0, misses = 0;
+        if (bfa_fibbi_preload(g_, path.c_str(), td_order, &hits, &misses, nullptr) == BFA_OK) {
+          fprintf(stderr, "FIBBI cache: preloaded %s (%lld hits, %lld computed)\n", path.c_str(), (long long)hits,
+                  (long long)misses);
+          cache_dirty_ = misses > 0;
+          return;
+        }
+        fprintf(stderr, "FIBBI cache: %s: %s (skipping cache preload)\n", path.c_str(), bfa_last_error());
+      }
+      cache_dirty_ = true;
+    }
     check(bfa_static_build(g_, td_order, nullptr), "static table");
+  }
+  // FIBBICache::Store's file name: ${BUFF_CACHE_PATH}/<base>.cache or
+  // ./<base>.cache, <base> = mesh file name without directory and extension
+  std::string cache_base() const
+  {
+    std::string b = mesh_.substr(mesh_.find_last_of('/') == std::string::npos ? 0 : mesh_.find_last_of('/') + 1);
+    const size_t dot = b.find_last_of('.');
+    if (dot != std::string::npos && dot > 0) b.resize(dot);
+    return b + ".cache";
+  }
+  std::vector<std::string> cache_candidates() const
+  {
+    std::vector<std::string> v;
+    if (const char *dir = getenv("BUFF_CACHE_PATH")) v.push_back(std::string(dir) + "/" + cache_base());
+    v.push_back(cache_base());
+    return v;
+  }
+  void store_cache()
+  {
+    if (mesh_.empty() || !cache_dirty_) return;
+    const std::string path = cache_candidates().front();
+    int64_t n = 0;
+    if (bfa_fibbi_store(g_, path.c_str(), &n) == BFA_OK) {
+      fprintf(stderr, "FIBBI cache: wrote %lld records to %s\n", (long long)n, path.c_str());
+      cache_dirty_ = false;
+    } else {
+      fprintf(stderr, "FIBBI cache: %s (aborting cache dump)\n", bfa_last_error());
+    }
   }
   bfa_geometry *g_ = nullptr;
   bfa_material mat_;
+  std::string mesh_;
+  bool cache_dirty_ = false;
 };
 
 }  // namespace buffamd

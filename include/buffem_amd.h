@@ -92,6 +92,26 @@ int  bfa_geometry_upload(bfa_geometry *g, void *stream);
  * GMatrixElements.cc:295-375); Taylor-Duffy order td_order (<=0: default). */
 int  bfa_static_build(bfa_geometry *g, int td_order, void *stream);
 int  bfa_static_download(const bfa_geometry *g, double *table /* nnz*6 */);
+/* records supplied by the caller (nnz*6, touching-pair CSR order); marks
+ * the static table built without the cold step                             */
+int  bfa_static_upload(bfa_geometry *g, const double *table);
+
+/* ---- FIBBI cache files (FIBBICache.cc:247-463) ------------------------- */
+/* "FIBBI_GCACHE\0" + records {27-float key, 6-double record}; the key of a
+ * touching pair a<=b is GetFIBBICacheKey's (FIBBICache.cc:171-203).        */
+/* keys[nnz*27] of the touching pairs in CSR order                          */
+int  bfa_fibbi_keys(const bfa_geometry *g, float *keys);
+/* raw file access: with keys == data == NULL, *nrec receives the count;
+ * otherwise *nrec is the room (records) and receives the count             */
+int  bfa_fibbi_read(const char *path, int64_t *nrec, float *keys, double *data);
+int  bfa_fibbi_write(const char *path, int64_t nrec, const float *keys, const double *data);
+/* FIBBICache::Store: each distinct key once (first pair's record)          */
+int  bfa_fibbi_store(const bfa_geometry *g, const char *path, int64_t *written);
+/* FIBBICache::PreLoad then lookup of every touching pair: hits take the
+ * file's record; misses are computed by bfa_static_build(td_order); no
+ * misses skips the cold step                                               */
+int  bfa_fibbi_preload(bfa_geometry *g, const char *path, int td_order, int64_t *hits,
+                       int64_t *misses, void *stream);
 
 /* ---- warm step: VIE matrix --------------------------------------------- */
 /* SWGGeometry::AssembleVIEMatrix (VIEMatrix.cc:431-460): d_M (N x N,
