@@ -34,7 +34,7 @@ class BFAError(RuntimeError):
 
 
 class bfa_material(ctypes.Structure):
-    _fields_ = [("kind", ctypes.c_int), ("p", ctypes.c_double * 6)]
+    _fields_ = [("kind", ctypes.c_int), ("p", ctypes.c_double * 6), ("expr", ctypes.c_void_p)]
 
 
 class bfa_planewave(ctypes.Structure):
@@ -62,6 +62,13 @@ _SIGS = {
     "bfa_static_build": (I, [P, I, P]),
     "bfa_static_download": (I, [P, P]),
     "bfa_static_upload": (I, [P, P]),
+    "bfa_material_svtensor": (I, [ctypes.c_char_p, I, P]),
+    "bfa_material_expr": (I, [ctypes.c_char_p, P]),
+    "bfa_material_release": (I, [P]),
+    "bfa_material_check": (I, [ctypes.c_char_p, I]),
+    "bfa_material_expr_check": (I, [ctypes.c_char_p]),
+    "bfa_material_eval": (I, [P, D, I, P, P]),
+    "bfa_buffgeo_load": (I, [ctypes.c_char_p, P, P, P, P, P]),
     "bfa_fibbi_keys": (I, [P, P]),
     "bfa_fibbi_read": (I, [ctypes.c_char_p, P, P, P]),
     "bfa_fibbi_write": (I, [ctypes.c_char_p, I64, P, P]),
@@ -180,6 +187,61 @@ def material(kind="iso", **kw):
     for i in range(6):
         m.p[i] = p[i]
     return m
+
+
+def _owned(m):
+    """Release the device program of a compiled material when it is collected."""
+    import weakref
+    keep = bfa_material()
+    ctypes.pointer(keep)[0] = m
+    weakref.finalize(m, lambda k=keep: lib().bfa_material_release(ctypes.byref(k)))
+    return m
+
+
+def material_svtensor(name, matprop=False):
+    """Compiled material from an SVTensor file or a MatProp name (CONST_EPS_<value>, MATERIAL sections)."""
+    m = bfa_material()
+    _check(lib().bfa_material_svtensor(os.fsencode(name), int(bool(matprop)), ctypes.byref(m)),
+           "bfa_material_svtensor")
+    return _owned(m)
+
+
+def material_expr(eps):
+    """Compiled isotropic material eps(w, x, y, z, r, Theta, Phi) from one expression."""
+    m = bfa_material()
+    _check(lib().bfa_material_expr(eps.encode(), ctypes.byref(m)), "bfa_material_expr")
+    return _owned(m)
+
+
+def material_check(name=None, matprop=False, expr=None):
+    """Parse and compile a material without a device (raises BFAError on errors)."""
+    if expr is not None:
+        _check(lib().bfa_material_expr_check(expr.encode()), "bfa_material_expr_check")
+    else:
+        _check(lib().bfa_material_check(os.fsencode(name), int(bool(matprop))), "bfa_material_check")
+
+
+def material_eval(mat, omega, pts):
+    """Permittivity tensors at points pts [n, 3] (device evaluation) -> [n, 3, 3] complex."""
+    x = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    q = np.zeros((len(x), 9, 2))
+    _check(lib().bfa_material_eval(ctypes.byref(mat), float(omega), len(x), x.ctypes.data, q.ctypes.data),
+           "bfa_material_eval")
+    return (q[..., 0] + 1j * q[..., 1]).reshape(-1, 3, 3)
+
+
+def buffgeo_load(path):
+    """.buffgeo file -> (V [nv, 3] transformed, T [nt, 4], compiled material)."""
+    L = lib()
+    nv, nt = I(), I()
+    b = os.fsencode(path)
+    _check(L.bfa_buffgeo_load(b, ctypes.byref(nv), ctypes.byref(nt), None, None, None), "bfa_buffgeo_load")
+    V = np.zeros((nv.value, 3))
+    T = np.zeros((nt.value, 4), dtype=np.int32)
+    m = bfa_material()
+    _check(L.bfa_buffgeo_load(b, ctypes.byref(nv), ctypes.byref(nt), V.ctypes.data, T.ctypes.data, ctypes.byref(m)),
+           "bfa_buffgeo_load")
+    return V, T, _owned(m)
 
 
 def planewaves(specs):

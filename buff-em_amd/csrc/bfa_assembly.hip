@@ -25,6 +25,7 @@
 #include "bfa_internal.h"
 #include "bfa_group.h"
 #include "bfa_rules.h"
+#include "bfa_expr.h"
 #define TD_QUAL __constant__
 #include "td_regions.h"
 
@@ -880,33 +881,158 @@ __device__ __forceinline__ cd exprel3(cd x)
   return cadd(first, sum);
 }
 
-__device__ void eps_at(const bfa_material &m, const double *x, cd eps[3])
+// ---- compiled SVTensor expressions (bfa_expr.h): complex functions and a
+// postfix interpreter over a small complex stack
+__device__ __forceinline__ cd csub(cd a, cd b) { return {a.re - b.re, a.im - b.im}; }
+__device__ cd cexp_d(cd a)
 {
+  double s, c;
+  sincos(a.im, &s, &c);
+  const double e = exp(a.re);
+  return {e * c, e * s};
+}
+__device__ cd clog_d(cd a) { return {log(hypot(a.re, a.im)), atan2(a.im, a.re)}; }
+__device__ cd csqrt_d(cd a)
+{
+  const double m = hypot(a.re, a.im);
+  if (m == 0.0) return {0.0, 0.0};
+  const double t = sqrt(0.5 * (m + fabs(a.re)));
+  return a.re >= 0.0 ? cd{t, a.im / (2.0 * t)} : cd{fabs(a.im) / (2.0 * t), a.im >= 0.0 ? t : -t};
+}
+__device__ cd csin_d(cd a) { double s, c; sincos(a.re, &s, &c); return {s * cosh(a.im), c * sinh(a.im)}; }
+__device__ cd ccos_d(cd a) { double s, c; sincos(a.re, &s, &c); return {c * cosh(a.im), -s * sinh(a.im)}; }
+__device__ cd csinh_d(cd a) { double s, c; sincos(a.im, &s, &c); return {sinh(a.re) * c, cosh(a.re) * s}; }
+__device__ cd ccosh_d(cd a) { double s, c; sincos(a.im, &s, &c); return {cosh(a.re) * c, sinh(a.re) * s}; }
+// principal branches: asin z = -i log(iz + sqrt(1 - z^2)), acos z = pi/2 - asin z,
+// atan z = (i/2) log((i + z)/(i - z))
+__device__ cd casin_d(cd z)
+{
+  const cd w = cadd(cd{-z.im, z.re}, csqrt_d(csub(cd{1.0, 0.0}, cmul(z, z))));
+  const cd l = clog_d(w);
+  return {l.im, -l.re};
+}
+__device__ cd catan_d(cd z)
+{
+  const cd l = clog_d(cdiv(cd{z.re, 1.0 + z.im}, cd{-z.re, 1.0 - z.im}));
+  return {-0.5 * l.im, 0.5 * l.re};
+}
+__device__ cd expr_fn(int f, cd a)
+{
+  switch (f) {
+    case EFN_EXP: return cexp_d(a);
+    case EFN_LOG: return clog_d(a);
+    case EFN_SQRT: return csqrt_d(a);
+    case EFN_SIN: return csin_d(a);
+    case EFN_COS: return ccos_d(a);
+    case EFN_TAN: return cdiv(csin_d(a), ccos_d(a));
+    case EFN_COT: return cdiv(ccos_d(a), csin_d(a));
+    case EFN_SEC: return cdiv(cd{1.0, 0.0}, ccos_d(a));
+    case EFN_CSC: return cdiv(cd{1.0, 0.0}, csin_d(a));
+    case EFN_ASIN: return casin_d(a);
+    case EFN_ACOS: { const cd s = casin_d(a); return {0.5 * M_PI - s.re, -s.im}; }
+    case EFN_ATAN: return catan_d(a);
+    case EFN_SINH: return csinh_d(a);
+    case EFN_COSH: return ccosh_d(a);
+    case EFN_TANH: return cdiv(csinh_d(a), ccosh_d(a));
+    case EFN_ABS: return {hypot(a.re, a.im), 0.0};
+    case EFN_STEP: return {a.re >= 0.0 ? 1.0 : 0.0, 0.0};
+    default: return {0.0, 0.0};
+  }
+}
+// a^b = exp(b log a); integer powers by repeated products (exact w^2 etc.)
+__device__ cd expr_pow(cd a, cd b)
+{
+  if (b.im == 0.0 && b.re == floor(b.re) && fabs(b.re) <= 16.0) {
+    int n = (int)fabs(b.re);
+    cd r = {1.0, 0.0}, x = a;
+    while (n) {
+      if (n & 1) r = cmul(r, x);
+      x = cmul(x, x);
+      n >>= 1;
+    }
+    return b.re < 0.0 ? cdiv(cd{1.0, 0.0}, r) : r;
+  }
+  if (a.re == 0.0 && a.im == 0.0) return {0.0, 0.0};
+  return cexp_d(cmul(b, clog_d(a)));
+}
+__device__ cd expr_run(const ExprHeader *h, int slot, const cd *vars)
+{
+  const ExprOp *ops = (const ExprOp *)(h + 1);
+  const double *k = (const double *)(ops + h->nops);
+  cd st[EXPR_STACK];
+  int sp = 0;
+  for (int i = h->start[slot], e = h->start[slot] + h->len[slot]; i < e; i++) {
+    const ExprOp o = ops[i];
+    if (o.op == EOP_CONST) { st[sp++] = {k[2 * o.arg], k[2 * o.arg + 1]}; continue; }
+    if (o.op == EOP_VAR) { st[sp++] = vars[o.arg]; continue; }
+    if (o.op == EOP_NEG) { st[sp - 1] = {-st[sp - 1].re, -st[sp - 1].im}; continue; }
+    if (o.op == EOP_FN) { st[sp - 1] = expr_fn(o.arg, st[sp - 1]); continue; }
+    const cd b = st[--sp], a = st[sp - 1];
+    st[sp - 1] = o.op == EOP_ADD ? cadd(a, b) : o.op == EOP_SUB ? csub(a, b) : o.op == EOP_MUL ? cmul(a, b)
+               : o.op == EOP_DIV ? cdiv(a, b) : expr_pow(a, b);
+  }
+  return st[0];
+}
+
+// Permittivity tensor at x (SVTensor::Evaluate, SVTensor.cc:306-373); kinds
+// 0-3 are the closed forms documented with bfa_material, kind 4 a compiled
+// SVTensor evaluated at w = omega * FreqUnit.
+__device__ void eps_tensor(const bfa_material &m, const double *x, double omega, cd Q[3][3])
+{
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) Q[r][c] = {0.0, 0.0};
   switch (m.kind) {
     case 1:
-      eps[0] = {m.p[0], m.p[1]}; eps[1] = {m.p[2], m.p[3]}; eps[2] = {m.p[4], m.p[5]};
-      break;
+      Q[0][0] = {m.p[0], m.p[1]}; Q[1][1] = {m.p[2], m.p[3]}; Q[2][2] = {m.p[4], m.p[5]};
+      return;
     case 2: {
-      double re = m.p[0] + m.p[2] * sin(M_PI * x[0]) * cos(M_PI * x[1]);
-      eps[0] = eps[1] = eps[2] = {re, m.p[1]};
-    } break;
+      const double re = m.p[0] + m.p[2] * sin(M_PI * x[0]) * cos(M_PI * x[1]);
+      Q[0][0] = Q[1][1] = Q[2][2] = {re, m.p[1]};
+    } return;
     case 3: {
-      bool up = x[2] > 0.0;
-      cd e = up ? cd{m.p[0], m.p[1]} : cd{m.p[2], m.p[3]};
-      eps[0] = eps[1] = eps[2] = e;
-    } break;
+      const bool up = x[2] > 0.0;
+      Q[0][0] = Q[1][1] = Q[2][2] = up ? cd{m.p[0], m.p[1]} : cd{m.p[2], m.p[3]};
+    } return;
+    case 4: {
+      const ExprHeader *h = (const ExprHeader *)m.expr;
+      cd v[EV_COUNT];
+      const double rho = sqrt(x[0] * x[0] + x[1] * x[1]);
+      v[EV_W] = {omega * h->freq_unit, 0.0};
+      v[EV_X] = {x[0], 0.0}; v[EV_Y] = {x[1], 0.0}; v[EV_Z] = {x[2], 0.0};
+      v[EV_R] = {sqrt(rho * rho + x[2] * x[2]), 0.0};
+      v[EV_THETA] = {atan2(rho, x[2]), 0.0};
+      v[EV_PHI] = {atan2(x[1], x[0]), 0.0};
+      v[EV_EPS1] = v[EV_EPS2] = v[EV_EPS3] = {0.0, 0.0};
+      for (int k = 0; k < h->num_mp && k < 3; k++) v[EV_EPS1 + k] = expr_run(h, 9 + k, v);
+      if (h->matprop) {
+        Q[0][0] = Q[1][1] = Q[2][2] = v[EV_EPS1];
+        return;
+      }
+      for (int s = 0; s < 9; s++) {
+        if (h->len[s]) Q[s / 3][s % 3] = expr_run(h, s, v);
+        else if (s % 4 == 0) Q[s / 3][s % 3] = {1.0, 0.0};
+      }
+      for (int mu = 0; mu < 3; mu++) {
+        const int nu = (mu + 1) % 3;
+        const cd avg = cscale(cadd(Q[mu][nu], Q[nu][mu]), 0.5);
+        Q[mu][nu] = Q[nu][mu] = avg;
+      }
+      if (h->len[0]) {
+        if (!h->len[4]) Q[1][1] = Q[0][0];
+        if (!h->len[8]) Q[2][2] = Q[0][0];
+      }
+    } return;
     default:
-      eps[0] = eps[1] = eps[2] = {m.p[0], m.p[1]};
+      Q[0][0] = Q[1][1] = Q[2][2] = {m.p[0], m.p[1]};
   }
 }
 
 // (eps - 1)^-1 by the cofactor formula of Invert3x3Matrix (VIEMatrix.cc:51-72)
-__device__ void inv_chi(const cd e[3], cd W[3][3])
+__device__ void inv_chi(const cd Q[3][3], cd W[3][3])
 {
   cd Mm[3][3];
   for (int r = 0; r < 3; r++)
-    for (int c = 0; c < 3; c++) Mm[r][c] = {0.0, 0.0};
-  for (int r = 0; r < 3; r++) Mm[r][r] = {e[r].re - 1.0, e[r].im};
+    for (int c = 0; c < 3; c++) Mm[r][c] = r == c ? cd{Q[r][c].re - 1.0, Q[r][c].im} : Q[r][c];
   auto sub = [](cd a, cd b) { return cd{a.re - b.re, a.im - b.im}; };
   cd M11 = Mm[0][0], M12 = Mm[0][1], M13 = Mm[0][2], M21 = Mm[1][0], M22 = Mm[1][1], M23 = Mm[1][2];
   cd M31 = Mm[2][0], M32 = Mm[2][1], M33 = Mm[2][2];
@@ -923,6 +1049,40 @@ __device__ void inv_chi(const cd e[3], cd W[3][3])
   W[2][2] = cdiv(sub(cmul(M11, M22), cmul(M12, M21)), Det);
 }
 
+// permittivity tensors at points (row-major 3x3 complex per point)
+__global__ void k_eps_points(bfa_material m, double omega, int n, const double *pts, double *Q)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  cd T[3][3];
+  eps_tensor(m, pts + 3 * (size_t)i, omega, T);
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) {
+      Q[18 * (size_t)i + 2 * (3 * r + c)] = T[r][c].re;
+      Q[18 * (size_t)i + 2 * (3 * r + c) + 1] = T[r][c].im;
+    }
+}
+
+extern "C" int bfa_material_eval(const bfa_material *mat, double omega, int npts, const double *pts, double *Q)
+{
+  if (!mat || npts < 0 || (npts > 0 && (!pts || !Q))) return fail(BFA_ERR_ARG, "bfa_material_eval: bad argument");
+  if (mat->kind == 4 && !mat->expr) return fail(BFA_ERR_ARG, "bfa_material_eval: compiled material without program");
+  if (npts == 0) return BFA_OK;
+  double *dp = nullptr, *dq = nullptr;
+  HIPCHECK(hipMalloc(&dp, (size_t)npts * 3 * 8));
+  hipError_t e = hipMalloc(&dq, (size_t)npts * 18 * 8);
+  if (e != hipSuccess) { (void)hipFree(dp); return fail(BFA_ERR_OOM, "bfa_material_eval: allocation"); }
+  int rc = BFA_OK;
+  if (hipMemcpy(dp, pts, (size_t)npts * 24, hipMemcpyHostToDevice) != hipSuccess) rc = fail(BFA_ERR_HIP, "upload");
+  if (!rc) {
+    hipLaunchKernelGGL(k_eps_points, dim3((npts + 127) / 128), dim3(128), 0, 0, *mat, omega, npts, dp, dq);
+    if (hipMemcpy(Q, dq, (size_t)npts * 18 * 8, hipMemcpyDeviceToHost) != hipSuccess) rc = fail(BFA_ERR_HIP, "download");
+  }
+  (void)hipFree(dp);
+  (void)hipFree(dq);
+  return rc;
+}
+
 // -(1/omega^2) int_t F_a . chi^-1 . F_b  over TetCR33 (GetOverlapIntegrand)
 __device__ cd overlap_tet(const DevGeom &g, int t, const double *QA, double pfA, const double *QB,
                           double pfB, double omega, const bfa_material &mat)
@@ -931,9 +1091,9 @@ __device__ cd overlap_tet(const DevGeom &g, int t, const double *QA, double pfA,
   cd acc = {0.0, 0.0};
   for (int i = 0; i < 33; i++) {
     const double x[3] = {q[4 * i], q[4 * i + 1], q[4 * i + 2]};
-    cd e[3], W[3][3];
-    eps_at(mat, x, e);
-    inv_chi(e, W);
+    cd Q[3][3], W[3][3];
+    eps_tensor(mat, x, omega, Q);
+    inv_chi(Q, W);
     double FA[3], FB[3];
     for (int d = 0; d < 3; d++) { FA[d] = pfA * (x[d] - QA[d]); FB[d] = pfB * (x[d] - QB[d]); }
     cd v = {0.0, 0.0};

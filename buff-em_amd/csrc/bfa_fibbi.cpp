@@ -61,21 +61,21 @@ std::string key_bytes(const float *k) { return std::string((const char *)k, kKey
 int read_file(const char *path, std::vector<float> &keys, std::vector<double> &data)
 {
   FILE *f = fopen(path, "rb");
-  if (!f) return fail(BFA_ERR_ARG, "FIBBI cache %s: cannot open", path);
+  if (!f) return fail(BFA_ERR_IO, "FIBBI cache %s: cannot open", path);
   struct stat st;
   if (fstat(fileno(f), &st) != 0 || st.st_size < (off_t)kSigSize) {
     fclose(f);
-    return fail(BFA_ERR_ARG, "FIBBI cache %s: invalid cache file", path);
+    return fail(BFA_ERR_IO, "FIBBI cache %s: invalid cache file", path);
   }
   char sig[kSigSize];
   if (fread(sig, kSigSize, 1, f) != 1 || memcmp(sig, kSignature, kSigSize) != 0) {
     fclose(f);
-    return fail(BFA_ERR_ARG, "FIBBI cache %s: invalid cache file", path);
+    return fail(BFA_ERR_IO, "FIBBI cache %s: invalid cache file", path);
   }
   const size_t body = (size_t)st.st_size - kSigSize, rec = kKeySize + kDataSize;
   if (body % rec != 0) {
     fclose(f);
-    return fail(BFA_ERR_ARG, "FIBBI cache %s: cache file has incorrect size", path);
+    return fail(BFA_ERR_IO, "FIBBI cache %s: cache file has incorrect size", path);
   }
   const size_t n = body / rec;
   keys.resize(n * kKeyLen);
@@ -83,7 +83,7 @@ int read_file(const char *path, std::vector<float> &keys, std::vector<double> &d
   for (size_t i = 0; i < n; i++) {
     if (fread(&keys[i * kKeyLen], kKeySize, 1, f) != 1 || fread(&data[i * kDataLen], kDataSize, 1, f) != 1) {
       fclose(f);
-      return fail(BFA_ERR_ARG, "FIBBI cache %s: read only %zu/%zu records", path, i, n);
+      return fail(BFA_ERR_IO, "FIBBI cache %s: read only %zu/%zu records", path, i, n);
     }
   }
   fclose(f);
@@ -94,7 +94,7 @@ int write_file(const char *path, size_t n, const float *keys, const double *data
                int64_t *written)
 {
   FILE *f = fopen(path, "wb");
-  if (!f) return fail(BFA_ERR_ARG, "FIBBI cache %s: cannot open for writing", path);
+  if (!f) return fail(BFA_ERR_IO, "FIBBI cache %s: cannot open for writing", path);
   int64_t w = 0;
   bool ok = fwrite(kSignature, kSigSize, 1, f) == 1;
   for (size_t i = 0; ok && i < n; i++) {
@@ -103,7 +103,7 @@ int write_file(const char *path, size_t n, const float *keys, const double *data
     w += ok;
   }
   ok = (fclose(f) == 0) && ok;
-  if (!ok) return fail(BFA_ERR_ARG, "FIBBI cache %s: write failed", path);
+  if (!ok) return fail(BFA_ERR_IO, "FIBBI cache %s: write failed", path);
   if (written) *written = w;
   return BFA_OK;
 }

@@ -6,7 +6,8 @@
 // assemble the RHS, solve in place and report the EMT absorbed/scattered
 // power (the --PFTFile output, OutputModules.cc:149-211).
 //
-//   buff_scatter_amd (--vmsh FILE | --kuhn n R) --eps RE IM
+//   buff_scatter_amd (--geometry FILE.buffgeo | --vmsh FILE | --kuhn n R)
+//                    [--eps RE IM]  (material of --vmsh / --kuhn)
 //                    --omega W [--omega W ...]
 //                    --pw EX EY EZ NX NY NZ [--pw ...]
 // Output: one line per (omega, incident field): omega  ifield  P_abs  P_scat
@@ -21,7 +22,7 @@ using namespace buffamd;
 
 int main(int argc, char **argv)
 {
-  const char *vmsh = nullptr;
+  const char *vmsh = nullptr, *geo = nullptr;
   int kuhn_n = 0;
   double R = 1.0, er = 4.0, ei = 0.1;
   std::vector<double> omegas;
@@ -31,6 +32,7 @@ int main(int argc, char **argv)
       if (i + k >= argc) { fprintf(stderr, "missing value for %s\n", argv[i]); exit(2); }
     };
     if (!strcmp(argv[i], "--vmsh")) { need(1); vmsh = argv[++i]; }
+    else if (!strcmp(argv[i], "--geometry")) { need(1); geo = argv[++i]; }
     else if (!strcmp(argv[i], "--kuhn")) { need(2); kuhn_n = atoi(argv[++i]); R = atof(argv[++i]); }
     else if (!strcmp(argv[i], "--eps")) { need(2); er = atof(argv[++i]); ei = atof(argv[++i]); }
     else if (!strcmp(argv[i], "--omega")) { need(1); omegas.push_back(atof(argv[++i])); }
@@ -42,13 +44,15 @@ int main(int argc, char **argv)
       fields.push_back(p);
     } else { fprintf(stderr, "unknown option %s\n", argv[i]); return 2; }
   }
-  if ((!vmsh && kuhn_n <= 0) || omegas.empty() || fields.empty()) {
-    fprintf(stderr, "usage: %s (--vmsh FILE | --kuhn n R) --eps RE IM --omega W --pw EX EY EZ NX NY NZ\n", argv[0]);
+  if ((!vmsh && !geo && kuhn_n <= 0) || omegas.empty() || fields.empty()) {
+    fprintf(stderr, "usage: %s (--geometry FILE.buffgeo | --vmsh FILE | --kuhn n R) [--eps RE IM] --omega W "
+                    "--pw EX EY EZ NX NY NZ\n", argv[0]);
     return 2;
   }
   try {
     bfa_material mat{0, {er, ei, 0, 0, 0, 0}};
-    SWGGeometry *G = vmsh ? new SWGGeometry(vmsh, mat) : new SWGGeometry(kuhn_n, R, 0.05, 0x5EED, mat);
+    SWGGeometry *G = geo ? new SWGGeometry(geo)
+                   : vmsh ? new SWGGeometry(vmsh, mat) : new SWGGeometry(kuhn_n, R, 0.05, 0x5EED, mat);
     HMatrix *M = G->AllocateVIEMatrix();
     HVector *J = G->AllocateRHSVector();
     for (double w : omegas) {

@@ -145,6 +145,20 @@ class SWGGeometry {
     check(bfa_mesh_read_vmsh(MeshFile, &nv, &nt, V.data(), T.data()), "mesh read");
     init(nv, V.data(), nt, T.data(), td_order);
   }
+  // .buffgeo geometry file (SWGGeometry::SWGGeometry(const char *GeoFileName),
+  // SWGGeometry.cc:214-383): one object, its MESHFILE, MATERIAL or SVTENSOR
+  // and DISPLACED/ROTATED transformations; the FIBBI cache is keyed by the
+  // mesh file as in the reference
+  explicit SWGGeometry(const char *GeoFile, int td_order = 0)
+  {
+    int nv = 0, nt = 0;
+    check(bfa_buffgeo_load(GeoFile, &nv, &nt, nullptr, nullptr, nullptr), "geometry file");
+    std::vector<double> V(3 * (size_t)nv);
+    std::vector<int> T(4 * (size_t)nt);
+    check(bfa_buffgeo_load(GeoFile, &nv, &nt, V.data(), T.data(), &mat_), "geometry file");
+    owns_mat_ = true;
+    init(nv, V.data(), nt, T.data(), td_order);
+  }
   // synthetic Kuhn-sphere mesh
   SWGGeometry(int n, double radius, double jitter, uint64_t seed, const bfa_material &mat, int td_order = 0)
       : mat_(mat)
@@ -159,6 +173,7 @@ class SWGGeometry {
   ~SWGGeometry()
   {
     if (g_) bfa_geometry_destroy(g_);
+    if (owns_mat_) bfa_material_release(&mat_);
   }
   SWGGeometry(const SWGGeometry &) = delete;
   SWGGeometry &operator=(const SWGGeometry &) = delete;
@@ -280,6 +295,7 @@ class SWGGeometry {
   bfa_material mat_;
   std::string mesh_;
   bool cache_dirty_ = false;
+  bool owns_mat_ = false;
 };
 
 }  // namespace buffamd

@@ -42,14 +42,16 @@ extern "C" {
 
 typedef struct bfa_geometry bfa_geometry;
 
-/* Isotropic / diagonal material eps(x), frequency independent closed forms.
- * Stands in for SVTensor::Evaluate (src/libs/libbuff/SVTensor.cc:306-373)
- * for the material families of the benchmark configurations.
+/* Material eps(omega, x): SVTensor::Evaluate (src/libs/libbuff/SVTensor.cc:
+ * 306-373).  Closed forms (expr unused):
  *   kind 0: eps = p0 + i p1
  *   kind 1: diag(p0+i p1, p2+i p3, p4+i p5)
  *   kind 2: eps(x) = p0 + p2 sin(pi x) cos(pi y) + i p1
- *   kind 3: Janus, z > 0: p0 + i p1, otherwise p2 + i p3                   */
-typedef struct { int kind; double p[6]; } bfa_material;
+ *   kind 3: Janus, z > 0: p0 + i p1, otherwise p2 + i p3
+ * kind 4: a compiled SVTensor / MatProp (bfa_material_svtensor,
+ *   bfa_material_expr): expr is the device program, released by
+ *   bfa_material_release.                                                  */
+typedef struct { int kind; double p[6]; const void *expr; } bfa_material;
 
 /* Plane wave E = E0 exp(i omega dir.x); E0 complex (re,im per component).
  * Stands in for SCUFF-EM's PlaneWave (buff-scatter.cc:186).               */
@@ -95,6 +97,28 @@ int  bfa_static_download(const bfa_geometry *g, double *table /* nnz*6 */);
 /* records supplied by the caller (nnz*6, touching-pair CSR order); marks
  * the static table built without the cold step                             */
 int  bfa_static_upload(bfa_geometry *g, const double *table);
+
+/* ---- materials and geometry files -------------------------------------- */
+/* SVTensor file or MatProp name -> kind 4 material on the current device.
+ * is_matprop != 0: `name` is a MatProp (MATERIAL keyword of .buffgeo):
+ * CONST_EPS_<complex> or a MATERIAL...ENDMATERIAL section of the file in
+ * BUFF_MATPROP_FILE; otherwise an SVTensor file found as given, then on
+ * BUFF_SVTENSOR_PATH (SVTensor.cc:55-127, 162-285).                        */
+int  bfa_material_svtensor(const char *name, int is_matprop, bfa_material *out);
+/* isotropic eps given by one expression in w, x, y, z, r, Theta, Phi      */
+int  bfa_material_expr(const char *eps_expr, bfa_material *out);
+int  bfa_material_release(bfa_material *m);
+/* parse and compile only (host, no device): 0 or the error of the above    */
+int  bfa_material_check(const char *name, int is_matprop);
+int  bfa_material_expr_check(const char *eps_expr);
+/* eps tensors at npts points (host xyz[3*npts]) -> Q[npts][3][3] complex
+ * (re, im interleaved, row-major), evaluated on the device                 */
+int  bfa_material_eval(const bfa_material *m, double omega, int npts, const double *pts, double *Q);
+/* .buffgeo file (SWGGeometry.cc:52-383): one OBJECT with MESHFILE, MATERIAL
+ * or SVTENSOR, DISPLACED / ROTATED; MESHPATH.  Two calls: V == T == NULL
+ * returns the sizes; then V, T (transformed mesh) and, if mat != NULL, the
+ * compiled material.                                                       */
+int  bfa_buffgeo_load(const char *path, int *nv, int *nt, double *V, int *tetVI, bfa_material *mat);
 
 /* ---- FIBBI cache files (FIBBICache.cc:247-463) ------------------------- */
 /* "FIBBI_GCACHE\0" + records {27-float key, 6-double record}; the key of a
