@@ -62,6 +62,7 @@ _SIGS = {
     "bfa_static_build": (I, [P, I, P]),
     "bfa_static_download": (I, [P, P]),
     "bfa_static_upload": (I, [P, P]),
+    "bfa_get_fields": (I, [P, D, P, I, P, I, P, P, P]),
     "bfa_material_svtensor": (I, [ctypes.c_char_p, I, P]),
     "bfa_material_expr": (I, [ctypes.c_char_p, P]),
     "bfa_material_release": (I, [P]),
@@ -341,6 +342,17 @@ class Geometry:
         t = np.zeros((nnz.value, 6))
         _check(lib().bfa_static_download(self.h, t.ctypes.data), "bfa_static_download")
         return t
+
+    def get_fields(self, omega, X, J=None, pws=None, stream=None):
+        """Total fields at points X [n, 3]: (E [n, 3], H [n, 3]) complex; J: N complex128 on the device."""
+        x = np.ascontiguousarray(X, dtype=np.float64).reshape(-1, 3)
+        F = np.zeros((len(x), 6, 2))
+        npw = len(pws) if pws is not None else 0
+        _check(lib().bfa_get_fields(self.h, float(omega), pws if npw else None, npw,
+                                    _ptr(J) if J is not None else None, len(x), x.ctypes.data, F.ctypes.data,
+                                    _stream_ptr(stream)), "bfa_get_fields")
+        Fc = F[..., 0] + 1j * F[..., 1]
+        return Fc[:, :3], Fc[:, 3:]
 
     def static_upload(self, table):
         """Static records (nnz x 6, touching-pair CSR order) from the caller."""

@@ -1475,6 +1475,149 @@ __global__ void k_emt_rows(DevGeom g, double omega, const double *Q, int64_t ldq
   }
 }
 
+// ===========================================================================
+// Fields at evaluation points (SWGGeometry::GetFields, GetFields.cc:288-382):
+// per point X and BF a, Get1BFFields' volume integral (GetFields.cc:181-224,
+// 285-299) with the 4-point rule when |X - centroid| > 10 radius, else 16:
+//   E = i Omega Z0 sum_x w G(X - x).b(x),   H = -i Omega sum_x w C(X - x).b(x),
+// G = (I + grad grad / k^2) e^{ikr}/(4 pi r), C.b = -(1 + i/kr) e^{ikr}/(4 pi r) Rhat x b
+// (the dyadic Green's function pair of CalcGC in vacuum, k = Omega; H = curl
+// of the vector potential, so curl E = i Omega Z0 H), times J_a, summed; plus
+// the incident plane waves, H_inc = nhat x E_inc / Z0.
+// ===========================================================================
+#define FLD_CHUNK 2048
+
+__global__ __launch_bounds__(256) void k_fields(DevGeom g, double omega, const cd *__restrict__ J, const double *X,
+                                               int nchunk, cd *part)
+{
+  const int p = blockIdx.y, tid = threadIdx.x;
+  const int a0 = blockIdx.x * FLD_CHUNK, a1 = min(g.nbf, a0 + FLD_CHUNK);
+  const double x0 = X[3 * p], x1 = X[3 * p + 1], x2 = X[3 * p + 2];
+  const double k = omega;
+  double acc[12];
+  for (int c = 0; c < 12; c++) acc[c] = 0.0;
+  for (int a = a0 + tid; a < a1; a += blockDim.x) {
+    const double *v1 = g.V + 3 * g.bfV[5 * a + 1], *v2 = g.V + 3 * g.bfV[5 * a + 2], *v3 = g.V + 3 * g.bfV[5 * a + 3];
+    const double d0 = x0 - (v1[0] + v2[0] + v3[0]) / 3.0, d1 = x1 - (v1[1] + v2[1] + v3[1]) / 3.0,
+                 d2 = x2 - (v1[2] + v2[2] + v3[2]) / 3.0;
+    const bool farp = sqrt(d0 * d0 + d1 * d1 + d2 * d2) > 10.0 * g.bfRad[a];
+    const int np = farp ? 4 : 16;
+    const double *qt = farp ? g.q4 : g.q16;
+    cd E[3] = {{0, 0}, {0, 0}, {0, 0}}, H[3] = {{0, 0}, {0, 0}, {0, 0}};
+    for (int side = 0; side < 2; side++) {
+      const int t = g.bfT[2 * a + side];
+      const double pf = g.bfPF[2 * a + side];
+      const double *Q = g.bfQ + 6 * a + 3 * side;
+      const double *q = qt + (size_t)t * np * 4;
+      for (int i = 0; i < np; i++) {
+        const double xs0 = q[4 * i], xs1 = q[4 * i + 1], xs2 = q[4 * i + 2], w = q[4 * i + 3];
+        const double b0 = pf * (xs0 - Q[0]), b1 = pf * (xs1 - Q[1]), b2 = pf * (xs2 - Q[2]);
+        const double R0 = x0 - xs0, R1 = x1 - xs1, R2 = x2 - xs2;
+        const double r = sqrt(R0 * R0 + R1 * R1 + R2 * R2), ir = 1.0 / r;
+        const double u0 = R0 * ir, u1 = R1 * ir, u2 = R2 * ir;
+        double sn, cs;
+        sincos(k * r, &sn, &cs);
+        const cd gf = {cs * ir / FOURPI, sn * ir / FOURPI};
+        const double ikr_inv = 1.0 / (k * r);
+        // f1 = 1 + i/(kr) - 1/(kr)^2, f2 = -1 - 3i/(kr) + 3/(kr)^2, f3 = -(1 + i/(kr))
+        const cd f1 = cmul(gf, cd{1.0 - ikr_inv * ikr_inv, ikr_inv});
+        const cd f2 = cmul(gf, cd{-1.0 + 3.0 * ikr_inv * ikr_inv, -3.0 * ikr_inv});
+        const cd f3 = cmul(gf, cd{-1.0, -ikr_inv});
+        const double ub = u0 * b0 + u1 * b1 + u2 * b2;
+        const double ux[3] = {u1 * b2 - u2 * b1, u2 * b0 - u0 * b2, u0 * b1 - u1 * b0};
+        const double bb[3] = {b0, b1, b2}, uu[3] = {u0, u1, u2};
+        for (int m = 0; m < 3; m++) {
+          const cd gb = cadd(cscale(f1, bb[m]), cscale(f2, uu[m] * ub));
+          E[m] = cadd(E[m], cscale(gb, w));
+          H[m] = cadd(H[m], cscale(f3, ux[m] * w));
+        }
+      }
+    }
+    // E *= i Omega Z0, H *= -i Omega, then times J_a
+    const cd ja = J[a];
+    const cd ef = cmul(ja, cd{0.0, omega * ZVAC}), hf = cmul(ja, cd{0.0, -omega});
+    for (int m = 0; m < 3; m++) {
+      const cd e = cmul(E[m], ef), h = cmul(H[m], hf);
+      acc[2 * m] += e.re; acc[2 * m + 1] += e.im;
+      acc[6 + 2 * m] += h.re; acc[6 + 2 * m + 1] += h.im;
+    }
+  }
+  __shared__ double red[12][256];
+  for (int c = 0; c < 12; c++) red[c][tid] = acc[c];
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (tid < off)
+      for (int c = 0; c < 12; c++) red[c][tid] += red[c][tid + off];
+    __syncthreads();
+  }
+  if (tid < 6) part[((size_t)p * nchunk + blockIdx.x) * 6 + tid] = {red[2 * tid][0], red[2 * tid + 1][0]};
+}
+
+__global__ void k_fields_final(int npts, int nchunk, const cd *part, double omega, PWList pws, int npw, const double *X,
+                               cd *F)
+{
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npts) return;
+  cd f[6];
+  for (int c = 0; c < 6; c++) {
+    f[c] = {0.0, 0.0};
+    for (int q = 0; q < nchunk; q++) f[c] = cadd(f[c], part[((size_t)p * nchunk + q) * 6 + c]);
+  }
+  const double *x = X + 3 * p;
+  for (int j = 0; j < npw; j++) {
+    const bfa_planewave &w = pws.pw[j];
+    double sn, cs;
+    sincos(omega * (w.dir[0] * x[0] + w.dir[1] * x[1] + w.dir[2] * x[2]), &sn, &cs);
+    cd e[3];
+    for (int m = 0; m < 3; m++) e[m] = cmul(cd{w.E0[2 * m], w.E0[2 * m + 1]}, cd{cs, sn});
+    const double *n = w.dir;
+    const cd h[3] = {csub(cscale(e[2], n[1]), cscale(e[1], n[2])), csub(cscale(e[0], n[2]), cscale(e[2], n[0])),
+                     csub(cscale(e[1], n[0]), cscale(e[0], n[1]))};
+    for (int m = 0; m < 3; m++) {
+      f[m] = cadd(f[m], e[m]);
+      f[3 + m] = cadd(f[3 + m], cscale(h[m], 1.0 / ZVAC));
+    }
+  }
+  for (int c = 0; c < 6; c++) F[(size_t)p * 6 + c] = f[c];
+}
+
+extern "C" int bfa_get_fields(bfa_geometry *g, double omega, const bfa_planewave *pw, int npw, const void *d_J,
+                              int npts, const double *X, double *F, void *stream)
+{
+  if (!g || !g->uploaded) return fail(BFA_ERR_STATE, "bfa_get_fields: geometry not uploaded");
+  if (npts < 0 || npw < 0 || npw > MAXPW || (npts > 0 && (!X || !F)) || (npw > 0 && !pw))
+    return fail(BFA_ERR_ARG, "bfa_get_fields: bad argument");
+  if (npts == 0) return BFA_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const int nchunk = d_J ? (g->nbf + FLD_CHUNK - 1) / FLD_CHUNK : 0;
+  double *dX = nullptr;
+  cd *dF = nullptr, *part = nullptr;
+  HIPCHECK(hipMalloc(&dX, (size_t)npts * 24));
+  int rc = BFA_OK;
+  if (hipMalloc(&dF, (size_t)npts * 6 * 16) != hipSuccess ||
+      hipMalloc(&part, std::max<size_t>((size_t)npts * nchunk * 6 * 16, 16)) != hipSuccess)
+    rc = fail(BFA_ERR_OOM, "bfa_get_fields: allocation");
+  if (!rc && hipMemcpyAsync(dX, X, (size_t)npts * 24, hipMemcpyHostToDevice, st) != hipSuccess)
+    rc = fail(BFA_ERR_HIP, "bfa_get_fields: upload");
+  if (!rc) {
+    PWList l;
+    memset(&l, 0, sizeof l);
+    for (int j = 0; j < npw; j++) l.pw[j] = pw[j];
+    if (nchunk)
+      hipLaunchKernelGGL(k_fields, dim3(nchunk, npts), dim3(256), 0, st, g->dg, omega, (const cd *)d_J,
+                         (const double *)dX, nchunk, part);
+    hipLaunchKernelGGL(k_fields_final, dim3((npts + 63) / 64), dim3(64), 0, st, npts, nchunk, (const cd *)part, omega,
+                       l, npw, (const double *)dX, dF);
+    if (hipMemcpyAsync(F, dF, (size_t)npts * 6 * 16, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+      rc = fail(BFA_ERR_HIP, "bfa_get_fields: %s", hipGetErrorString(hipGetLastError()));
+  }
+  (void)hipFree(dX);
+  (void)hipFree(dF);
+  (void)hipFree(part);
+  return rc;
+}
+
 extern "C" int bfa_emt_power(bfa_geometry *g, double omega, const bfa_planewave *pw, int npw, const void *d_J,
                              int64_t ldj, double *out, void *stream)
 {

@@ -218,6 +218,28 @@ class SWGGeometry {
     return RHS;
   }
 
+  // SWGGeometry::GetFields (GetFields.cc:288-382): total E and H at the rows
+  // of X (NR x 3, row-major) from the incident chain IF and the currents J
+  // (either may be empty / null); returns NR x 6 (Ex Ey Ez Hx Hy Hz).
+  std::vector<cdouble> GetFields(const std::vector<PlaneWave> &IF, const HVector *J, cdouble Omega,
+                                 const std::vector<double> &X)
+  {
+    if (X.size() % 3) throw Error(BFA_ERR_ARG, "GetFields: X must hold 3 coordinates per point");
+    std::vector<bfa_planewave> pw(IF.size());
+    for (size_t i = 0; i < IF.size(); i++)
+      for (int c = 0; c < 3; c++) {
+        pw[i].E0[2 * c] = IF[i].E0[c].real();
+        pw[i].E0[2 * c + 1] = IF[i].E0[c].imag();
+        pw[i].dir[c] = IF[i].nHat[c];
+      }
+    const int npts = (int)(X.size() / 3);
+    std::vector<cdouble> F((size_t)npts * 6);
+    check(bfa_get_fields(g_, Omega.real(), pw.empty() ? nullptr : pw.data(), (int)pw.size(), J ? J->data() : nullptr,
+                         npts, X.data(), (double *)F.data(), nullptr),
+          "GetFields");
+    return F;
+  }
+
   // GetPFTMatrix with the EMT method: returns {P_abs, P_scat} for the
   // object (columns 0 and 1 of the reference's NumObjects x 8 matrix).
   std::vector<double> GetPFTMatrix(const HVector *J, cdouble Omega, const std::vector<PlaneWave> &IF)

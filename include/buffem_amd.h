@@ -193,6 +193,13 @@ int  bfa_cgemm_sub(int m, int n, int k, const void *d_A, int64_t lda, const void
 /* out[2*j] = P_abs, out[2*j+1] = P_scat for RHS column j (host array).    */
 int  bfa_emt_power(bfa_geometry *g, double omega, const bfa_planewave *pw,
                    int npw, const void *d_J, int64_t ldj, double *out, void *stream);
+/* SWGGeometry::GetFields (GetFields.cc:288-382): total E and H at npts
+ * points X[3*npts] (host) from the incident plane waves (summed, npw may be
+ * 0) and the currents d_J (N complex128 on the device, or NULL for the
+ * incident field alone); F[npts][6] complex (Ex Ey Ez Hx Hy Hz, re/im
+ * interleaved) on the host.                                                */
+int  bfa_get_fields(bfa_geometry *g, double omega, const bfa_planewave *pw, int npw,
+                    const void *d_J, int npts, const double *X, double *F, void *stream);
 
 /* ---- batching form (no reference counterpart: the reference solves one
  * problem per process, one frequency at a time, buff-scatter.cc:236-363) --

@@ -270,3 +270,64 @@ def lu_solve(M, B):
     import scipy.linalg as sl
     lu, piv = sl.lu_factor(M, check_finite=False)
     return sl.lu_solve((lu, piv), B, check_finite=False)
+
+
+ZVAC = 376.73031346177
+
+
+def fields(geom, V, T, J, omega, X, pws=()):
+    """SWGGeometry::GetFields restated in numpy (GetFields.cc:181-224, 285-382).
+
+    Per point X and BF a: the volume integral of Get1BFFields_VI over the
+    BF's two tetrahedra with TetCR4 when |X - face centroid| > 10 x the
+    face radius (InitFaceList.cc:194-200: max distance from the centroid to
+    QP, V1, V2, V3, QM), else TetCR16 (BFInt, Cubature.cc:234-258);
+    E = i w Z0 G.b, H = -i w C.b with the vacuum dyadic pair of CalcGC
+    (restated from the Helmholtz Green's function: G = (I + grad grad/k^2) g,
+    C.b = -(1 + i/kr) g Rhat x b, g = e^{ikr}/4 pi r, k = w), times J_a; plus
+    the incident plane waves (E0 e^{ik n.x}, H = n x E / Z0).
+    """
+    F = geom.faces()
+    Vt = np.asarray(V, dtype=np.float64)
+    Tt = np.asarray(T)
+    rules = {4: tetcr(4), 16: tetcr(16)}
+    N = len(J)
+    k = float(omega)
+    out = np.zeros((len(X), 6), dtype=complex)
+    for p, x in enumerate(np.asarray(X, dtype=np.float64)):
+        Et = np.zeros(3, complex)
+        Ht = np.zeros(3, complex)
+        for a in range(N):
+            fv = Vt[F["V"][a]]
+            cen = fv.mean(0)
+            rad = max(np.linalg.norm(Vt[i] - cen) for i in (F["QP"][a], *F["V"][a], F["QM"][a]))
+            npts = 4 if np.linalg.norm(x - cen) > 10.0 * rad else 16
+            r4 = rules[npts]
+            E = np.zeros(3, complex)
+            H = np.zeros(3, complex)
+            for tet, qv, sign in ((F["PT"][a], F["QP"][a], 1.0), (F["MT"][a], F["QM"][a], -1.0)):
+                P0 = Vt[Tt[tet][0]]
+                L = Vt[Tt[tet][1:]] - P0
+                vol = np.linalg.det(L) / 6.0
+                pf = sign * F["Area"][a] / (3.0 * vol)
+                for u1, u2, u3, w in r4:
+                    xs = P0 + u1 * L[0] + u2 * L[1] + u3 * L[2]
+                    b = pf * (xs - Vt[qv])
+                    R = x - xs
+                    r = np.linalg.norm(R)
+                    u = R / r
+                    g = np.exp(1j * k * r) / (4 * np.pi * r)
+                    kr = k * r
+                    G = g * ((1 + 1j / kr - 1 / kr ** 2) * np.eye(3) + (-1 - 3j / kr + 3 / kr ** 2) * np.outer(u, u))
+                    E += 6 * vol * w * (G @ b)
+                    H += 6 * vol * w * (-(1 + 1j / kr) * g * np.cross(u, b))
+            Et += J[a] * 1j * k * ZVAC * E
+            Ht += J[a] * (-1j * k) * H
+        for e0, d in pws:
+            e0 = np.asarray(e0, complex)
+            d = np.asarray(d, float)
+            Ei = e0 * np.exp(1j * k * d @ x)
+            Et += Ei
+            Ht += np.cross(d, Ei) / ZVAC
+        out[p, :3], out[p, 3:] = Et, Ht
+    return out
