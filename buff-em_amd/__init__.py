@@ -63,6 +63,8 @@ _SIGS = {
     "bfa_static_download": (I, [P, P]),
     "bfa_static_upload": (I, [P, P]),
     "bfa_get_fields": (I, [P, D, P, I, P, I, P, P, P]),
+    "bfa_assemble_g": (I, [P, D, D, P, I64, P]),
+    "bfa_overlap_blocks": (I, [P, D, P, P, D, P, P, P, P, P]),
     "bfa_material_svtensor": (I, [ctypes.c_char_p, I, P]),
     "bfa_material_expr": (I, [ctypes.c_char_p, P]),
     "bfa_material_release": (I, [P]),
@@ -353,6 +355,26 @@ class Geometry:
                                     _stream_ptr(stream)), "bfa_get_fields")
         Fc = F[..., 0] + 1j * F[..., 1]
         return Fc[:, :3], Fc[:, 3:]
+
+    def assemble_g(self, omega, d_M, ld=None, stream=None):
+        """AssembleGBlock (self block): the VIE matrix without the chi^-1 overlap terms."""
+        omega = complex(omega)
+        _check(lib().bfa_assemble_g(self.h, omega.real, omega.imag, _ptr(d_M), int(ld or self.N), _stream_ptr(stream)),
+               "bfa_assemble_g")
+
+    def overlap_blocks(self, omega, mat, temperature=None, t_env=0.0):
+        """AssembleOverlapBlocks: per row <= 7 entries -> (cnt [N], cols [N, 7], V, VInv [N, 7] complex, Rytov [N, 7])."""
+        N = self.N
+        cnt = np.zeros(N, dtype=np.int32)
+        cols = np.zeros((N, 7), dtype=np.int32)
+        V = np.zeros((N, 7, 2))
+        VI = np.zeros((N, 7, 2))
+        R = np.zeros((N, 7))
+        _check(lib().bfa_overlap_blocks(self.h, float(omega), ctypes.byref(mat),
+                                        ctypes.byref(temperature) if temperature is not None else None, float(t_env),
+                                        cnt.ctypes.data, cols.ctypes.data, V.ctypes.data, VI.ctypes.data,
+                                        R.ctypes.data), "bfa_overlap_blocks")
+        return cnt, cols, V[..., 0] + 1j * V[..., 1], VI[..., 0] + 1j * VI[..., 1], R
 
     def static_upload(self, table):
         """Static records (nnz x 6, touching-pair CSR order) from the caller."""

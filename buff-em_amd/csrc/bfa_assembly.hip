@@ -1104,6 +1104,150 @@ __device__ cd overlap_tet(const DevGeom &g, int t, const double *QA, double pfA,
   return cscale(acc, -1.0 / (omega * omega));
 }
 
+// ---------------------------------------------------------------------------
+// AssembleOverlapBlocks / GetOverlaps (VIEMatrix.cc:122-360): per row a the
+// self entry (both tetrahedra) then the other interior faces of a's positive
+// and negative tetrahedra in FI order (<= 7 entries), each the TetCR33
+// integral of F_a . M . F_b with
+//   V:     -Omega^2 (eps - 1)        VInv: -(eps - 1)^-1 / Omega^2
+//   Rytov: 4 Omega RelDeltaTheta / (pi Z0) Im(eps - 1)
+// RelDeltaTheta = Theta(Omega, T(x)) - Theta_env, divided by
+// DeltaThetaHat = Theta(T_avg) - Theta_env when that is nonzero;
+// Theta(Omega, T) = Omega / (exp(Omega / (k T)) - 1), k = 4.36763e-4,
+// Theta = 0 at T = 0 (GetThetaFactor, VIEMatrix.cc:78-84).
+#define BOLTZMANN_K 4.36763e-4
+#define MAXOVERLAP 7
+__device__ double theta_factor(double omega, double T)
+{
+  if (T == 0.0) return 0.0;
+  return omega / (exp(omega / (BOLTZMANN_K * T)) - 1.0);
+}
+
+__device__ void overlap_entry(const DevGeom &g, int t, const double *QA, double pfA, const double *QB, double pfB,
+                              double omega, const bfa_material &mat, const bfa_material &temp, bool has_temp,
+                              double theta_env, double dtheta_hat, cd &V, cd &VI, double &R)
+{
+  const double *q = g.q33 + (size_t)t * 33 * 4;
+  cd v = {0.0, 0.0}, vi = {0.0, 0.0};
+  double ry = 0.0;
+  for (int i = 0; i < 33; i++) {
+    const double x[3] = {q[4 * i], q[4 * i + 1], q[4 * i + 2]};
+    cd Q[3][3], W[3][3];
+    eps_tensor(mat, x, omega, Q);
+    inv_chi(Q, W);
+    double rel = 1.0;
+    if (has_temp) {
+      cd Tq[3][3];
+      eps_tensor(temp, x, 0.0, Tq);
+      rel = theta_factor(omega, Tq[0][0].re) - theta_env;
+      if (dtheta_hat != 0.0) rel /= dtheta_hat;
+    }
+    double FA[3], FB[3];
+    for (int d = 0; d < 3; d++) { FA[d] = pfA * (x[d] - QA[d]); FB[d] = pfB * (x[d] - QB[d]); }
+    cd sv = {0.0, 0.0}, sw = {0.0, 0.0};
+    double sr = 0.0;
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) {
+        const double f = FA[r] * FB[c];
+        const cd em1 = r == c ? cd{Q[r][c].re - 1.0, Q[r][c].im} : Q[r][c];
+        sv = cadd(sv, cscale(em1, f));
+        sw = cadd(sw, cscale(W[r][c], f));
+        sr += f * em1.im;
+      }
+    const double w = q[4 * i + 3];
+    v = cadd(v, cscale(sv, w));
+    vi = cadd(vi, cscale(sw, w));
+    ry += w * sr * rel;
+  }
+  V = cscale(v, -omega * omega);
+  VI = cscale(vi, -1.0 / (omega * omega));
+  R = ry * 4.0 * omega / (M_PI * ZVAC);
+}
+
+__global__ void k_overlap_rows(DevGeom g, double omega, bfa_material mat, bfa_material temp, int has_temp,
+                               double theta_env, double dtheta_hat, int *cnt, int *cols, cd *V, cd *VI, double *R)
+{
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= g.nbf) return;
+  int n = 1;
+  cd v0 = {0.0, 0.0}, vi0 = {0.0, 0.0};
+  double r0 = 0.0;
+  const size_t o = (size_t)a * MAXOVERLAP;
+  for (int side = 0; side < 2; side++) {
+    const int t = g.bfT[2 * a + side];
+    const double *QA = g.bfQ + 6 * a + 3 * side;
+    const double pfA = g.bfPF[2 * a + side];
+    for (int f = 0; f < 4; f++) {
+      const int b = g.tetFI[4 * t + f];
+      if (b >= g.nbf) continue;
+      const int sb = g.bfT[2 * b] == t ? 0 : 1;
+      cd v, vi;
+      double r;
+      overlap_entry(g, t, QA, pfA, g.bfQ + 6 * b + 3 * sb, g.bfPF[2 * b + sb], omega, mat, temp, has_temp != 0,
+                    theta_env, dtheta_hat, v, vi, r);
+      if (b == a) {
+        v0 = cadd(v0, v); vi0 = cadd(vi0, vi); r0 += r;
+      } else {
+        cols[o + n] = b; V[o + n] = v; VI[o + n] = vi; R[o + n] = r;
+        n++;
+      }
+    }
+  }
+  cols[o] = a; V[o] = v0; VI[o] = vi0; R[o] = r0;
+  cnt[a] = n;
+}
+
+// AssembleOverlapBlocks for one object: row a holds cnt[a] <= 7 entries at
+// [a*7, a*7 + cnt[a]) of cols / V / VInv / Rytov (host arrays)
+extern "C" int bfa_overlap_blocks(bfa_geometry *g, double omega, const bfa_material *eps,
+                                  const bfa_material *temperature, double t_env, int *cnt, int *cols, double *V,
+                                  double *VInv, double *Rytov)
+{
+  if (!g || !g->uploaded || !eps || !cnt || !cols || !V || !VInv || !Rytov)
+    return fail(BFA_ERR_ARG, "bfa_overlap_blocks: bad argument");
+  if (omega == 0.0) return fail(BFA_ERR_ARG, "bfa_overlap_blocks: omega = 0");
+  const int N = g->nbf, nt = g->nt;
+  bfa_material temp;
+  memset(&temp, 0, sizeof temp);
+  double theta_env = 0.0, dhat = 0.0;
+  if (temperature) {
+    temp = *temperature;
+    // T_avg: volume-weighted temperature at the tetrahedron centroids
+    // (GetAverageTemperature, VIEMatrix.cc:86-101)
+    std::vector<double> cen(3 * (size_t)nt), Tq(18 * (size_t)nt);
+    for (int t = 0; t < nt; t++)
+      for (int d = 0; d < 3; d++) cen[3 * t + d] = g->tetCen[3 * t + d];
+    int rc = bfa_material_eval(temperature, 0.0, nt, cen.data(), Tq.data());
+    if (rc) return rc;
+    double tv = 0.0, vol = 0.0;
+    for (int t = 0; t < nt; t++) { vol += g->tetVol[t]; tv += g->tetVol[t] * Tq[18 * (size_t)t]; }
+    const double TAvg = tv / vol;
+    auto theta = [](double w, double T) { return T == 0.0 ? 0.0 : w / (std::exp(w / (BOLTZMANN_K * T)) - 1.0); };
+    theta_env = theta(omega, t_env);
+    dhat = theta(omega, TAvg) - theta_env;
+  }
+  int *dc = nullptr, *dcols = nullptr;
+  cd *dV = nullptr, *dVI = nullptr;
+  double *dR = nullptr;
+  const size_t m = (size_t)N * MAXOVERLAP;
+  int rc = BFA_OK;
+  if (hipMalloc(&dc, (size_t)N * 4) != hipSuccess || hipMalloc(&dcols, m * 4) != hipSuccess ||
+      hipMalloc(&dV, m * 16) != hipSuccess || hipMalloc(&dVI, m * 16) != hipSuccess || hipMalloc(&dR, m * 8) != hipSuccess)
+    rc = fail(BFA_ERR_OOM, "bfa_overlap_blocks: allocation");
+  if (!rc) {
+    hipLaunchKernelGGL(k_overlap_rows, dim3((N + 127) / 128), dim3(128), 0, 0, g->dg, omega, *eps, temp,
+                       temperature ? 1 : 0, theta_env, dhat, dc, dcols, dV, dVI, dR);
+    if (hipMemcpy(cnt, dc, (size_t)N * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(cols, dcols, m * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(V, dV, m * 16, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(VInv, dVI, m * 16, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(Rytov, dR, m * 8, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = fail(BFA_ERR_HIP, "bfa_overlap_blocks: %s", hipGetErrorString(hipGetLastError()));
+  }
+  (void)hipFree(dc); (void)hipFree(dcols); (void)hipFree(dV); (void)hipFree(dVI); (void)hipFree(dR);
+  return rc;
+}
+
 __device__ __forceinline__ void near_entry(const DevGeom &g, double kre, double kim, const double *table,
                                            const bfa_material &mat, cd *M, int64_t ld, int64_t j)
 {
@@ -1131,9 +1275,10 @@ __device__ __forceinline__ void near_entry(const DevGeom &g, double kre, double 
   const cd t1 = {G[2] - cdiv({G[3], 0.0}, k2).re, -cdiv({G[3], 0.0}, k2).im};
   const cd t2 = {G[4] - cdiv({G[5], 0.0}, k2).re, -cdiv({G[5], 0.0}, k2).im};
   acc = cadd(acc, cadd(t0, cadd(cmul(ik, t1), cmul(ik2, t2))));
-  // Gram / chi^-1 block on shared tetrahedra (GetOverlaps)
+  // Gram / chi^-1 block on shared tetrahedra (GetOverlaps); a material of
+  // kind < 0 asks for the G block alone (AssembleGBlock)
   const double omega = kre;
-  for (int sa = 0; sa < 2; sa++)
+  for (int sa = 0; sa < 2 && mat.kind >= 0; sa++)
     for (int sb = 0; sb < 2; sb++) {
       const int ta = g.bfT[2 * a + sa], tb = g.bfT[2 * b + sb];
       if (ta != tb) continue;
@@ -1191,6 +1336,16 @@ extern "C" int bfa_assemble_vie(bfa_geometry *g, double omega_re, double omega_i
                                 void *d_M, int64_t ld, void *stream)
 {
   return assemble_vie(g, omega_re, omega_im, mat, d_M, ld, false, stream);
+}
+
+// SWGGeometry::AssembleGBlock(no, no, ...) (VIEMatrix.cc:365-426): the G
+// block without the chi^-1 overlap terms
+extern "C" int bfa_assemble_g(bfa_geometry *g, double omega_re, double omega_im, void *d_M, int64_t ld, void *stream)
+{
+  bfa_material none;
+  memset(&none, 0, sizeof none);
+  none.kind = -1;
+  return assemble_vie(g, omega_re, omega_im, &none, d_M, ld, false, stream);
 }
 
 extern "C" int bfa_assemble_vie_far32(bfa_geometry *g, double omega_re, double omega_im, const bfa_material *mat,

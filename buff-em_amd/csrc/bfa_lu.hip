@@ -40,6 +40,8 @@ template <typename T> __device__ __forceinline__ cplx<T> c_mul(cplx<T> a, cplx<T
 { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
 template <typename T> __device__ __forceinline__ cplx<T> c_sub(cplx<T> a, cplx<T> b)
 { return {a.re - b.re, a.im - b.im}; }
+template <typename T> __device__ __forceinline__ cplx<T> c_add(cplx<T> a, cplx<T> b)
+{ return {a.re + b.re, a.im + b.im}; }
 template <typename T> __device__ __forceinline__ cplx<T> c_inv(cplx<T> a)
 { T d = a.re * a.re + a.im * a.im; return {a.re / d, -a.im / d}; }
 template <typename T> __device__ __forceinline__ T c_abs1(cplx<T> a) { return fabs(a.re) + fabs(a.im); }
@@ -82,21 +84,15 @@ __device__ __forceinline__ void gemm64_body(int m, int n, int k, const cplx<T> *
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   const int l16 = lane & 15, lq = lane >> 4;
+  // the accumulators collect -A*B from zero and C is added once at the end:
+  // one rounding into C per update, not one per k step at C's magnitude
   acc_t cr[2][2], ci[2][2];
-  // load C into accumulators
 #pragma unroll
   for (int rb = 0; rb < 2; rb++)
 #pragma unroll
-    for (int cb = 0; cb < 2; cb++) {
-      const int col = col0 + wc * 32 + cb * 16 + l16;
+    for (int cb = 0; cb < 2; cb++)
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int row = row0 + wr * 32 + rb * 16 + mf<T>::row(j, lq);
-        cplx<T> v = {0, 0};
-        if (row < m && col < n) v = C[(size_t)col * ldc + row];
-        cr[rb][cb][j] = v.re; ci[rb][cb][j] = v.im;
-      }
-    }
+      for (int j = 0; j < 4; j++) cr[rb][cb][j] = ci[rb][cb][j] = 0;
   for (int k0 = 0; k0 < k; k0 += GBK) {
     // A tile: 64 rows x 16 k; thread -> row tid%64, k = tid/64 + 4i
 #pragma unroll
@@ -155,7 +151,11 @@ __device__ __forceinline__ void gemm64_body(int m, int n, int k, const cplx<T> *
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const int row = row0 + wr * 32 + rb * 16 + mf<T>::row(j, lq);
-        if (row < m && col < n) C[(size_t)col * ldc + row] = {cr[rb][cb][j], ci[rb][cb][j]};
+        if (row < m && col < n) {
+          cplx<T> *c = &C[(size_t)col * ldc + row];
+          const cplx<T> v = *c;
+          *c = {v.re + cr[rb][cb][j], v.im + ci[rb][cb][j]};
+        }
       }
     }
 }
@@ -193,20 +193,14 @@ __device__ __forceinline__ void gemm128_body(int m, int n, int k, const cplx<T> 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 2, wc = wave & 3;
   const int l16 = lane & 15, lq = lane >> 4;
+  // -A*B accumulated from zero, C added once in the epilogue
   acc_t cr[4][2], ci[4][2];
 #pragma unroll
   for (int rb = 0; rb < 4; rb++)
 #pragma unroll
-    for (int cb = 0; cb < 2; cb++) {
-      const int col = col0 + wc * 32 + cb * 16 + l16;
+    for (int cb = 0; cb < 2; cb++)
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int row = row0 + wr * 64 + rb * 16 + mf<T>::row(j, lq);
-        cplx<T> v = {0, 0};
-        if (row < m && col < n) v = C[(size_t)col * ldc + row];
-        cr[rb][cb][j] = v.re; ci[rb][cb][j] = v.im;
-      }
-    }
+      for (int j = 0; j < 4; j++) cr[rb][cb][j] = ci[rb][cb][j] = 0;
   cplx<T> pa[4], pb[4];
   auto gload = [&](int k0) {
 #pragma unroll
@@ -274,7 +268,11 @@ __device__ __forceinline__ void gemm128_body(int m, int n, int k, const cplx<T> 
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const int row = row0 + wr * 64 + rb * 16 + mf<T>::row(j, lq);
-        if (row < m && col < n) C[(size_t)col * ldc + row] = {cr[rb][cb][j], ci[rb][cb][j]};
+        if (row < m && col < n) {
+          cplx<T> *c = &C[(size_t)col * ldc + row];
+          const cplx<T> v = *c;
+          *c = {v.re + cr[rb][cb][j], v.im + ci[rb][cb][j]};
+        }
       }
     }
 }
@@ -467,14 +465,14 @@ __device__ __forceinline__ void trsm_lf_body(int kb, int n, const cplx<T> *__res
     for (int r = b0 + bk + tid; r < kb; r += blockDim.x) {
       cplx<T> acc[SC];
 #pragma unroll
-      for (int c = 0; c < SC; c++) acc[c] = X[c][r];
+      for (int c = 0; c < SC; c++) acc[c] = cplx<T>{0, 0};
       for (int k = 0; k < bk; k++) {
         const cplx<T> l = L[(size_t)(b0 + k) * ldl + r];
 #pragma unroll
-        for (int c = 0; c < SC; c++) acc[c] = c_sub(acc[c], c_mul(l, X[c][b0 + k]));
+        for (int c = 0; c < SC; c++) acc[c] = c_add(acc[c], c_mul(l, X[c][b0 + k]));
       }
 #pragma unroll
-      for (int c = 0; c < SC; c++) X[c][r] = acc[c];
+      for (int c = 0; c < SC; c++) X[c][r] = c_sub(X[c][r], acc[c]);
     }
     __syncthreads();
   }
@@ -490,6 +488,62 @@ __global__ __launch_bounds__(256) void k_trsm_lf(int kb, int n, const cplx<T> *_
 {
   __shared__ cplx<T> X[TfShape<T>::SC][TF_MAXK];
   trsm_lf_body<T>(kb, n, L, ldl, B, ldb, blockIdx.x * TfShape<T>::SC, X);
+}
+
+// B(kb x n) := U^-1 B, U upper with its diagonal, kb <= TF_MAXK: the mirror
+// of trsm_lf_body (32-row blocks from the bottom; in the diagonal block lane
+// k divides by U(k,k) when its value is final, then passes it on)
+template <typename T>
+__global__ __launch_bounds__(256) void k_trsm_uf(int kb, int n, const cplx<T> *__restrict__ U, int64_t ldu, cplx<T> *B,
+                                                 int64_t ldb)
+{
+  constexpr int SC = TfShape<T>::SC;
+  __shared__ cplx<T> X[SC][TF_MAXK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int c0 = blockIdx.x * SC, nc = min(SC, n - c0);
+  for (int idx = tid; idx < kb * SC; idx += blockDim.x) {
+    const int r = idx % kb, c = idx / kb;
+    X[c][r] = c < nc ? B[(size_t)(c0 + c) * ldb + r] : cplx<T>{0, 0};
+  }
+  __syncthreads();
+  const int row = lane & 31, half = lane >> 5;
+  for (int b0 = ((kb - 1) / 32) * 32; b0 >= 0; b0 -= 32) {
+    const int bk = min(32, kb - b0);
+    cplx<T> lrow[32];
+#pragma unroll
+    for (int k = 0; k < 32; k++)
+      lrow[k] = (k > row && k < bk) ? U[(size_t)(b0 + k) * ldu + b0 + row] : cplx<T>{0, 0};
+    const cplx<T> dinv = row < bk ? c_inv(U[(size_t)(b0 + row) * ldu + b0 + row]) : cplx<T>{0, 0};
+    for (int pr = wave; pr < SC / 2; pr += 4) {
+      const int c = 2 * pr + half;
+      cplx<T> x = row < bk ? X[c][b0 + row] : cplx<T>{0, 0};
+#pragma unroll
+      for (int k = 31; k >= 0; k--) {
+        if (row == k) x = c_mul(x, dinv);
+        const cplx<T> xk = {__shfl(x.re, k + 32 * half, 64), __shfl(x.im, k + 32 * half, 64)};
+        if (row < k) x = c_sub(x, c_mul(lrow[k], xk));
+      }
+      if (row < bk) X[c][b0 + row] = x;
+    }
+    __syncthreads();
+    for (int r = tid; r < b0; r += blockDim.x) {
+      cplx<T> acc[SC];
+#pragma unroll
+      for (int c = 0; c < SC; c++) acc[c] = cplx<T>{0, 0};
+      for (int k = 0; k < bk; k++) {
+        const cplx<T> u = U[(size_t)(b0 + k) * ldu + r];
+#pragma unroll
+        for (int c = 0; c < SC; c++) acc[c] = c_add(acc[c], c_mul(u, X[c][b0 + k]));
+      }
+#pragma unroll
+      for (int c = 0; c < SC; c++) X[c][r] = c_sub(X[c][r], acc[c]);
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < kb * SC; idx += blockDim.x) {
+    const int r = idx % kb, c = idx / kb;
+    if (c < nc) B[(size_t)(c0 + c) * ldb + r] = X[c][r];
+  }
 }
 
 // B(kb x n) := L^-1 B, L unit lower: one launch up to TF_MAXK rows
@@ -1013,9 +1067,11 @@ __global__ void k_gather_rows(const cplx<T> *B, int64_t ldb, cplx<T> *X, int64_t
   if (i < n && j < nrhs) X[(size_t)j * ldx + i] = B[(size_t)j * ldb + perm[i]];
 }
 
-// triangular solves: SV_KB-row blocks, up to SV_RHS right-hand sides per sweep
+// triangular solves: SV_KB-row blocks, up to SV_RHS right-hand sides per
+// sweep; from SV_BLOCKED_RHS right-hand sides on, the blocked GEMM form
 #define SV_KB 64
 #define SV_RHS 8
+#define SV_BLOCKED_RHS 64
 
 // ---------------------------------------------------------------------------
 // Streaming block substitution.  The diagonal SV_KB x SV_KB blocks of L
@@ -1303,6 +1359,34 @@ int lu_solve_prep(const cplx<T> *A, int n, int64_t lda, const int *ipiv, hipStre
   return BFA_OK;
 }
 
+// Many right-hand sides (LUSolve(HMatrix*), GetFlux.cc:258-265): blocked by
+// TF_MAXK rows, one-launch TRSMs on the diagonal blocks and the MFMA GEMM for
+// the updates, so the work runs at GEMM speed (8 n^2 nrhs flop per sweep)
+template <typename T>
+static int lu_solve_blocked(const cplx<T> *A, int n, int64_t lda, cplx<T> *X, int nrhs, hipStream_t st)
+{
+  constexpr int SC = TfShape<T>::SC;
+  for (int k = 0; k < n; k += TF_MAXK) {
+    const int kb = std::min(TF_MAXK, n - k);
+    trsm_lower<T>(kb, nrhs, A + (size_t)k * lda + k, lda, X + k, n, st);
+    if (k + kb < n) {
+      int rc = gemm_sub<T>(n - k - kb, nrhs, kb, A + (size_t)k * lda + k + kb, lda, X + k, n, X + k + kb, n, st);
+      if (rc) return rc;
+    }
+  }
+  for (int k = ((n - 1) / TF_MAXK) * TF_MAXK; k >= 0; k -= TF_MAXK) {
+    const int kb = std::min(TF_MAXK, n - k);
+    hipLaunchKernelGGL(k_trsm_uf<T>, dim3((nrhs + SC - 1) / SC), dim3(256), 0, st, kb, nrhs,
+                       A + (size_t)k * lda + k, lda, X + k, (int64_t)n);
+    if (k > 0) {
+      int rc = gemm_sub<T>(k, nrhs, kb, A + (size_t)k * lda, lda, X + k, n, X, n, st);
+      if (rc) return rc;
+    }
+  }
+  HIPCHECK(hipGetLastError());
+  return BFA_OK;
+}
+
 // B <- (LU)^-1 P B with the data lu_solve_prep left in scratch
 template <typename T>
 int lu_solve_apply(const cplx<T> *A, int n, int64_t lda, cplx<T> *B, int nrhs, int64_t ldb, hipStream_t st,
@@ -1313,13 +1397,20 @@ int lu_solve_apply(const cplx<T> *A, int n, int64_t lda, cplx<T> *B, int nrhs, i
   const cplx<T> *inv = (const cplx<T> *)((char *)scratch + sv_inv_off(n));
   const size_t vec = sv_align((size_t)n * nrhs * sizeof(cplx<T>));
   cplx<T> *Pb = (cplx<T> *)((char *)scratch + sv_x_off(n, sizeof(cplx<T>)));
+  hipLaunchKernelGGL(k_gather_rows<T>, dim3((n + 255) / 256, nrhs), dim3(256), 0, st, B, ldb, Pb, (int64_t)n, n, nrhs,
+                     perm);
+  if (nrhs >= SV_BLOCKED_RHS) {
+    int rc = lu_solve_blocked<T>(A, n, lda, Pb, nrhs, st);
+    if (rc) return rc;
+    HIPCHECK(hipMemcpy2DAsync(B, ldb * sizeof(cplx<T>), Pb, n * sizeof(cplx<T>), n * sizeof(cplx<T>), nrhs,
+                              hipMemcpyDeviceToDevice, st));
+    return BFA_OK;
+  }
   cplx<T> *Y = (cplx<T> *)((char *)Pb + vec);
   cplx<T> *X = (cplx<T> *)((char *)Y + vec);
   unsigned *tickets = (unsigned *)((char *)X + vec);
   const int nt = sv_nticket(nrhs);
   const size_t count = (size_t)n * nrhs;
-  hipLaunchKernelGGL(k_gather_rows<T>, dim3((n + 255) / 256, nrhs), dim3(256), 0, st, B, ldb, Pb, (int64_t)n, n, nrhs,
-                     perm);
   hipLaunchKernelGGL(k_sv_reset<T>, dim3((unsigned)((std::max(count, (size_t)nt) + 255) / 256)), dim3(256), 0, st, Y,
                      X, count, tickets, nt);
   const int nblk = (n + SV_KB - 1) / SV_KB;
@@ -1348,6 +1439,8 @@ int lu_solve(const cplx<T> *A, int n, int64_t lda, const int *ipiv, cplx<T> *B, 
 
 size_t lu_solve_scratch_bytes(int n, int nrhs, size_t elem)
 {
+  // the blocked form needs the permuted right-hand sides only
+  if (nrhs >= SV_BLOCKED_RHS) return sv_x_off(n, elem) + sv_align((size_t)n * nrhs * elem);
   return sv_x_off(n, elem) + 3 * sv_align((size_t)n * nrhs * elem) + sv_align((size_t)sv_nticket(nrhs) * 4);
 }
 

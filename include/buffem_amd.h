@@ -148,6 +148,19 @@ int  bfa_assemble_vie(bfa_geometry *g, double omega_re, double omega_im,
 /* Same, with the far-field (non-touching) cubature evaluated and summed in
  * FP32 (the precision form's far field); touching pairs, static records and
  * the Gram block stay FP64.  Entries are stored as complex128.             */
+/* SWGGeometry::AssembleGBlock(no, no, ...) (VIEMatrix.cc:365-426): the
+ * self G block, i.e. the VIE matrix without the chi^-1 overlap terms        */
+int  bfa_assemble_g(bfa_geometry *g, double omega_re, double omega_im, void *d_M, int64_t ld,
+                    void *stream);
+/* SWGGeometry::AssembleOverlapBlocks (VIEMatrix.cc:321-360, GetOverlaps
+ * :200-316): row a has cnt[a] <= 7 entries at [7a, 7a + cnt[a]) of cols,
+ * V, VInv (complex, re/im) and Rytov (real); temperature (NULL: none) is a
+ * compiled material whose eps_xx real part is T(x), t_env the environment
+ * temperature (buff-neq units: Theta = omega / (exp(omega / (k T)) - 1),
+ * k = 4.36763e-4).  Host outputs.                                          */
+int  bfa_overlap_blocks(bfa_geometry *g, double omega, const bfa_material *eps,
+                        const bfa_material *temperature, double t_env, int *cnt, int *cols,
+                        double *V, double *VInv, double *Rytov);
 int  bfa_assemble_vie_far32(bfa_geometry *g, double omega_re, double omega_im,
                             const bfa_material *mat, void *d_M, int64_t ld, void *stream);
 int  bfa_assemble_rhs(bfa_geometry *g, double omega, const bfa_planewave *pw,
