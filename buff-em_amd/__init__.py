@@ -65,6 +65,10 @@ _SIGS = {
     "bfa_get_fields": (I, [P, D, P, I, P, I, P, P, P]),
     "bfa_assemble_g": (I, [P, D, D, P, I64, P]),
     "bfa_overlap_blocks": (I, [P, D, P, P, D, P, P, P, P, P]),
+    "bfa_gram_inverse": (I, [P, P, I64, P]),
+    "bfa_dressed_rytov_work_bytes": (ctypes.c_size_t, [I, I64]),
+    "bfa_dressed_rytov": (I, [P, D, P, P, D, P, P, I64, P, P]),
+    "bfa_emt_power_dr": (I, [P, D, P, I64, P, P]),
     "bfa_material_svtensor": (I, [ctypes.c_char_p, I, P]),
     "bfa_material_expr": (I, [ctypes.c_char_p, P]),
     "bfa_material_release": (I, [P]),
@@ -375,6 +379,27 @@ class Geometry:
                                         cnt.ctypes.data, cols.ctypes.data, V.ctypes.data, VI.ctypes.data,
                                         R.ctypes.data), "bfa_overlap_blocks")
         return cnt, cols, V[..., 0] + 1j * V[..., 1], VI[..., 0] + 1j * VI[..., 1], R
+
+    def gram_inverse(self, d_Sinv, ld=None, stream=None):
+        """S^-1 of the SWG overlap (Gram) matrix into d_Sinv (N x N complex128), as buff-neq's CreateBNEQData."""
+        _check(lib().bfa_gram_inverse(self.h, _ptr(d_Sinv), int(ld or self.N), _stream_ptr(stream)), "bfa_gram_inverse")
+
+    def dressed_rytov(self, omega, mat, d_Sinv, d_DR, d_work, temperature=None, t_env=0.0, ld=None, stream=None):
+        """ComputeDressedRytovMatrix: d_DR <- W (S^-1 R S^-1)^H; d_work of dressed_rytov_work_bytes() bytes."""
+        _check(lib().bfa_dressed_rytov(self.h, float(omega), ctypes.byref(mat),
+                                       ctypes.byref(temperature) if temperature is not None else None, float(t_env),
+                                       _ptr(d_Sinv), _ptr(d_DR), int(ld or self.N), _ptr(d_work),
+                                       _stream_ptr(stream)), "bfa_dressed_rytov")
+
+    def dressed_rytov_work_bytes(self, ld=None):
+        return int(lib().bfa_dressed_rytov_work_bytes(self.N, int(ld or self.N)))
+
+    def emt_power_dr(self, omega, d_DR, ld=None, stream=None):
+        """EMT power of a dressed Rytov matrix (no incident field): (P_abs, P_scat)."""
+        out = np.zeros(2)
+        _check(lib().bfa_emt_power_dr(self.h, float(omega), _ptr(d_DR), int(ld or self.N), out.ctypes.data,
+                                      _stream_ptr(stream)), "bfa_emt_power_dr")
+        return out
 
     def static_upload(self, table):
         """Static records (nnz x 6, touching-pair CSR order) from the caller."""

@@ -1197,6 +1197,48 @@ __global__ void k_overlap_rows(DevGeom g, double omega, bfa_material mat, bfa_ma
   cnt[a] = n;
 }
 
+// Theta factors of AssembleOverlapBlocks' Rytov entries: the environment's
+// and Theta(T_avg) - Theta_env, with T_avg the volume-weighted temperature at
+// the tetrahedron centroids (GetThetaFactor, GetAverageTemperature,
+// VIEMatrix.cc:78-101), both evaluated at omega = 0 as the reference does
+static int rytov_thetas(const bfa_geometry *g, double omega, const bfa_material *temperature, double t_env,
+                        double &theta_env, double &dhat)
+{
+  theta_env = dhat = 0.0;
+  if (!temperature) return BFA_OK;
+  const int nt = g->nt;
+  std::vector<double> cen(3 * (size_t)nt), Tq(18 * (size_t)nt);
+  for (int t = 0; t < nt; t++)
+    for (int d = 0; d < 3; d++) cen[3 * t + d] = g->tetCen[3 * t + d];
+  int rc = bfa_material_eval(temperature, 0.0, nt, cen.data(), Tq.data());
+  if (rc) return rc;
+  double tv = 0.0, vol = 0.0;
+  for (int t = 0; t < nt; t++) { vol += g->tetVol[t]; tv += g->tetVol[t] * Tq[18 * (size_t)t]; }
+  const double TAvg = tv / vol;
+  auto theta = [](double w, double T) { return T == 0.0 ? 0.0 : w / (std::exp(w / (BOLTZMANN_K * T)) - 1.0); };
+  theta_env = theta(omega, t_env);
+  dhat = theta(omega, TAvg) - theta_env;
+  return BFA_OK;
+}
+
+namespace bfa {
+// overlap rows into device arrays (N x MAXOVERLAP each), on stream st
+int overlap_rows_device(bfa_geometry *g, double omega, const bfa_material *eps, const bfa_material *temperature,
+                        double t_env, int *dcnt, int *dcols, void *dV, void *dVI, double *dR, hipStream_t st)
+{
+  double theta_env = 0.0, dhat = 0.0;
+  int rc = rytov_thetas(g, omega, temperature, t_env, theta_env, dhat);
+  if (rc) return rc;
+  bfa_material temp;
+  memset(&temp, 0, sizeof temp);
+  if (temperature) temp = *temperature;
+  hipLaunchKernelGGL(k_overlap_rows, dim3((g->nbf + 127) / 128), dim3(128), 0, st, g->dg, omega, *eps, temp,
+                     temperature ? 1 : 0, theta_env, dhat, dcnt, dcols, (cd *)dV, (cd *)dVI, dR);
+  HIPCHECK(hipGetLastError());
+  return BFA_OK;
+}
+}  // namespace bfa
+
 // AssembleOverlapBlocks for one object: row a holds cnt[a] <= 7 entries at
 // [a*7, a*7 + cnt[a]) of cols / V / VInv / Rytov (host arrays)
 extern "C" int bfa_overlap_blocks(bfa_geometry *g, double omega, const bfa_material *eps,
@@ -1206,26 +1248,7 @@ extern "C" int bfa_overlap_blocks(bfa_geometry *g, double omega, const bfa_mater
   if (!g || !g->uploaded || !eps || !cnt || !cols || !V || !VInv || !Rytov)
     return fail(BFA_ERR_ARG, "bfa_overlap_blocks: bad argument");
   if (omega == 0.0) return fail(BFA_ERR_ARG, "bfa_overlap_blocks: omega = 0");
-  const int N = g->nbf, nt = g->nt;
-  bfa_material temp;
-  memset(&temp, 0, sizeof temp);
-  double theta_env = 0.0, dhat = 0.0;
-  if (temperature) {
-    temp = *temperature;
-    // T_avg: volume-weighted temperature at the tetrahedron centroids
-    // (GetAverageTemperature, VIEMatrix.cc:86-101)
-    std::vector<double> cen(3 * (size_t)nt), Tq(18 * (size_t)nt);
-    for (int t = 0; t < nt; t++)
-      for (int d = 0; d < 3; d++) cen[3 * t + d] = g->tetCen[3 * t + d];
-    int rc = bfa_material_eval(temperature, 0.0, nt, cen.data(), Tq.data());
-    if (rc) return rc;
-    double tv = 0.0, vol = 0.0;
-    for (int t = 0; t < nt; t++) { vol += g->tetVol[t]; tv += g->tetVol[t] * Tq[18 * (size_t)t]; }
-    const double TAvg = tv / vol;
-    auto theta = [](double w, double T) { return T == 0.0 ? 0.0 : w / (std::exp(w / (BOLTZMANN_K * T)) - 1.0); };
-    theta_env = theta(omega, t_env);
-    dhat = theta(omega, TAvg) - theta_env;
-  }
+  const int N = g->nbf;
   int *dc = nullptr, *dcols = nullptr;
   cd *dV = nullptr, *dVI = nullptr;
   double *dR = nullptr;
@@ -1234,9 +1257,8 @@ extern "C" int bfa_overlap_blocks(bfa_geometry *g, double omega, const bfa_mater
   if (hipMalloc(&dc, (size_t)N * 4) != hipSuccess || hipMalloc(&dcols, m * 4) != hipSuccess ||
       hipMalloc(&dV, m * 16) != hipSuccess || hipMalloc(&dVI, m * 16) != hipSuccess || hipMalloc(&dR, m * 8) != hipSuccess)
     rc = fail(BFA_ERR_OOM, "bfa_overlap_blocks: allocation");
+  if (!rc) rc = bfa::overlap_rows_device(g, omega, eps, temperature, t_env, dc, dcols, dV, dVI, dR, 0);
   if (!rc) {
-    hipLaunchKernelGGL(k_overlap_rows, dim3((N + 127) / 128), dim3(128), 0, 0, g->dg, omega, *eps, temp,
-                       temperature ? 1 : 0, theta_env, dhat, dc, dcols, dV, dVI, dR);
     if (hipMemcpy(cnt, dc, (size_t)N * 4, hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(cols, dcols, m * 4, hipMemcpyDeviceToHost) != hipSuccess ||
         hipMemcpy(V, dV, m * 16, hipMemcpyDeviceToHost) != hipSuccess ||
@@ -1773,6 +1795,22 @@ extern "C" int bfa_get_fields(bfa_geometry *g, double omega, const bfa_planewave
   return rc;
 }
 
+namespace bfa {
+// Q_ab = Im Q_PSCAT of GetScatteredPFTIntegrals for every BF pair, both
+// triangles (N x N real, ld >= N): far pairs by tetrahedron-pair tiles,
+// touching pairs by the 16/33-point rule
+void emt_q_matrix(const bfa_geometry *g, double omega, double *Q, int64_t ldq, hipStream_t st)
+{
+  const int N = g->nbf;
+  const int nt2 = (N + FT2 - 1) / FT2;
+  hipLaunchKernelGGL(k_emt_q, dim3((unsigned)((int64_t)nt2 * (nt2 + 1) / 2)), dim3(FT_THREADS), 0, st, g->dg, omega,
+                     Q, ldq, nt2);
+  if (g->dg.nnz > 0)
+    hipLaunchKernelGGL(k_emt_near, dim3((unsigned)((g->dg.nnz + 255) / 256)), dim3(256), 0, st, g->dg, omega, Q,
+                       ldq);
+}
+}  // namespace bfa
+
 extern "C" int bfa_emt_power(bfa_geometry *g, double omega, const bfa_planewave *pw, int npw, const void *d_J,
                              int64_t ldj, double *out, void *stream)
 {
@@ -1790,12 +1828,7 @@ extern "C" int bfa_emt_power(bfa_geometry *g, double omega, const bfa_planewave 
   PWList l;
   memset(&l, 0, sizeof l);
   for (int j = 0; j < npw; j++) l.pw[j] = pw[j];
-  const int nt2 = (N + FT2 - 1) / FT2;
-  hipLaunchKernelGGL(k_emt_q, dim3((unsigned)((int64_t)nt2 * (nt2 + 1) / 2)), dim3(FT_THREADS), 0, st, g->dg, omega,
-                     Q, (int64_t)N, nt2);
-  if (g->dg.nnz > 0)
-    hipLaunchKernelGGL(k_emt_near, dim3((unsigned)((g->dg.nnz + 255) / 256)), dim3(256), 0, st, g->dg, omega, Q,
-                       (int64_t)N);
+  bfa::emt_q_matrix(g, omega, Q, N, st);
   hipLaunchKernelGGL(k_emt_rows, dim3(nb, npw), dim3(256), 0, st, g->dg, omega, (const double *)Q, (int64_t)N,
                      (const cd *)d_J, ldj, l, npw, parts, parts + (size_t)nb * npw);
   std::vector<double> h((size_t)2 * nb * npw);

@@ -172,6 +172,8 @@ class SWGGeometry {
   }
   ~SWGGeometry()
   {
+    delete sinv_;
+    if (work_) bfa_device_free(work_);
     if (g_) bfa_geometry_destroy(g_);
     if (owns_mat_) bfa_material_release(&mat_);
   }
@@ -257,6 +259,74 @@ class SWGGeometry {
     return {out[0], out[1]};
   }
 
+  // ---- buff-neq (one object) ----------------------------------------------
+  // AssembleGBlock(0, 0, Omega, G) (VIEMatrix.cc:365-426)
+  HMatrix *AssembleGBlock(int noa, int nob, cdouble Omega, HMatrix *G)
+  {
+    if (noa != 0 || nob != 0) throw Error(BFA_ERR_ARG, "AssembleGBlock: one object");
+    if (!G) G = AllocateVIEMatrix();
+    check(bfa_assemble_g(g_, Omega.real(), Omega.imag(), G->data(), TotalBFs, nullptr), "AssembleGBlock");
+    return G;
+  }
+
+  // AssembleOverlapBlocks(0, Omega, TSVT, TAvg, ThetaEnv, V, VInv, Rytov)
+  // (VIEMatrix.cc:321-360): rows of <= 7 entries; the temperature is a
+  // compiled expression (nullptr: none) and T_avg is taken from it as
+  // GetAverageTemperature does.
+  struct OverlapRows {
+    std::vector<int> cnt, cols;             // N, N x 7
+    std::vector<cdouble> V, VInv;           // N x 7
+    std::vector<double> Rytov;              // N x 7
+  };
+  OverlapRows AssembleOverlapBlocks(cdouble Omega, const bfa_material *Temperature = nullptr,
+                                    double TEnvironment = 0.0)
+  {
+    const size_t n = (size_t)TotalBFs, m = n * 7;
+    OverlapRows r{std::vector<int>(n), std::vector<int>(m), std::vector<cdouble>(m), std::vector<cdouble>(m),
+                  std::vector<double>(m)};
+    check(bfa_overlap_blocks(g_, Omega.real(), &mat_, Temperature, TEnvironment, r.cnt.data(), r.cols.data(),
+                             (double *)r.V.data(), (double *)r.VInv.data(), r.Rytov.data()),
+          "AssembleOverlapBlocks");
+    return r;
+  }
+
+  // SInverse of CreateBNEQData (CreateBNEQData.cc:119-142): the inverse SWG
+  // overlap matrix, computed on first use and kept
+  const HMatrix *GetOverlapInverse()
+  {
+    if (!sinv_) {
+      sinv_ = new HMatrix(TotalBFs, TotalBFs);
+      check(bfa_gram_inverse(g_, sinv_->data(), TotalBFs, nullptr), "overlap matrix inverse");
+    }
+    return sinv_;
+  }
+
+  // ComputeDressedRytovMatrix (buff-neq GetFlux.cc:155-262) for this
+  // object: DR = W (S^-1 Rytov S^-1)^H, W = (1 + S^-1 V S^-1 G)^-1.  DR is
+  // reallocated when missing or of the wrong size; the work matrices are kept.
+  HMatrix *ComputeDressedRytovMatrix(cdouble Omega, const bfa_material *Temperature, double TEnvironment,
+                                     HMatrix *DR)
+  {
+    if (DR && (DR->NR != TotalBFs || DR->NC != TotalBFs)) { delete DR; DR = nullptr; }
+    if (!DR) DR = AllocateVIEMatrix();
+    const HMatrix *S = GetOverlapInverse();
+    if (!work_) check(bfa_device_malloc(&work_, bfa_dressed_rytov_work_bytes(TotalBFs, TotalBFs)), "work alloc");
+    check(bfa_dressed_rytov(g_, Omega.real(), &mat_, Temperature, TEnvironment, S->data(), DR->data(), TotalBFs,
+                            work_, nullptr),
+          "ComputeDressedRytovMatrix");
+    return DR;
+  }
+
+  // GetPFTMatrix(IF = 0, Omega, {EMT, DRMatrix}) as GetFlux calls it:
+  // {P_abs, P_scat} of the dressed Rytov matrix (EMTPFT.cc:389-476)
+  std::vector<double> GetPFTMatrix(const HMatrix *DRMatrix, cdouble Omega)
+  {
+    if (DRMatrix->NR != TotalBFs || DRMatrix->NC != TotalBFs) throw Error(BFA_ERR_ARG, "GetPFTMatrix: DR size");
+    double out[2];
+    check(bfa_emt_power_dr(g_, Omega.real(), DRMatrix->data(), TotalBFs, out, nullptr), "GetPFTMatrix(DR)");
+    return {out[0], out[1]};
+  }
+
  private:
   void init(int nv, const double *V, int nt, const int *T, int td_order)
   {
@@ -318,6 +388,8 @@ class SWGGeometry {
   std::string mesh_;
   bool cache_dirty_ = false;
   bool owns_mat_ = false;
+  HMatrix *sinv_ = nullptr;     // buff-neq: S^-1, built on first use
+  void *work_ = nullptr;        // buff-neq: dressed-Rytov work matrices
 };
 
 }  // namespace buffamd

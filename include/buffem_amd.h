@@ -161,6 +161,25 @@ int  bfa_assemble_g(bfa_geometry *g, double omega_re, double omega_im, void *d_M
 int  bfa_overlap_blocks(bfa_geometry *g, double omega, const bfa_material *eps,
                         const bfa_material *temperature, double t_env, int *cnt, int *cols,
                         double *V, double *VInv, double *Rytov);
+/* buff-neq (one object).  S^-1, S the SWG overlap (Gram) matrix of
+ * GetOverlapElements (SWGVolume.cc:405-480), as CreateBNEQData builds and
+ * inverts it once per geometry (CreateBNEQData.cc:119-142): d_Sinv is
+ * N x N complex128, ld >= N.                                                */
+int  bfa_gram_inverse(bfa_geometry *g, void *d_Sinv, int64_t ld, void *stream);
+/* ComputeDressedRytovMatrix (buff-neq GetFlux.cc:155-262) at omega:
+ * DR = W (S^-1 R S^-1)^H with W = (1 + S^-1 V S^-1 G)^-1, G the G block,
+ * V and R the overlap blocks of bfa_overlap_blocks (same eps, temperature,
+ * t_env).  d_DR: N x N complex128 out (ld >= N); d_work: device memory of
+ * bfa_dressed_rytov_work_bytes(N, ld) bytes (the reference's WorkMatrix).   */
+size_t bfa_dressed_rytov_work_bytes(int n, int64_t ld);
+int  bfa_dressed_rytov(bfa_geometry *g, double omega, const bfa_material *eps,
+                       const bfa_material *temperature, double t_env, const void *d_Sinv,
+                       void *d_DR, int64_t ld, void *d_work, void *stream);
+/* GetPFTMatrix(IF = 0, Omega, {EMT, DRMatrix}) of GetFlux (GetPFT.cc:108-115,
+ * EMTPFT.cc:389-476): out[0] = P_abs, out[1] = P_scat (= -P_abs) from
+ * sum over a <= b of Re(Z0 DR[b,a]) Im Q_ab.  Host output.                 */
+int  bfa_emt_power_dr(bfa_geometry *g, double omega, const void *d_DR, int64_t ld, double *out,
+                      void *stream);
 int  bfa_assemble_vie_far32(bfa_geometry *g, double omega_re, double omega_im,
                             const bfa_material *mat, void *d_M, int64_t ld, void *stream);
 int  bfa_assemble_rhs(bfa_geometry *g, double omega, const bfa_planewave *pw,

@@ -107,6 +107,7 @@ void orc_rhs(const orc_geom *g, double omega, const orc_planewave *pw, int npw,
 /* EMT power (EMTPFT.cc:151-211,216-319,324-546): out[2*j] = Pabs, [2*j+1] = Pscat */
 void orc_emt_power(const orc_geom *g, double omega, const orc_planewave *pw, int npw,
                    const double *J /* N x npw */, double *out);
+void orc_emt_power_dr(const orc_geom *g, double omega, const double *DR, double *out);
 
 #ifdef __cplusplus
 }

@@ -60,6 +60,7 @@ def lib():
             "orc_overlaps": (I, [P, I, D, P, P, P]),
             "orc_rhs": (None, [P, D, P, I, P]),
             "orc_emt_power": (None, [P, D, P, I, P, P]),
+            "orc_emt_power_dr": (None, [P, D, P, P]),
             "orc_tp_block": (None, [P, P, D, D, I, I, P]),
         }
         for name, (res, args) in sig.items():
@@ -263,6 +264,53 @@ class Geometry:
         out = np.zeros(2 * len(pws))
         lib().orc_emt_power(self.h, float(omega), pws, len(pws), Jc.ctypes.data, out.ctypes.data)
         return out.reshape(-1, 2)
+
+    def emt_power_dr(self, omega, DR):
+        """GetEMTPFT with a DRMatrix, no incident field: (P_abs, P_scat)."""
+        D = np.asfortranarray(DR, dtype=np.complex128)
+        out = np.zeros(2)
+        lib().orc_emt_power_dr(self.h, float(omega), D.ctypes.data, out.ctypes.data)
+        return out
+
+    def gram(self):
+        """Dense SWG overlap (Gram) matrix, GetOverlapElements row by row
+        (SWGVolume.cc:405-480): for each of BF a's tetrahedra (P then M), the
+        self term Area_a^2 (|X-Q|^2/20 + sum|L|^2/180)/Vol and, per other
+        interior face b of the tetrahedron,
+        2 s_a s_b Area_a Area_b/(3 Vol) (3|X-Q|^2/40 + sum|L|^2/120 + (X-Q).(Q_a-Q_b)/8),
+        with L_i = V_i - Q_a over face a's vertices and X its centroid;
+        Vol is the signed volume (SWGVolume.cc:70-80)."""
+        F = self.faces()
+        V, T, N = self.V, self.T, self.N
+        S = np.zeros((N, N))
+        for a in range(N):
+            fv = V[F["V"][a]]
+            for sA, tet, qa in ((1.0, F["PT"][a], F["QP"][a]), (-1.0, F["MT"][a], F["QM"][a])):
+                QA = V[qa]
+                vol = np.linalg.det(V[T[tet][1:]] - V[T[tet][0]]) / 6.0
+                L = fv - QA
+                l2 = (L * L).sum()
+                xmq = fv.mean(0) - QA
+                x2 = xmq @ xmq
+                S[a, a] += F["Area"][a] ** 2 * (x2 / 20.0 + l2 / 180.0) / vol
+                for b in F["tetFI"][tet]:
+                    if b == a or b >= N:
+                        continue
+                    sB, QB = (1.0, V[F["QP"][b]]) if F["PT"][b] == tet else (-1.0, V[F["QM"][b]])
+                    pre = 2.0 * sA * sB * F["Area"][a] * F["Area"][b] / (3.0 * vol)
+                    S[a, b] = pre * (3.0 * x2 / 40.0 + l2 / 120.0 + xmq @ (QA - QB) / 8.0)
+        return S
+
+
+def dressed_rytov(G, Sinv, Vb, R):
+    """ComputeDressedRytovMatrix (buff-neq GetFlux.cc:155-262), step by step
+    as the reference orders it: M3 = 1 + S^-1 (V (S^-1 G)); M1 = (S^-1 R) S^-1;
+    DR = M3 \\ M1^H (LAPACK zgetrf/zgetrs)."""
+    M2 = Sinv @ G
+    M1 = Vb @ M2
+    M3 = Sinv @ M1 + np.eye(len(G))
+    M1 = (Sinv @ R) @ Sinv
+    return lu_solve(M3, M1.conj().T)
 
 
 def lu_solve(M, B):

@@ -651,6 +651,23 @@ static double scat_tettet(const tetpts *A, const tetpts *B, double k)
   return acc;
 }
 
+/* Im Q_PSCAT of GetScatteredPFTIntegrals for BF pair (a, b): the rule by
+ * common vertices and k*radius (EMTPFT.cc:294-318), four tetrahedron pairs */
+static double emt_pair_q(const orc_geom *g, int a, int b, double omega)
+{
+  int ncv = orc_compare_bfs(g, a, b);
+  double kR = fabs(omega) * fmax(g->fRad[a], g->fRad[b]);
+  int hf = kR > 5.0;
+  int np = hf ? (ncv > 0 ? 33 : 16) : (ncv > 0 ? 16 : 4);
+  tetpts PA, MA, PB, MB;
+  tet_points(g, g->fPT[a], g->fPI[a], +1.0, np, &PA);
+  tet_points(g, g->fMT[a], g->fMI[a], -1.0, np, &MA);
+  tet_points(g, g->fPT[b], g->fPI[b], +1.0, np, &PB);
+  tet_points(g, g->fMT[b], g->fMI[b], -1.0, np, &MB);
+  return scat_tettet(&PA, &PB, omega) + scat_tettet(&PA, &MB, omega) + scat_tettet(&MA, &PB, omega)
+       + scat_tettet(&MA, &MB, omega);
+}
+
 void orc_emt_power(const orc_geom *g, double omega, const orc_planewave *pw, int npw,
                    const double *J, double *out)
 {
@@ -666,17 +683,7 @@ void orc_emt_power(const orc_geom *g, double omega, const orc_planewave *pw, int
 #pragma omp for schedule(dynamic, 1)
     for (int a = 0; a < N; a++) {
       for (int b = a; b < N; b++) {
-        int ncv = orc_compare_bfs(g, a, b);
-        double kR = fabs(omega) * fmax(g->fRad[a], g->fRad[b]);
-        int hf = kR > 5.0;
-        int np = hf ? (ncv > 0 ? 33 : 16) : (ncv > 0 ? 16 : 4);
-        tetpts PA, MA, PB, MB;
-        tet_points(g, g->fPT[a], g->fPI[a], +1.0, np, &PA);
-        tet_points(g, g->fMT[a], g->fMI[a], -1.0, np, &MA);
-        tet_points(g, g->fPT[b], g->fPI[b], +1.0, np, &PB);
-        tet_points(g, g->fMT[b], g->fMI[b], -1.0, np, &MB);
-        double q = scat_tettet(&PA, &PB, k) + scat_tettet(&PA, &MB, k)
-                 + scat_tettet(&MA, &PB, k) + scat_tettet(&MA, &MB, k);
+        double q = emt_pair_q(g, a, b, omega);
         for (int j = 0; j < npw; j++) {
           cplx jj = ZVAC * conj(Z[(size_t)j * N + a]) * Z[(size_t)j * N + b];
           loc[j] += creal(jj) * q;
@@ -699,4 +706,20 @@ void orc_emt_power(const orc_geom *g, double omega, const orc_planewave *pw, int
     out[2 * j + 1] = pscat[j];
   }
   free(pscat);
+}
+
+/* GetEMTPFT with a DRMatrix and no incident field (EMTPFT.cc:389-476,
+ * GetJJ GetPFT.cc:108-115): u0JJ = Z0 DR(b, a) for a <= b, P_scat = sum
+ * Re(u0JJ) Im Q, P_abs = -P_scat.  DR: N x N complex, column-major.      */
+void orc_emt_power_dr(const orc_geom *g, double omega, const double *DR, double *out)
+{
+  init_rules();
+  int N = g->nf_int;
+  const cplx *Z = (const cplx *)DR;
+  double pscat = 0.0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : pscat)
+  for (int a = 0; a < N; a++)
+    for (int b = a; b < N; b++) pscat += creal(ZVAC * Z[(size_t)a * N + b]) * emt_pair_q(g, a, b, omega);
+  out[0] = -pscat;
+  out[1] = pscat;
 }

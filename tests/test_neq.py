@@ -165,3 +165,105 @@ def test_many_rhs_solve(bfa, torch_cuda, n, nrhs):
         print(f"[neq] many-RHS solve n={n} nrhs={nrhs} c: forward {err:.2e} backward {eta:.2e}"
               f" (LAPACK complex64 {ref32:.2e}, {eta32:.2e})", flush=True)
         assert err < 3 * ref32 and eta < 3 * eta32
+
+
+# ---- dressed Rytov matrix (ComputeDressedRytovMatrix) and its EMT power ----
+
+def _neq_bufs(torch, g):
+    N = g.N
+    Sinv = torch.empty((N, N), dtype=torch.complex128, device="cuda:0")
+    g.gram_inverse(Sinv)
+    DR = torch.empty_like(Sinv)
+    work = torch.empty(g.dressed_rytov_work_bytes(), dtype=torch.uint8, device="cuda:0")
+    return Sinv, DR, work
+
+
+def _oracle_G(orc, o, w, omat):
+    """The G block from the oracle: its VIE matrix minus GetOverlaps' V^-1 entries."""
+    G = o.assemble(w, omat, o.static_table(10))
+    for a in range(o.N):
+        idx, val = o.overlaps(a, w, omat)
+        G[a, idx] -= val
+    return G
+
+
+def _dense_rows(N, cnt, cols, vals):
+    M = np.zeros((N, N), dtype=vals.dtype)
+    for a in range(N):
+        M[a, cols[a, :cnt[a]]] = vals[a, :cnt[a]]
+    return M
+
+
+def test_gram_inverse(bfa, orc, torch_cuda, geo):
+    V, T, g, o = geo
+    Sinv, _, _ = _neq_bufs(torch_cuda, g)
+    want = np.linalg.inv(o.gram())
+    err = np.abs(Sinv.cpu().numpy().T - want).max() / np.abs(want).max()
+    print(f"[neq] S^-1 vs oracle {err:.2e}", flush=True)
+    assert err < 1e-12
+
+
+def test_dressed_rytov_constant_eps(bfa, orc, torch_cuda, geo):
+    """Every input from the oracle: G, S, and V, R by their closed forms in
+    V^-1 for a constant isotropic eps (no temperature field: RelDeltaTheta = 1)."""
+    V, T, g, o = geo
+    N = g.N
+    w, eps = 0.8, 4.0 + 0.6j
+    Sinv, DR, work = _neq_bufs(torch_cuda, g)
+    g.dressed_rytov(w, bfa.material("iso", eps=eps), Sinv, DR, work)
+    omat = orc.material("iso", eps=eps)
+    G = _oracle_G(orc, o, w, omat)
+    VI = np.zeros((N, N), dtype=complex)
+    for a in range(N):
+        idx, val = o.overlaps(a, w, omat)
+        VI[a, idx] = val
+    Vb = w ** 4 * (eps - 1) ** 2 * VI
+    R = (-4 * w ** 3 / (np.pi * Z0) * (eps - 1).imag * (eps - 1) * VI).real
+    want = orc.dressed_rytov(G, np.linalg.inv(o.gram()), Vb, R)
+    got = DR.cpu().numpy().T
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print(f"[neq] dressed Rytov (constant eps) vs oracle {err:.2e}", flush=True)
+    assert err < 1e-10
+    # its EMT power against the oracle's, and P_abs = -P_scat
+    p = g.emt_power_dr(w, DR)
+    po = o.emt_power_dr(w, want)
+    print(f"[neq] EMT(DR) {p[1]:.6e} oracle {po[1]:.6e}", flush=True)
+    assert abs(p[1] - po[1]) <= 1e-9 * abs(po[1]) and p[0] == -p[1]
+
+
+def test_dressed_rytov_temperature_gradient(bfa, orc, torch_cuda, geo):
+    """sincos eps and a temperature gradient: V, R from AssembleOverlapBlocks
+    (pinned in test_overlap_blocks / test_rytov_with_temperature_gradient), G
+    and S from the oracle."""
+    V, T, g, o = geo
+    N = g.N
+    w, t_env = 1.1, 280.0
+    mat = bfa.material("sincos", base=4.0, amp=2.0, imag=0.1)
+    Tm = bfa.material_expr("300 + 40*x - 15*z")
+    Sinv, DR, work = _neq_bufs(torch_cuda, g)
+    g.dressed_rytov(w, mat, Sinv, DR, work, temperature=Tm, t_env=t_env)
+    cnt, cols, Vr, _, Rr = g.overlap_blocks(w, mat, temperature=Tm, t_env=t_env)
+    G = _oracle_G(orc, o, w, orc.material("sincos", base=4.0, amp=2.0, imag=0.1))
+    want = orc.dressed_rytov(G, np.linalg.inv(o.gram()), _dense_rows(N, cnt, cols, Vr), _dense_rows(N, cnt, cols, Rr))
+    err = np.abs(DR.cpu().numpy().T - want).max() / np.abs(want).max()
+    print(f"[neq] dressed Rytov (sincos, T gradient) vs oracle {err:.2e}", flush=True)
+    assert err < 1e-10
+
+
+def test_emt_power_dr_is_the_j_path_for_a_rank_one_dr(bfa, torch_cuda, geo):
+    """DR = J J^H gives the J-vector EMT's scattered power (GetJJ's two forms)."""
+    torch = torch_cuda
+    V, T, g, o = geo
+    N = g.N
+    rng = np.random.default_rng(21)
+    J = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+    w = 0.9
+    pw = bfa.planewaves([((1, 0, 0), (0, 0, 1))])
+    dJ = torch.from_numpy(J[None, :].copy()).to("cuda:0")
+    pj = g.emt_power(w, pw, dJ)
+    DRm = np.outer(J, J.conj())
+    U = np.triu(rng.standard_normal((N, N)), 1)          # never read
+    dDR = torch.from_numpy(np.ascontiguousarray((DRm + U).T)).to("cuda:0")
+    p = g.emt_power_dr(w, dDR)
+    print(f"[neq] EMT(J J^H) {p[1]:.6e} J path {pj[0][1]:.6e}", flush=True)
+    assert abs(p[1] - pj[0][1]) <= 1e-12 * abs(pj[0][1])
