@@ -466,10 +466,22 @@ __device__ __forceinline__ void trsm_lf_body(int kb, int n, const cplx<T> *__res
       cplx<T> acc[SC];
 #pragma unroll
       for (int c = 0; c < SC; c++) acc[c] = cplx<T>{0, 0};
-      for (int k = 0; k < bk; k++) {
-        const cplx<T> l = L[(size_t)(b0 + k) * ldl + r];
+      if (bk == 32) {
+        // a full block: the row's 32 factor entries are loaded together
+        // before use, so their latencies overlap instead of adding up
+        cplx<T> fr[32];
 #pragma unroll
-        for (int c = 0; c < SC; c++) acc[c] = c_add(acc[c], c_mul(l, X[c][b0 + k]));
+        for (int k = 0; k < 32; k++) fr[k] = L[(size_t)(b0 + k) * ldl + r];
+#pragma unroll
+        for (int k = 0; k < 32; k++)
+#pragma unroll
+          for (int c = 0; c < SC; c++) acc[c] = c_add(acc[c], c_mul(fr[k], X[c][b0 + k]));
+      } else {
+        for (int k = 0; k < bk; k++) {
+          const cplx<T> l = L[(size_t)(b0 + k) * ldl + r];
+#pragma unroll
+          for (int c = 0; c < SC; c++) acc[c] = c_add(acc[c], c_mul(l, X[c][b0 + k]));
+        }
       }
 #pragma unroll
       for (int c = 0; c < SC; c++) X[c][r] = c_sub(X[c][r], acc[c]);
@@ -530,10 +542,22 @@ __global__ __launch_bounds__(256) void k_trsm_uf(int kb, int n, const cplx<T> *_
       cplx<T> acc[SC];
 #pragma unroll
       for (int c = 0; c < SC; c++) acc[c] = cplx<T>{0, 0};
-      for (int k = 0; k < bk; k++) {
-        const cplx<T> u = U[(size_t)(b0 + k) * ldu + r];
+      if (bk == 32) {
+        // a full block: the row's 32 factor entries are loaded together
+        // before use, so their latencies overlap instead of adding up
+        cplx<T> fr[32];
 #pragma unroll
-        for (int c = 0; c < SC; c++) acc[c] = c_add(acc[c], c_mul(u, X[c][b0 + k]));
+        for (int k = 0; k < 32; k++) fr[k] = U[(size_t)(b0 + k) * ldu + r];
+#pragma unroll
+        for (int k = 0; k < 32; k++)
+#pragma unroll
+          for (int c = 0; c < SC; c++) acc[c] = c_add(acc[c], c_mul(fr[k], X[c][b0 + k]));
+      } else {
+        for (int k = 0; k < bk; k++) {
+          const cplx<T> u = U[(size_t)(b0 + k) * ldu + r];
+#pragma unroll
+          for (int c = 0; c < SC; c++) acc[c] = c_add(acc[c], c_mul(u, X[c][b0 + k]));
+        }
       }
 #pragma unroll
       for (int c = 0; c < SC; c++) X[c][r] = c_sub(X[c][r], acc[c]);
