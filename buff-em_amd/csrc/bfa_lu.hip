@@ -13,8 +13,10 @@
 // trailing update C -= A*B runs on MFMA (4 real products per complex
 // product, v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 #include <climits>
 #include <cstdlib>
+#include <cstring>
 #include <cmath>
 #include <vector>
 
@@ -313,6 +315,53 @@ static void prof_begin(hipStream_t st, hipEvent_t &a)
   (void)hipEventRecord(a, st);
 }
 
+// The large complex128 trailing updates (m, n >= 256) are plain C -= A B
+// products and go to rocBLAS's zgemm (alpha = -1, beta = 1), 70 TF/s on these
+// shapes against 48 TF/s for k_gemm_sub2<double>: factorisation 517 -> 484 ms
+// at N = 19,872.  The complex64 ones stay on k_gemm_sub2<float>: its 64 KB
+// workgroups leave room for the look-ahead panel's workgroups on every CU,
+// and with rocBLAS's cgemm beside it the panel slows enough that the
+// factorisation takes longer (317 against 297 ms).  BFA_GEMM=own or =blas
+// selects one kind for both precisions.  Streams being captured into a graph
+// always use the hand-written kernels.
+static rocblas_handle blas_handle(hipStream_t st, bool f64)
+{
+  static rocblas_handle h[64] = {};
+  static int mode = -1;     // 0: own kernels, 1: complex128 only, 2: both
+  if (mode < 0) {
+    const char *e = getenv("BFA_GEMM");
+    mode = !e ? 1 : strcmp(e, "own") == 0 ? 0 : strcmp(e, "blas") == 0 ? 2 : 1;
+  }
+  if (mode == 0 || (mode == 1 && !f64)) return nullptr;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return nullptr;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  if (!h[dev] && rocblas_create_handle(&h[dev]) != rocblas_status_success) {
+    h[dev] = nullptr;
+    return nullptr;
+  }
+  if (rocblas_set_stream(h[dev], st) != rocblas_status_success) return nullptr;
+  return h[dev];
+}
+
+static bool blas_gemm_sub(rocblas_handle h, int m, int n, int k, const cplx<double> *A, int64_t lda,
+                          const cplx<double> *B, int64_t ldb, cplx<double> *C, int64_t ldc)
+{
+  const rocblas_double_complex alpha(-1.0, 0.0), beta(1.0, 0.0);
+  return rocblas_zgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &alpha,
+                       (const rocblas_double_complex *)A, lda, (const rocblas_double_complex *)B, ldb, &beta,
+                       (rocblas_double_complex *)C, ldc) == rocblas_status_success;
+}
+static bool blas_gemm_sub(rocblas_handle h, int m, int n, int k, const cplx<float> *A, int64_t lda,
+                          const cplx<float> *B, int64_t ldb, cplx<float> *C, int64_t ldc)
+{
+  const rocblas_float_complex alpha(-1.0f, 0.0f), beta(1.0f, 0.0f);
+  return rocblas_cgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, k, &alpha,
+                       (const rocblas_float_complex *)A, lda, (const rocblas_float_complex *)B, ldb, &beta,
+                       (rocblas_float_complex *)C, ldc) == rocblas_status_success;
+}
+
 template <typename T>
 static int gemm_sub(int m, int n, int k, const cplx<T> *A, int64_t lda, const cplx<T> *B, int64_t ldb, cplx<T> *C,
                     int64_t ldc, hipStream_t st)
@@ -323,7 +372,10 @@ static int gemm_sub(int m, int n, int k, const cplx<T> *A, int64_t lda, const cp
   const int p = sizeof(T) == 8 ? 0 : 1;
   if (g_prof.on) prof_begin(st, e0);
   const bool big = m >= 256 && n >= 256;
-  if (big) {
+  rocblas_handle bh = big ? blas_handle(st, sizeof(T) == 8) : nullptr;
+  if (bh && blas_gemm_sub(bh, m, n, k, A, lda, B, ldb, C, ldc)) {
+    // done by rocBLAS
+  } else if (big) {
     const int tm = (m + G2M - 1) / G2M, tn = (n + G2N - 1) / G2N;
     hipLaunchKernelGGL(k_gemm_sub2<T>, dim3(tm * tn), dim3(512), 0, st, m, n, k, A, lda, B, ldb, C, ldc, tm, tn);
   } else {
@@ -717,6 +769,9 @@ __device__ __forceinline__ void panel_body(cplx<T> *A, int m, int w, int64_t lda
                                            PanelShared<T> &sh)
 {
 #define RW(r, c) rows[(size_t)(c) * rpw + (r)]
+  // The panel is latency-bound and runs beside the trailing GEMM: its waves
+  // take instruction-issue priority over the GEMM's waves on shared SIMDs.
+  __builtin_amdgcn_s_setprio(3);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = blockDim.x >> 6;
   const int r0 = g * rpw, r1 = min(m, r0 + rpw), nr = max(0, r1 - r0);
   cplx<T> *rdata = (cplx<T> *)pw.rowdata;
