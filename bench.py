@@ -31,7 +31,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of the roofline kernel at N = 19,872, measured by two
 # rocprofv3 --pmc passes (FETCH_SIZE x2, WRITE_SIZE x1; the scale factors are
 # read off a streaming kernel of known size in the same run)
-MEASURED_TRAFFIC = {("k_gemm_sub2<float>", 19872): 2.169e9}
+MEASURED_TRAFFIC = {("rocblas_cgemm", 19872): 1.972e9}
 
 
 def trailing_gemm_schedule(N, nb=256, elem=8):
@@ -339,13 +339,13 @@ def main():
     peak = PEAK_FP64_TFLOPS if best == "fp64" else PEAK_FP32_TFLOPS
     gemm_tflops = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else None
     roofline = {
-        "kernel": ("rocBLAS zgemm (LU trailing update, complex128 on f64 MFMA)"
+        "kernel": ("rocBLAS zgemm (LU trailing update, complex128 on f64 MFMA, 192 of 256 CUs)"
                    if best == "fp64" else
-                   "k_gemm_sub2<float> (LU trailing update, 128x128 tiles, complex64 on f32 MFMA)"),
+                   "rocBLAS cgemm (LU trailing update, complex64 on f32 MFMA, 192 of 256 CUs)"),
         "bound": "mfma", "unit": "TFLOP/s",
         "achieved": gemm_tflops, "peak": peak,
         "frac": gemm_tflops / peak if gemm_tflops else None,
-        "traffic": MEASURED_TRAFFIC.get(("k_gemm_sub2<float>" if best != "fp64" else "k_gemm_sub2<double>", N)),
+        "traffic": MEASURED_TRAFFIC.get(("rocblas_cgemm" if best != "fp64" else "rocblas_zgemm", N)),
         "algorithmic_bytes_per_launch": (lambda c: c[1] / c[0] if c[0] else None)(
             trailing_gemm_schedule(N, elem=16 if best == "fp64" else 8)),
         "per_launch_flops": gemm_flops / max(gemm_launches, 1),

@@ -13,6 +13,7 @@
 // trailing update C -= A*B runs on MFMA (4 real products per complex
 // product, v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <rocblas/rocblas.h>
 #include <climits>
 #include <cstdlib>
@@ -324,15 +325,15 @@ static void prof_begin(hipStream_t st, hipEvent_t &a)
 // factorisation takes longer (317 against 297 ms).  BFA_GEMM=own or =blas
 // selects one kind for both precisions.  Streams being captured into a graph
 // always use the hand-written kernels.
-static rocblas_handle blas_handle(hipStream_t st, bool f64)
+static rocblas_handle blas_handle(hipStream_t st, bool want)
 {
   static rocblas_handle h[64] = {};
-  static int mode = -1;     // 0: own kernels, 1: complex128 only, 2: both
+  static int mode = -1;     // 0: own kernels, 1: where the caller wants rocBLAS, 2: always
   if (mode < 0) {
     const char *e = getenv("BFA_GEMM");
     mode = !e ? 1 : strcmp(e, "own") == 0 ? 0 : strcmp(e, "blas") == 0 ? 2 : 1;
   }
-  if (mode == 0 || (mode == 1 && !f64)) return nullptr;
+  if (mode == 0 || (mode == 1 && !want)) return nullptr;
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return nullptr;
   int dev = 0;
@@ -364,7 +365,7 @@ static bool blas_gemm_sub(rocblas_handle h, int m, int n, int k, const cplx<floa
 
 template <typename T>
 static int gemm_sub(int m, int n, int k, const cplx<T> *A, int64_t lda, const cplx<T> *B, int64_t ldb, cplx<T> *C,
-                    int64_t ldc, hipStream_t st)
+                    int64_t ldc, hipStream_t st, bool lib_ok = false)
 {
   if (m <= 0 || n <= 0 || k <= 0) return BFA_OK;
   dim3 grid((m + GBM - 1) / GBM, (n + GBN - 1) / GBN);
@@ -372,7 +373,7 @@ static int gemm_sub(int m, int n, int k, const cplx<T> *A, int64_t lda, const cp
   const int p = sizeof(T) == 8 ? 0 : 1;
   if (g_prof.on) prof_begin(st, e0);
   const bool big = m >= 256 && n >= 256;
-  rocblas_handle bh = big ? blas_handle(st, sizeof(T) == 8) : nullptr;
+  rocblas_handle bh = big ? blas_handle(st, sizeof(T) == 8 || lib_ok) : nullptr;
   if (bh && blas_gemm_sub(bh, m, n, k, A, lda, B, ldb, C, ldc)) {
     // done by rocBLAS
   } else if (big) {
@@ -450,7 +451,7 @@ static void trsm_diag(int kb, int n, const cplx<T> *L, int64_t ldl, cplx<T> *B, 
 }
 
 template <typename T> static int gemm_sub(int m, int n, int k, const cplx<T> *A, int64_t lda, const cplx<T> *B,
-                                          int64_t ldb, cplx<T> *C, int64_t ldc, hipStream_t st);
+                                          int64_t ldb, cplx<T> *C, int64_t ldc, hipStream_t st, bool lib_ok);
 
 // B(kb x n) := L^-1 B or U^-1 B for any kb, blocked by TRSM_KB
 template <typename T, bool LOWER>
@@ -724,7 +725,9 @@ __device__ __forceinline__ void grid_barrier(unsigned *bar, int g, unsigned targ
 #pragma unroll
       for (int off = 4; off > 0; off >>= 1) c += __shfl_xor(c, off, 64);
       if (__shfl(c, 0, 64) >= target) break;
-      if (++spins > SPIN_LIMIT) {
+      // a barrier that already timed out elsewhere ends this one at once
+      if ((++spins & 1023) == 0 && __shfl(lane == 0 ? ld_agent(&flags[0]) : 0, 0, 64)) break;
+      if (spins > SPIN_LIMIT) {
         if (lane == 0) atomicOr(&flags[0], 1);
         break;
       }
@@ -975,6 +978,10 @@ struct LUWork {
   int num_cu = 0;
   hipStream_t side = nullptr;    // bulk trailing updates of the look-ahead LU
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  // CU-partitioned look-ahead: the panel on mask_cus CUs, the updates on the rest
+  int mask_cus = -1;             // -1 undecided, 0 off
+  hipStream_t s_pan = nullptr, s_bulk = nullptr;
+  hipEvent_t evm[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 static LUWork g_lu[16];
 
@@ -1004,8 +1011,12 @@ static int lu_work(LUWork *&w)
 }
 
 template <typename T>
-static int panel_base(cplx<T> *A, int m, int w, int64_t lda, int *ipiv, int piv_off, LUWork *lw, hipStream_t st)
+static int panel_base(cplx<T> *A, int m, int w, int64_t lda, int *ipiv, int piv_off, LUWork *lw, hipStream_t st,
+                      int cus)
 {
+  // cus < num_cu: the panel's stream owns cus CUs (CU-partitioned
+  // look-ahead) and nothing else runs there, so its workgroups take the
+  // largest LDS blocks (fewest workgroups, at most one per CU).
   // complex128: as few workgroups as the LDS allows (fewer arrivals per
   // grid barrier), up to ~150 KB of rows each, at most one per CU.
   // complex64: the look-ahead runs this panel beside the trailing update,
@@ -1020,11 +1031,23 @@ static int panel_base(cplx<T> *A, int m, int w, int64_t lda, int *ipiv, int piv_
                                         (int)lds_cap) == hipSuccess ? 1 : -1;
   }
   if (attr_done[ti] < 0) lds_cap = 64 * 1024;
+  const bool masked = cus < lw->num_cu;
+  if (masked) {
+    static int attr_m[2] = {0, 0};
+    if (!attr_m[ti])
+      attr_m[ti] = hipFuncSetAttribute((const void *)k_panel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       158 * 1024) == hipSuccess ? 1 : -1;
+    if (attr_m[ti] < 0) return fail(BFA_ERR_HIP, "panel LDS attribute for the CU-partitioned LU");
+    lds_cap = 158 * 1024;
+  }
   const int row_bytes = w * (int)sizeof(cplx<T>);
   int rpw = std::max(16, (int)(lds_cap / row_bytes));
   int nwg = (m + rpw - 1) / rpw;
-  if (nwg > lw->num_cu) { nwg = lw->num_cu; rpw = (m + nwg - 1) / nwg; }
-  if (sizeof(T) == 4) {
+  if (nwg > cus) { nwg = cus; rpw = (m + nwg - 1) / nwg; }
+  // masked: one workgroup per panel CU (at least 16 rows each), so the
+  // per-column row updates spread over every panel CU's LDS
+  if (masked) { nwg = std::min(cus, std::max(1, (m + 15) / 16)); rpw = (m + nwg - 1) / nwg; }
+  if (sizeof(T) == 4 && !masked) {
     const int rco = std::max(16, (31 * 1024) / row_bytes);
     if ((m + rco - 1) / rco <= lw->num_cu) { rpw = rco; nwg = (m + rpw - 1) / rpw; }
   }
@@ -1036,14 +1059,15 @@ static int panel_base(cplx<T> *A, int m, int w, int64_t lda, int *ipiv, int piv_
   // The grid barrier needs every workgroup resident at once: the grid never
   // exceeds what the occupancy calculator says fits on the device, and the
   // barrier's bounded spin reports (flags[0]) rather than hangs otherwise.
-  static int resident[2] = {0, 0};
-  if (!resident[ti]) {
+  static int per_cu[2][2] = {{-1, -1}, {-1, -1}};
+  int &pc = per_cu[masked ? 1 : 0][ti];
+  if (pc < 0) {
     int per = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_panel<T>, 256, lds_cap) != hipSuccess) per = 0;
-    resident[ti] = per * lw->num_cu;
+    pc = per;
   }
-  if (nwg > resident[ti])
-    return fail(BFA_ERR_HIP, "panel needs %d co-resident workgroups, device fits %d", nwg, resident[ti]);
+  if (nwg > pc * cus)
+    return fail(BFA_ERR_HIP, "panel needs %d co-resident workgroups, %d CUs fit %d", nwg, cus, pc * cus);
   hipLaunchKernelGGL(k_zero_u32, dim3((PB_WORDS + 255) / 256), dim3(256), 0, st, lw->pw.bar, PB_WORDS);   // barrier and keys
   hipLaunchKernelGGL(k_panel<T>, dim3(nwg), dim3(256), lds, st, A, m, w, lda, ipiv, rpw, lw->pw, piv_off);
   HIPCHECK(hipGetLastError());
@@ -1052,11 +1076,12 @@ static int panel_base(cplx<T> *A, int m, int w, int64_t lda, int *ipiv, int piv_
 
 // recursive panel: m x w, pivots written absolute (offset piv_off + local)
 template <typename T>
-static int panel_rec(cplx<T> *A, int m, int w, int64_t lda, int *ipiv, int piv_off, LUWork *lw, hipStream_t st)
+static int panel_rec(cplx<T> *A, int m, int w, int64_t lda, int *ipiv, int piv_off, LUWork *lw, hipStream_t st,
+                     int cus)
 {
-  if (w <= PW) return panel_base<T>(A, m, w, lda, ipiv, piv_off, lw, st);
+  if (w <= PW) return panel_base<T>(A, m, w, lda, ipiv, piv_off, lw, st, cus);
   const int w1 = w / 2, w2 = w - w1;
-  int rc = panel_rec<T>(A, m, w1, lda, ipiv, piv_off, lw, st);
+  int rc = panel_rec<T>(A, m, w1, lda, ipiv, piv_off, lw, st, cus);
   if (rc) return rc;
   // apply left pivots to the right half (rows relative to panel top)
   cplx<T> *Ar = A + (size_t)w1 * lda;
@@ -1064,12 +1089,108 @@ static int panel_rec(cplx<T> *A, int m, int w, int64_t lda, int *ipiv, int piv_o
                      w1, (const int *)(ipiv - piv_off));
   trsm_lower<T>(w1, w2, A, lda, Ar, lda, st);
   gemm_sub<T>(m - w1, w2, w1, A + w1, lda, Ar, lda, Ar + w1, lda, st);
-  rc = panel_rec<T>(A + (size_t)w1 * lda + w1, m - w1, w2, lda, ipiv + w1, piv_off + w1, lw, st);
+  rc = panel_rec<T>(A + (size_t)w1 * lda + w1, m - w1, w2, lda, ipiv + w1, piv_off + w1, lw, st, cus);
   if (rc) return rc;
   // apply right pivots to the left half
   hipLaunchKernelGGL(k_laswp<T>, dim3((w1 + 127) / 128), dim3(128), 0, st, A - (ptrdiff_t)piv_off, lda, 0, w1,
                      piv_off + w1, w2, (const int *)(ipiv - piv_off));
   return BFA_OK;
+}
+
+// CU-partitioned look-ahead.  The panel's stream owns P CUs (BFA_LU_MASK,
+// default 64; 0: off) and the update stream the other 256 - P, through
+// per-queue CU masks.  Beside an unmasked GEMM the latency-bound panel shares
+// SIMDs with it and runs ~3x slower than alone (5.2 against 1.8 ms per 256
+// columns at N = 19,872); on its own CUs it keeps its speed while the bulk
+// update runs on rocBLAS's GEMM over the rest: factorisation at N = 19,872
+// 297 -> 276 ms (complex64), 496 -> 465 ms (complex128).  A matrix whose
+// first panel does not fit the panel CUs' LDS takes the shared-CU path.
+static int lu_mask_setup(LUWork *lw)
+{
+  if (lw->mask_cus >= 0) return lw->mask_cus;
+  lw->mask_cus = 0;
+  const char *e = getenv("BFA_LU_MASK");
+  const int P = e ? atoi(e) : 64;
+  // 8 XCDs of 32 CUs.  The driver deals mask bits to the XCDs round-robin
+  // (bit i -> XCD i % 8; measured on MI355X by the XCC_ID of workgroups run
+  // under a mask with one extra bit), so bits [0, P) give each XCD P/8 CUs;
+  // a queue with no CU on some XCD could never run that XCD's workgroups.
+  // Workgroups are also dealt round-robin over each XCD's shader engines, so
+  // P is a multiple of 32 (P/32 panel CUs per engine): with 6 CUs per XCD
+  // some panel workgroups found no free CU and the grid barrier timed out.
+  if (P <= 0 || P > 128 || P % 32 != 0 || lw->num_cu != 256) return 0;
+  uint32_t mp[8] = {}, mb[8];
+  for (int b = 0; b < P; b++) mp[b / 32] |= 1u << (b % 32);
+  for (int i = 0; i < 8; i++) mb[i] = ~mp[i];
+  const uint32_t words = 8;
+  if (hipExtStreamCreateWithCUMask(&lw->s_pan, words, mp) != hipSuccess ||
+      hipExtStreamCreateWithCUMask(&lw->s_bulk, words, mb) != hipSuccess)
+    return 0;
+  for (auto &ev : lw->evm)
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return 0;
+  lw->mask_cus = P;
+  return P;
+}
+
+template <typename T>
+static int lu_factor_masked(cplx<T> *A, int n, int64_t lda, int *ipiv, int *info, hipStream_t st, int nb, LUWork *lw)
+{
+  const int P = lw->mask_cus;
+  hipStream_t sp = lw->s_pan, sb = lw->s_bulk;
+  hipEvent_t ev_s = lw->evm[0], ev_p = lw->evm[1], ev_n = lw->evm[2], ev_e = lw->evm[3];
+  HIPCHECK(hipMemsetAsync(lw->pw.flags, 0, 8, st));
+  HIPCHECK(hipEventRecord(ev_s, st));
+  HIPCHECK(hipStreamWaitEvent(sp, ev_s, 0));
+  HIPCHECK(hipStreamWaitEvent(sb, ev_s, 0));
+  int rc = panel_rec<T>(A, n, std::min(nb, n), lda, ipiv, 0, lw, sp, P);
+  if (rc) return rc;
+  for (int j = 0; j < n; j += nb) {
+    const int jb = std::min(nb, n - j);
+    cplx<T> *Ajj = A + (size_t)j * lda + j;
+    const int next = j + jb;
+    HIPCHECK(hipEventRecord(ev_p, sp));          // panel j done
+    HIPCHECK(hipStreamWaitEvent(sb, ev_p, 0));
+    laswp<T>(A, lda, 0, j, j, jb, ipiv, sb);    // rows of panel j, columns left of it
+    if (next >= n) break;
+    const int nb2 = std::min(nb, n - next);
+    cplx<T> *Anext = A + (size_t)next * lda;
+    // block column next on the update CUs, then its panel on the panel CUs
+    laswp<T>(A, lda, next, next + nb2, j, jb, ipiv, sb);
+    trsm_lower<T>(jb, nb2, Ajj, lda, Anext + j, lda, sb);
+    gemm_sub<T>(n - next, nb2, jb, Ajj + jb, lda, Anext + j, lda, Anext + next, lda, sb, true);
+    HIPCHECK(hipEventRecord(ev_n, sb));
+    HIPCHECK(hipStreamWaitEvent(sp, ev_n, 0));
+    const int rest = n - next - nb2;
+    if (rest > 0) {
+      cplx<T> *Arest = A + (size_t)(next + nb2) * lda;
+      laswp<T>(A, lda, next + nb2, n, j, jb, ipiv, sb);
+      trsm_lower<T>(jb, rest, Ajj, lda, Arest + j, lda, sb);
+      gemm_sub<T>(n - next, rest, jb, Ajj + jb, lda, Arest + j, lda, Arest + next, lda, sb, true);
+    }
+    rc = panel_rec<T>(Anext + next, n - next, nb2, lda, ipiv + next, next, lw, sp, P);
+    if (rc) return rc;
+  }
+  HIPCHECK(hipEventRecord(ev_e, sb));
+  HIPCHECK(hipStreamWaitEvent(st, ev_e, 0));
+  HIPCHECK(hipEventRecord(ev_e, sp));
+  HIPCHECK(hipStreamWaitEvent(st, ev_e, 0));
+  HIPCHECK(hipGetLastError());
+  int flags[2] = {0, 0};
+  HIPCHECK(hipMemcpyAsync(flags, lw->pw.flags, 8, hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  if (flags[0]) return fail(BFA_ERR_HIP, "LU panel barrier timed out");
+  if (info) *info = flags[1];
+  return BFA_OK;
+}
+
+// whether every panel of an n x n factorisation fits the panel CUs at once
+// (the first, tallest panel decides): one workgroup of <= 158 KB per CU,
+// dealt round-robin to the XCDs, each of which holds P/8 panel CUs
+template <typename T> static bool masked_fits(int n, int nb, const LUWork *lw)
+{
+  const size_t row_bytes = (size_t)std::min(nb, PW) * sizeof(cplx<T>);
+  const size_t rpw = ((size_t)n + lw->mask_cus - 1) / lw->mask_cus;
+  return rpw * row_bytes <= 158 * 1024;
 }
 
 // Blocked right-looking LU with one block of look-ahead.  Stream st carries
@@ -1086,6 +1207,8 @@ int lu_factor(cplx<T> *A, int n, int64_t lda, int *ipiv, int *info, hipStream_t 
   LUWork *lw = nullptr;
   int rc = lu_work(lw);
   if (rc) return rc;
+  if (lu_mask_setup(lw) > 0 && masked_fits<T>(n, nb, lw))
+    return lu_factor_masked<T>(A, n, lda, ipiv, info, st, nb, lw);
   if (!lw->side) {
     HIPCHECK(hipStreamCreateWithFlags(&lw->side, hipStreamNonBlocking));
     for (auto &e : lw->ev) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -1095,7 +1218,7 @@ int lu_factor(cplx<T> *A, int n, int64_t lda, int *ipiv, int *info, hipStream_t 
   HIPCHECK(hipMemsetAsync(lw->pw.flags, 0, 8, st));
   HIPCHECK(hipEventRecord(ev_s, st));
   HIPCHECK(hipStreamWaitEvent(sb, ev_s, 0));
-  rc = panel_rec<T>(A, n, std::min(nb, n), lda, ipiv, 0, lw, st);
+  rc = panel_rec<T>(A, n, std::min(nb, n), lda, ipiv, 0, lw, st, lw->num_cu);
   if (rc) return rc;
   for (int j = 0; j < n; j += nb) {
     const int jb = std::min(nb, n - j);
@@ -1120,7 +1243,7 @@ int lu_factor(cplx<T> *A, int n, int64_t lda, int *ipiv, int *info, hipStream_t 
       gemm_sub<T>(n - next, rest, jb, Ajj + jb, lda, Arest + j, lda, Arest + next, lda, sb);
     }
     HIPCHECK(hipEventRecord(ev_b, sb));
-    rc = panel_rec<T>(Anext + next, n - next, nb2, lda, ipiv + next, next, lw, st);
+    rc = panel_rec<T>(Anext + next, n - next, nb2, lda, ipiv + next, next, lw, st, lw->num_cu);
     if (rc) return rc;
     // the next block column was updated by this bulk pass
     HIPCHECK(hipStreamWaitEvent(st, ev_b, 0));
