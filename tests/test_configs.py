@@ -10,6 +10,8 @@
   integrated in FP32 the error is reported (it is set by the FP32 cubature,
   not by the solver) and bounded loosely.
 - An exactly singular matrix reports the LAPACK-style first zero pivot.
+- A larger matrix (n = 9,000) factors through the CU-partitioned
+  look-ahead with a backward error at the precision's level.
 """
 import numpy as np
 import pytest
@@ -124,3 +126,33 @@ def test_lu_singular_reports_first_zero_pivot(bfa, torch_cuda):
             dp = torch.zeros(n, dtype=torch.int32, device="cuda")
             info = bfa.lu_factor(dA, n, dp) if f64 else bfa.lu_factor_c(dA, n, dp)
             assert info == zc + 1, (n, f64, info)
+
+
+@pytest.mark.parametrize("f64", [False, True])
+def test_lu_tall_matrix_backward_error(bfa, torch_cuda, f64):
+    """n = 9,000 through the CU-partitioned look-ahead (panel workgroups of
+    ~141 rows on each of the 64 panel CUs); the solve's normwise backward
+    error ||A x - b|| / (||A|| ||x||) is computed in complex128 on the
+    device."""
+    torch = torch_cuda
+    n, nrhs = 9000, 4
+    gen = torch.Generator(device="cuda:0").manual_seed(9000)
+    A = torch.randn((n, n), dtype=torch.complex128, device="cuda:0", generator=gen)   # A[c, r]: column-major
+    A += n ** 0.5 * torch.eye(n, dtype=torch.complex128, device="cuda:0")
+    B = torch.randn((nrhs, n), dtype=torch.complex128, device="cuda:0", generator=gen)
+    dt = torch.complex128 if f64 else torch.complex64
+    LU = A.to(dt).clone()
+    X = B.to(dt).clone()
+    piv = torch.zeros(n, dtype=torch.int32, device="cuda:0")
+    if f64:
+        assert bfa.lu_factor(LU, n, piv) == 0
+        bfa.lu_solve(LU, n, piv, X, nrhs)
+    else:
+        assert bfa.lu_factor_c(LU, n, piv) == 0
+        bfa.lu_solve_c(LU, n, piv, X, nrhs)
+    Am = A.T.contiguous()                             # the matrix, row-major
+    Xm = X.to(torch.complex128).T.contiguous()
+    R = Am @ Xm - B.T
+    eta = (torch.linalg.norm(R) / (torch.linalg.norm(A) * torch.linalg.norm(Xm))).item()
+    print(f"[lu] n={n} {'z' if f64 else 'c'} backward error {eta:.2e}", flush=True)
+    assert eta < (1e-15 if f64 else 1e-6)
