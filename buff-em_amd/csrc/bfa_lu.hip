@@ -981,7 +981,7 @@ struct LUWork {
   // CU-partitioned look-ahead: the panel on mask_cus CUs, the updates on the rest
   int mask_cus = -1;             // -1 undecided, 0 off
   hipStream_t s_pan = nullptr, s_bulk = nullptr;
-  hipEvent_t evm[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t evm[6] = {};
 };
 static LUWork g_lu[16];
 
@@ -1132,18 +1132,38 @@ static int lu_mask_setup(LUWork *lw)
   return P;
 }
 
+// Columns of the bulk update that the panel stream takes after its panel,
+// so that both streams finish a step together: panel time t_p(m) plus X
+// columns at the panel CUs' GEMM rate equals the other columns at the update
+// CUs' rate.  Rates: rocBLAS GEMM on 192 CUs at ~100 (complex64) and ~53
+// (complex128) TF/s, a third of that on 64; panel recursion of 256 columns
+// ~1.5 ms + 65 ns per row (kernel trace, N = 19,872).  0 when the panel is
+// the longer stream; a multiple of 128, and the next block column stays with
+// the update stream.
+template <typename T> static int bulk_split(int m, int rest, int jb, int nb_next, int P, int ncu)
+{
+  const double rate_b = (sizeof(T) == 4 ? 100e12 : 53e12) * (ncu - P) / 192.0, rate_p = rate_b * P / (ncu - P);
+  const double c = 8.0 * m * jb, t_pan = 1.5e-3 + 65e-9 * m;
+  const double X = (rest * c / rate_b - t_pan) / (c / rate_p + c / rate_b);
+  int x = X > 0 ? (int)(X / 128) * 128 : 0;
+  x = std::min(x, rest - nb_next);
+  return x >= 256 ? x : 0;
+}
+
 template <typename T>
 static int lu_factor_masked(cplx<T> *A, int n, int64_t lda, int *ipiv, int *info, hipStream_t st, int nb, LUWork *lw)
 {
   const int P = lw->mask_cus;
   hipStream_t sp = lw->s_pan, sb = lw->s_bulk;
-  hipEvent_t ev_s = lw->evm[0], ev_p = lw->evm[1], ev_n = lw->evm[2], ev_e = lw->evm[3];
+  hipEvent_t ev_s = lw->evm[0], ev_p = lw->evm[1], ev_n = lw->evm[2], ev_e = lw->evm[3], ev_t = lw->evm[4],
+             ev_g = lw->evm[5];
   HIPCHECK(hipMemsetAsync(lw->pw.flags, 0, 8, st));
   HIPCHECK(hipEventRecord(ev_s, st));
   HIPCHECK(hipStreamWaitEvent(sp, ev_s, 0));
   HIPCHECK(hipStreamWaitEvent(sb, ev_s, 0));
   int rc = panel_rec<T>(A, n, std::min(nb, n), lda, ipiv, 0, lw, sp, P);
   if (rc) return rc;
+  bool sliced = false;                           // the panel stream updated a column slice last step
   for (int j = 0; j < n; j += nb) {
     const int jb = std::min(nb, n - j);
     cplx<T> *Ajj = A + (size_t)j * lda + j;
@@ -1161,14 +1181,27 @@ static int lu_factor_masked(cplx<T> *A, int n, int64_t lda, int *ipiv, int *info
     HIPCHECK(hipEventRecord(ev_n, sb));
     HIPCHECK(hipStreamWaitEvent(sp, ev_n, 0));
     const int rest = n - next - nb2;
+    int x = 0;
     if (rest > 0) {
       cplx<T> *Arest = A + (size_t)(next + nb2) * lda;
+      // last step's slice, updated on the panel stream, is swapped again here
+      if (sliced) HIPCHECK(hipStreamWaitEvent(sb, ev_g, 0));
       laswp<T>(A, lda, next + nb2, n, j, jb, ipiv, sb);
       trsm_lower<T>(jb, rest, Ajj, lda, Arest + j, lda, sb);
-      gemm_sub<T>(n - next, rest, jb, Ajj + jb, lda, Arest + j, lda, Arest + next, lda, sb, true);
+      x = bulk_split<T>(n - next, rest, jb, std::min(nb, rest), P, lw->num_cu);
+      if (x > 0) HIPCHECK(hipEventRecord(ev_t, sb));
+      gemm_sub<T>(n - next, rest - x, jb, Ajj + jb, lda, Arest + j, lda, Arest + next, lda, sb, true);
     }
     rc = panel_rec<T>(Anext + next, n - next, nb2, lda, ipiv + next, next, lw, sp, P);
     if (rc) return rc;
+    if (x > 0) {
+      // the right-most x columns of the bulk update, on the panel CUs
+      cplx<T> *As = A + (size_t)(n - x) * lda;
+      HIPCHECK(hipStreamWaitEvent(sp, ev_t, 0));
+      gemm_sub<T>(n - next, x, jb, Ajj + jb, lda, As + j, lda, As + next, lda, sp, true);
+      HIPCHECK(hipEventRecord(ev_g, sp));
+    }
+    sliced = x > 0;
   }
   HIPCHECK(hipEventRecord(ev_e, sb));
   HIPCHECK(hipStreamWaitEvent(st, ev_e, 0));

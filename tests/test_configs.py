@@ -10,8 +10,9 @@
   integrated in FP32 the error is reported (it is set by the FP32 cubature,
   not by the solver) and bounded loosely.
 - An exactly singular matrix reports the LAPACK-style first zero pivot.
-- A larger matrix (n = 9,000) factors through the CU-partitioned
-  look-ahead with a backward error at the precision's level.
+- A larger matrix (n = 14,000) factors through the CU-partitioned
+  look-ahead, column slices of the bulk update included, with a backward
+  error at the precision's level.
 """
 import numpy as np
 import pytest
@@ -130,13 +131,14 @@ def test_lu_singular_reports_first_zero_pivot(bfa, torch_cuda):
 
 @pytest.mark.parametrize("f64", [False, True])
 def test_lu_tall_matrix_backward_error(bfa, torch_cuda, f64):
-    """n = 9,000 through the CU-partitioned look-ahead (panel workgroups of
-    ~141 rows on each of the 64 panel CUs); the solve's normwise backward
-    error ||A x - b|| / (||A|| ||x||) is computed in complex128 on the
-    device."""
+    """n = 14,000 through the CU-partitioned look-ahead: panel workgroups of
+    ~219 rows on each of the 64 panel CUs, and in the early steps the panel
+    stream also updates a slice of the trailing columns; the solve's normwise
+    backward error ||A x - b|| / (||A|| ||x||) is computed in complex128 on
+    the device."""
     torch = torch_cuda
-    n, nrhs = 9000, 4
-    gen = torch.Generator(device="cuda:0").manual_seed(9000)
+    n, nrhs = 14000, 4
+    gen = torch.Generator(device="cuda:0").manual_seed(14000)
     A = torch.randn((n, n), dtype=torch.complex128, device="cuda:0", generator=gen)   # A[c, r]: column-major
     A += n ** 0.5 * torch.eye(n, dtype=torch.complex128, device="cuda:0")
     B = torch.randn((nrhs, n), dtype=torch.complex128, device="cuda:0", generator=gen)
