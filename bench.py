@@ -28,30 +28,11 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
-# HBM bytes per launch of the roofline kernel at N = 19,872, measured by two
-# rocprofv3 --pmc passes (FETCH_SIZE x2, WRITE_SIZE x1; the scale factors are
-# read off a streaming kernel of known size in the same run)
-MEASURED_TRAFFIC = {("rocblas_cgemm", 19872): 1.972e9}
-
-
-def trailing_gemm_schedule(N, nb=256, elem=8):
-    """(launches, algorithmic bytes, flops) of the 128x128-tile trailing-update
-    GEMMs of the look-ahead LU: per block step the next block column and the
-    bulk rest, each C(m x n) -= A(m x nb) B(nb x n) with m, n >= 256."""
-    cnt, byt, flo = 0, 0.0, 0.0
-    for j in range(0, N, nb):
-        nxt = min(j + nb, N)
-        if nxt >= N:
-            break
-        nb2 = min(nb, N - nxt)
-        m = N - nxt
-        for n in (nb2, N - nxt - nb2):
-            if m >= 256 and n >= 256:
-                k = min(nb, N - j)
-                byt += elem * (m * k + k * n + m * n) + elem * m * n
-                flo += 8.0 * m * n * k
-                cnt += 1
-    return cnt, byt, flo
+# HBM bytes per launch of the roofline kernel (its launches on the LU's update
+# CUs) at N = 19,872, measured by two rocprofv3 --pmc passes (FETCH_SIZE x2,
+# WRITE_SIZE x1; the scale factors are read off a streaming kernel of known
+# size in the same run)
+MEASURED_TRAFFIC = {("rocblas_cgemm", 19872): 1.788e9}
 
 
 def load_bfa():
@@ -107,7 +88,8 @@ def bench_graph_form(args, bfa, g, mat, pw, omega, log):
     tm = b.timing()
     rp = b.run([omega], use_graph=True, with_power=True)
     out = {"s_per_freq": t / args.steps, "phases_s": {"device": tm["solve_path_ms"] / 1e3 / args.steps},
-           "gemm": (0.0, 0.0, 0), "gemm_big": (0.0, 0.0, 0), "P": rp["power"][0, 0].copy(),
+           "gemm": (0.0, 0.0, 0), "gemm_big": (0.0, 0.0, 0), "gemm_slice": (0.0, 0.0, 0),
+           "P": rp["power"][0, 0].copy(),
            "refine": {"iters": int(r["iters"].max()), "resid": float(r["resid"].max())}}
     log(f"graph: {out['s_per_freq']:.4f} s/frequency  device {out['phases_s']['device']:.4f} s  refine {out['refine']}")
     del b
@@ -308,12 +290,13 @@ def main():
         t_total = time.perf_counter() - t0
         bfa.profile_enable(False)
         prof_big = bfa.profile_read(2 if form == "fp64" else 3)
+        prof_slice = bfa.profile_read(4 if form == "fp64" else 5)
         prof = bfa.profile_read(0 if form == "fp64" else 1)
         for k in phase:
             phase[k] /= args.steps
         P = g.emt_power(omega, pw, dB)
         results[form] = {"s_per_freq": t_total / args.steps, "phases_s": dict(phase), "gemm": prof,
-                         "gemm_big": prof_big,
+                         "gemm_big": prof_big, "gemm_slice": prof_slice,
                          "P": P.copy(), "refine": dict(refine) if form == "mixed" else None}
         log(f"{form}: {t_total / args.steps:.4f} s/frequency  phases {phase}  refine {refine if form == 'mixed' else ''}")
 
@@ -338,19 +321,25 @@ def main():
     all_ms, all_flops, all_launches = rk["gemm"]
     peak = PEAK_FP64_TFLOPS if best == "fp64" else PEAK_FP32_TFLOPS
     gemm_tflops = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else None
+    sl_ms, sl_flops, sl_launches = rk["gemm_slice"]
     roofline = {
-        "kernel": ("rocBLAS zgemm (LU trailing update, complex128 on f64 MFMA, 192 of 256 CUs)"
+        "kernel": ("rocBLAS zgemm (LU trailing update, complex128 on f64 MFMA, on the 192 update CUs)"
                    if best == "fp64" else
-                   "rocBLAS cgemm (LU trailing update, complex64 on f32 MFMA, 192 of 256 CUs)"),
+                   "rocBLAS cgemm (LU trailing update, complex64 on f32 MFMA, on the 192 update CUs)"),
         "bound": "mfma", "unit": "TFLOP/s",
         "achieved": gemm_tflops, "peak": peak,
         "frac": gemm_tflops / peak if gemm_tflops else None,
+        # HBM bytes per launch on the update CUs (rocprofv3 --pmc, split by queue)
         "traffic": MEASURED_TRAFFIC.get(("rocblas_cgemm" if best != "fp64" else "rocblas_zgemm", N)),
-        "algorithmic_bytes_per_launch": (lambda c: c[1] / c[0] if c[0] else None)(
-            trailing_gemm_schedule(N, elem=16 if best == "fp64" else 8)),
+        # the peak is the whole chip's; these launches own 192 of its 256 CUs
+        "cu_share": 192 / 256,
+        "frac_of_cu_share": gemm_tflops / peak / (192 / 256) if gemm_tflops else None,
         "per_launch_flops": gemm_flops / max(gemm_launches, 1),
         "avg_launch_ms": gemm_ms / max(gemm_launches, 1),
         "launches_per_step": gemm_launches / args.steps,
+        # column slices of the bulk update run on the 64 panel CUs beside
+        "panel_cu_slices": {"launches_per_step": sl_launches / args.steps,
+                            "tflops": sl_flops / (sl_ms * 1e-3) / 1e12 if sl_ms > 0 else None},
         "all_gemm_launches_tflops": all_flops / (all_ms * 1e-3) / 1e12 if all_ms > 0 else None,
         "lu_tflops_effective": (lu_flops / rk["phases_s"]["factor"] / 1e12
                                 if rk["phases_s"].get("factor", 0) > 0 else None),

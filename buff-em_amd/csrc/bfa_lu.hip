@@ -296,10 +296,11 @@ __global__ __launch_bounds__(512) void k_gemm_sub2(int m, int n, int k, const cp
 struct GemmProfile {
   bool on = false;
   std::vector<hipEvent_t> pool;
-  // [type * 2 + variant]: type 0 complex128 / 1 complex64; variant 0 the
-  // 64x64 kernel (k_gemm_sub), 1 the 128x128 kernel (k_gemm_sub2)
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> rec[4];
-  double flops[4] = {0, 0, 0, 0};
+  // [type * 3 + class]: type 0 complex128 / 1 complex64; class 0 the 64x64
+  // kernel (k_gemm_sub), 1 the large updates (k_gemm_sub2 or rocBLAS),
+  // 2 large updates run as column slices on the LU's panel CUs
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> rec[6];
+  double flops[6] = {0, 0, 0, 0, 0, 0};
   size_t used = 0;
 };
 static GemmProfile g_prof;
@@ -365,7 +366,7 @@ static bool blas_gemm_sub(rocblas_handle h, int m, int n, int k, const cplx<floa
 
 template <typename T>
 static int gemm_sub(int m, int n, int k, const cplx<T> *A, int64_t lda, const cplx<T> *B, int64_t ldb, cplx<T> *C,
-                    int64_t ldc, hipStream_t st, bool lib_ok = false)
+                    int64_t ldc, hipStream_t st, bool lib_ok = false, bool slice = false)
 {
   if (m <= 0 || n <= 0 || k <= 0) return BFA_OK;
   dim3 grid((m + GBM - 1) / GBM, (n + GBN - 1) / GBN);
@@ -384,8 +385,9 @@ static int gemm_sub(int m, int n, int k, const cplx<T> *A, int64_t lda, const cp
   }
   if (g_prof.on) {
     prof_begin(st, e1);
-    g_prof.rec[2 * p + (big ? 1 : 0)].push_back({e0, e1});
-    g_prof.flops[2 * p + (big ? 1 : 0)] += 8.0 * m * (double)n * k;
+    const int cls = big ? (slice ? 2 : 1) : 0;
+    g_prof.rec[3 * p + cls].push_back({e0, e1});
+    g_prof.flops[3 * p + cls] += 8.0 * m * (double)n * k;
   }
   return BFA_OK;
 }
@@ -451,7 +453,8 @@ static void trsm_diag(int kb, int n, const cplx<T> *L, int64_t ldl, cplx<T> *B, 
 }
 
 template <typename T> static int gemm_sub(int m, int n, int k, const cplx<T> *A, int64_t lda, const cplx<T> *B,
-                                          int64_t ldb, cplx<T> *C, int64_t ldc, hipStream_t st, bool lib_ok);
+                                          int64_t ldb, cplx<T> *C, int64_t ldc, hipStream_t st, bool lib_ok,
+                                          bool slice);
 
 // B(kb x n) := L^-1 B or U^-1 B for any kb, blocked by TRSM_KB
 template <typename T, bool LOWER>
@@ -1198,7 +1201,7 @@ static int lu_factor_masked(cplx<T> *A, int n, int64_t lda, int *ipiv, int *info
       // the right-most x columns of the bulk update, on the panel CUs
       cplx<T> *As = A + (size_t)(n - x) * lda;
       HIPCHECK(hipStreamWaitEvent(sp, ev_t, 0));
-      gemm_sub<T>(n - next, x, jb, Ajj + jb, lda, As + j, lda, As + next, lda, sp, true);
+      gemm_sub<T>(n - next, x, jb, Ajj + jb, lda, As + j, lda, As + next, lda, sp, true, true);
       HIPCHECK(hipEventRecord(ev_g, sp));
     }
     sliced = x > 0;
@@ -1852,10 +1855,12 @@ extern "C" int bfa_profile_enable(int on)
 extern "C" int bfa_profile_read(int which, double *ms, double *flops, int *launches)
 {
   // which: 0 / 1 = every complex128 / complex64 trailing-update launch;
-  // 2 / 3 = only the 128x128-tile kernel (k_gemm_sub2) of that type
-  if (which < 0 || which > 3) return fail(BFA_ERR_ARG, "bfa_profile_read: which must be 0..3");
+  // 2 / 3 = the large updates (m, n >= 256) on all or the update CUs;
+  // 4 / 5 = the large updates run as column slices on the panel CUs
+  if (which < 0 || which > 5) return fail(BFA_ERR_ARG, "bfa_profile_read: which must be 0..5");
   const int type = which & 1;
-  const int lo = which < 2 ? 2 * type : 2 * type + 1, hi = 2 * type + 1;
+  const int lo = which < 2 ? 3 * type : which < 4 ? 3 * type + 1 : 3 * type + 2;
+  const int hi = which < 2 ? 3 * type + 2 : lo;
   double t = 0.0, f = 0.0;
   int n = 0;
   for (int r = lo; r <= hi; r++) {
@@ -1871,9 +1876,9 @@ extern "C" int bfa_profile_read(int which, double *ms, double *flops, int *launc
   if (ms) *ms = t;
   if (flops) *flops = f;
   if (launches) *launches = n;
-  // reading 0 / 1 clears the records of that type (both kernels); 2 / 3 keep them
+  // reading 0 / 1 clears the records of that type (every class); 2..5 keep them
   if (which >= 2) return BFA_OK;
-  for (int r = 2 * type; r <= 2 * type + 1; r++) {
+  for (int r = 3 * type; r <= 3 * type + 2; r++) {
     g_prof.rec[r].clear();
     g_prof.flops[r] = 0.0;
   }

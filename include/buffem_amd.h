@@ -336,8 +336,10 @@ int  bfa_surr_params(bfa_surrogate *s, float *out, const float *in);
 /* When enabled, every trailing-update GEMM launch is bracketed by HIP
  * events; bfa_profile_read returns the summed kernel time (ms), the summed
  * algorithmic flops (8mnk per complex product) and the launch count for
- * which = 0 (complex128) or 1 (complex64), all launches, or 2 / 3 for only
- * the 128x128-tile kernel of that type; reading 0 / 1 resets that type.  */
+ * which = 0 (complex128) or 1 (complex64), all launches; 2 / 3 the large
+ * updates (m, n >= 256) on all CUs or, in the CU-partitioned LU, on the
+ * update CUs; 4 / 5 the large updates run as column slices on the panel
+ * CUs; reading 0 / 1 resets that type.                                    */
 int  bfa_profile_enable(int on);
 int  bfa_profile_read(int which, double *ms, double *flops, int *launches);
 
