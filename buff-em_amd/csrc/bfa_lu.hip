@@ -1431,6 +1431,11 @@ template <typename T> struct StreamShared {
 
 // Bin: right-hand sides (n x nrhs, read-only here); Xo: tag-filled output.
 // ticket: this sweep's counter; flag: set when a wait gives up.
+// x_i = D_i^-1 (b_i - sum over k != p of L_ik x_k) - (D_i^-1 L_ip) x_p, p the
+// block solved just before i: M = D_i^-1 L_ip is formed while the chain is
+// still upstream, everything but the M x_p term is finished before x_p
+// lands, and the step that waits on x_p is one 64 x 64 product between two
+// workgroup barriers.
 template <typename T, bool LOWER>
 __device__ __forceinline__ void trsv_stream_body(int n, int nrhs, const cplx<T> *__restrict__ A, int64_t lda,
                                                  const cplx<T> *__restrict__ inv, const cplx<T> *Bin, cplx<T> *Xo,
@@ -1449,10 +1454,34 @@ __device__ __forceinline__ void trsv_stream_body(int n, int nrhs, const cplx<T> 
   cplx<T> dcol[SV_KB / 4];
 #pragma unroll
   for (int q = 0; q < SV_KB / 4; q++) dcol[q] = Di[(size_t)(wv + 4 * q) * SV_KB + lane];
+  // M = D^-1 L_ip, this thread's row (lane) and the wave's columns: row tt of
+  // block (i, p) is held by lane tt of the wave, passed along by shuffles
+  const int kp = t > 0 ? (LOWER ? t - 1 : nblk - t) : 0;
+  const int cp0 = kp * SV_KB, cbp = t > 0 ? min(SV_KB, n - cp0) : 0;
+  cplx<T> mcol[SV_KB / 4];
+#pragma unroll
+  for (int q = 0; q < SV_KB / 4; q++) mcol[q] = {0, 0};
+  if (t > 0) {
+    cplx<T> lp[SV_KB / 4];
+#pragma unroll
+    for (int q = 0; q < SV_KB / 4; q++) {
+      const int c = wv + 4 * q;
+      lp[q] = (lane < kb && c < cbp) ? A[(size_t)(cp0 + c) * lda + r] : cplx<T>{0, 0};
+    }
+    for (int tt = 0; tt < SV_KB; tt++) {
+      const cplx<T> d = Di[(size_t)tt * SV_KB + lane];      // D^-1 (lane, tt)
+#pragma unroll
+      for (int q = 0; q < SV_KB / 4; q++) {
+        const T pr = __shfl(lp[q].re, tt, 64), pi = __shfl(lp[q].im, tt, 64);
+        mcol[q].re += d.re * pr - d.im * pi;
+        mcol[q].im += d.re * pi + d.im * pr;
+      }
+    }
+  }
   T ar[SV_RHS], ai[SV_RHS];
 #pragma unroll
   for (int j = 0; j < SV_RHS; j++) { ar[j] = 0; ai[j] = 0; }
-  for (int s = 0; s < t; s++) {
+  for (int s = 0; s < t - 1; s++) {
     const int k = LOWER ? s : nblk - 1 - s;
     const int c0 = k * SV_KB, cb = min(SV_KB, n - c0);
     // this wave's columns of block (bi, k), loaded before waiting on x_k
@@ -1491,7 +1520,7 @@ __device__ __forceinline__ void trsv_stream_body(int n, int nrhs, const cplx<T> 
   }
   for (int j = 0; j < nrhs; j++) sh.part[wv][j][lane] = {ar[j], ai[j]};
   __syncthreads();
-  // raw right-hand side of this block: b - sum of updates
+  // raw right-hand side of this block without x_p's term: b - sum of updates
   if (wv == 0) {
     for (int j = 0; j < nrhs; j++) {
       cplx<T> v = {0, 0};
@@ -1504,7 +1533,7 @@ __device__ __forceinline__ void trsv_stream_body(int n, int nrhs, const cplx<T> 
     }
   }
   __syncthreads();
-  // x = inv(D) * raw, D the diagonal block
+  // z = inv(D) * raw, D the diagonal block
 #pragma unroll
   for (int j = 0; j < SV_RHS; j++) { ar[j] = 0; ai[j] = 0; }
 #pragma unroll
@@ -1522,12 +1551,54 @@ __device__ __forceinline__ void trsv_stream_body(int n, int nrhs, const cplx<T> 
   }
   for (int j = 0; j < nrhs; j++) sh.part[wv][j][lane] = {ar[j], ai[j]};
   __syncthreads();
-  if (wv == 0 && lane < kb) {
-    for (int j = 0; j < nrhs; j++) {
-      const T xr = sh.part[0][j][lane].re + sh.part[1][j][lane].re + sh.part[2][j][lane].re + sh.part[3][j][lane].re;
-      const T xi = sh.part[0][j][lane].im + sh.part[1][j][lane].im + sh.part[2][j][lane].im + sh.part[3][j][lane].im;
-      sv_store(&Xo[(size_t)j * ldx + r], cplx<T>{xr, xi});
+  T zr[SV_RHS], zi[SV_RHS];
+#pragma unroll
+  for (int j = 0; j < SV_RHS; j++) {
+    zr[j] = zi[j] = 0;
+    if (wv == 0 && j < nrhs) {
+      zr[j] = sh.part[0][j][lane].re + sh.part[1][j][lane].re + sh.part[2][j][lane].re + sh.part[3][j][lane].re;
+      zi[j] = sh.part[0][j][lane].im + sh.part[1][j][lane].im + sh.part[2][j][lane].im + sh.part[3][j][lane].im;
     }
+  }
+  if (t > 0) {
+    // wave 0 polls x_p (xs is free: every wave read it before the last barrier)
+    if (wv == 0) {
+      unsigned spins = 0;
+      for (int idx = lane; idx < cbp * nrhs; idx += 64) {
+        const int c = idx % cbp, j = idx / cbp;
+        cplx<T> v;
+        while (!sv_load(&Xo[(size_t)j * ldx + cp0 + c], v)) {
+          if (++spins > SPIN_LIMIT) { atomicOr(flag, 1u); break; }
+        }
+        sh.xs[j][c] = v;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < SV_RHS; j++) { ar[j] = 0; ai[j] = 0; }
+#pragma unroll
+    for (int q = 0; q < SV_KB / 4; q++) {
+      const int c = wv + 4 * q;
+#pragma unroll
+      for (int j = 0; j < SV_RHS; j++) {
+        if (j < nrhs) {
+          const cplx<T> x = sh.xs[j][c < cbp ? c : 0];
+          ar[j] += mcol[q].re * x.re - mcol[q].im * x.im;
+          ai[j] += mcol[q].re * x.im + mcol[q].im * x.re;
+        }
+      }
+    }
+    for (int j = 0; j < nrhs; j++) sh.part[wv][j][lane] = {ar[j], ai[j]};
+    __syncthreads();
+    if (wv == 0 && lane < kb) {
+      for (int j = 0; j < nrhs; j++) {
+        const T mr = sh.part[0][j][lane].re + sh.part[1][j][lane].re + sh.part[2][j][lane].re + sh.part[3][j][lane].re;
+        const T mi = sh.part[0][j][lane].im + sh.part[1][j][lane].im + sh.part[2][j][lane].im + sh.part[3][j][lane].im;
+        sv_store(&Xo[(size_t)j * ldx + r], cplx<T>{zr[j] - mr, zi[j] - mi});
+      }
+    }
+  } else if (wv == 0 && lane < kb) {
+    for (int j = 0; j < nrhs; j++) sv_store(&Xo[(size_t)j * ldx + r], cplx<T>{zr[j], zi[j]});
   }
 }
 
