@@ -4,8 +4,8 @@
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the
  * reference's algorithm (HomerReid/buff-em, src/libs/libbuff and
  * src/libs/libTTaylorDuffy), used by tests/, __graft_entry__.smoke() and
- * bench.py's cpu_baseline leg as the checker.  No product code links or
- * calls it.
+ * bench.py's cpu_baseline leg, as what results are compared against.  No
+ * product code links or calls it.
  *
  * Every function cites the reference file:line it restates.  Parity pins
  * (closed forms, known-answer tests, brute force) live in tests/.

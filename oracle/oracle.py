@@ -1,7 +1,8 @@
 """Python driver of the CPU oracle.
 
 TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg as the checker, never by the product path.
+bench.py's cpu_baseline leg, as what results are compared against, never by
+the product path.
 
 Wraps liboracle.so (the C restatement of the reference algorithm, see
 buffem_oracle.h) and adds the dense solve through LAPACK (scipy's zgetrf /
