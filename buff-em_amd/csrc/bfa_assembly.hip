@@ -646,6 +646,17 @@ struct FarTpShared {
 
 // 4-point product-rule moments of one tetrahedron pair (signed weights);
 // out[i * stride] for i < 16
+// 1/sqrt(x): the hardware estimate (v_rsq_f64, ~23 correct bits) refined by
+// two Newton steps y <- y + y/2 (1 - x y^2) to full double precision;
+// replaces a correctly rounded sqrt and a division in the pair loop
+__device__ __forceinline__ double rsqrt_nr(double x)
+{
+  double y = __builtin_amdgcn_rsq(x);
+  y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+  y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+  return y;
+}
+
 template <bool REALK>
 __device__ __forceinline__ void tp_moments(const double *xs, const double *cS, const double *ys, const double *cT,
                                            double kre, double kim, double *out, int stride)
@@ -661,10 +672,10 @@ __device__ __forceinline__ void tp_moments(const double *xs, const double *cS, c
     for (int q = 0; q < 4; q++) {
       const double *y = ys + 4 * q;
       const double R0 = x[0] - y[0], R1 = x[1] - y[1], R2 = x[2] - y[2];
-      const double r = sqrt(R0 * R0 + R1 * R1 + R2 * R2);
+      const double r2 = R0 * R0 + R1 * R1 + R2 * R2, ir = rsqrt_nr(r2), r = r2 * ir;
       double sn, co;
       sincos(kre * r, &sn, &co);
-      double amp = x[3] * y[3] / r;
+      double amp = x[3] * y[3] * ir;
       if (!REALK) amp *= exp(-kim * r);
       const double gr = amp * co, gi = amp * sn;
       const double e0 = y[0] - cT[0], e1 = y[1] - cT[1], e2 = y[2] - cT[2];
