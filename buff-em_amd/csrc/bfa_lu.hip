@@ -872,12 +872,18 @@ __device__ __forceinline__ void panel_body(cplx<T> *A, int m, int w, int64_t lda
 #undef RW
 }
 
+// The single-matrix panels alternate between two sets of barrier counters
+// and keys: launch i uses one (cleared by launch i - 1, which completed
+// before it on the same stream) and its workgroup 0 clears the other for
+// launch i + 1, in place of a separate clearing launch per panel.
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel(cplx<T> *A, int m, int w, int64_t lda, int *ipiv, int rpw, PanelWork pw,
-                                               int piv_off)
+                                               int piv_off, unsigned *clear)
 {
   extern __shared__ unsigned char smem_raw[];
   __shared__ PanelShared<T> sh;
+  if (blockIdx.x == 0)
+    for (int i = threadIdx.x; i < PB_WORDS; i += blockDim.x) st_agent(&clear[i], 0u);
   panel_body<T>(A, m, w, lda, ipiv, rpw, pw, piv_off, blockIdx.x, gridDim.x, (cplx<T> *)smem_raw, sh);
 }
 
@@ -979,6 +985,8 @@ struct LUWork {
   PanelWork pw{};
   void *mem = nullptr;
   int num_cu = 0;
+  unsigned *bars = nullptr;      // two sets of panel barrier counters + keys
+  unsigned long long npanel = 0; // panel launches so far (selects the set)
   hipStream_t side = nullptr;    // bulk trailing updates of the look-ahead LU
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   // CU-partitioned look-ahead: the panel on mask_cus CUs, the updates on the rest
@@ -999,15 +1007,16 @@ static int lu_work(LUWork *&w)
   HIPCHECK(hipGetDeviceProperties(&prop, dev));
   w->num_cu = prop.multiProcessorCount;
   w->maxwg = 1024;
-  const size_t bytes = 2 * (size_t)w->maxwg * PW * 16 + 2 * PW * 16 + PB_WORDS * 4 + 64;
+  const size_t bytes = 2 * (size_t)w->maxwg * PW * 16 + 2 * PW * 16 + 2 * PB_WORDS * 4 + 64;
   HIPCHECK(hipMalloc(&w->mem, bytes));
   HIPCHECK(hipMemset(w->mem, 0, bytes));
   char *p = (char *)w->mem;
   w->pw.rowdata = p; p += 2 * (size_t)w->maxwg * PW * 16;
   w->pw.rowj = p; p += 2 * PW * 16;
-  w->pw.bar = (unsigned *)p;
+  w->bars = (unsigned *)p;                       // two sets of PB_WORDS, both zero
+  w->pw.bar = w->bars;
   w->pw.key = (unsigned long long *)(w->pw.bar + PB_LANES * PB_BAR_STRIDE);
-  p += PB_WORDS * 4;
+  p += 2 * PB_WORDS * 4;
   w->pw.flags = (int *)p;
   w->device = dev;
   return BFA_OK;
@@ -1071,9 +1080,18 @@ static int panel_base(cplx<T> *A, int m, int w, int64_t lda, int *ipiv, int piv_
   }
   if (nwg > pc * cus)
     return fail(BFA_ERR_HIP, "panel needs %d co-resident workgroups, %d CUs fit %d", nwg, cus, pc * cus);
-  hipLaunchKernelGGL(k_zero_u32, dim3((PB_WORDS + 255) / 256), dim3(256), 0, st, lw->pw.bar, PB_WORDS);   // barrier and keys
-  hipLaunchKernelGGL(k_panel<T>, dim3(nwg), dim3(256), lds, st, A, m, w, lda, ipiv, rpw, lw->pw, piv_off);
-  HIPCHECK(hipGetLastError());
+  const int set = (int)(lw->npanel++ & 1);
+  PanelWork pw = lw->pw;
+  pw.bar = lw->bars + set * PB_WORDS;
+  pw.key = (unsigned long long *)(pw.bar + PB_LANES * PB_BAR_STRIDE);
+  hipLaunchKernelGGL(k_panel<T>, dim3(nwg), dim3(256), lds, st, A, m, w, lda, ipiv, rpw, pw, piv_off,
+                     lw->bars + (set ^ 1) * PB_WORDS);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    // the launch that would have cleared the other set did not run
+    (void)hipMemset(lw->bars, 0, 2 * PB_WORDS * sizeof(unsigned));
+    return fail(BFA_ERR_HIP, "panel launch: %s", hipGetErrorString(e));
+  }
   return BFA_OK;
 }
 
