@@ -874,6 +874,17 @@ __global__ __launch_bounds__(256) void k_far32(DevGeom g, double kre_d, double k
 
 // ---------------------------------------------------------------------------
 // near field + static record + Gram, one thread per touching pair
+// 1/n for the series below, correctly rounded at compile time (the same
+// values a division gives, without a division per term)
+struct InvIntTable {
+  double v[100];
+  constexpr InvIntTable() : v()
+  {
+    for (int i = 1; i < 100; i++) v[i] = 1.0 / i;
+  }
+};
+__constant__ InvIntTable c_inv_int = InvIntTable();
+
 __device__ __forceinline__ cd exprel3(cd x)
 {
   // GetExpRelTable(x, 3, 3): x^3/3! + sum_{n>=4} x^n/n!, relative 1e-8 break
@@ -883,7 +894,7 @@ __device__ __forceinline__ cd exprel3(cd x)
   term = cscale(cmul(term, x), 1.0 / 3.0);
   cd first = term, sum = {0.0, 0.0};
   for (int n = 4; n < 100; n++) {
-    term = cscale(cmul(term, x), 1.0 / n);
+    term = cscale(cmul(term, x), c_inv_int.v[n]);
     sum = cadd(sum, term);
     const double mt = term.re * term.re + term.im * term.im;
     const double ms = sum.re * sum.re + sum.im * sum.im;
@@ -1291,16 +1302,18 @@ __device__ __forceinline__ void near_entry(const DevGeom &g, double kre, double 
   double ap[8][7], bp[8][7];
   for (int pt = 0; pt < 8; pt++) { bf_points4(g, a, pt, ap[pt]); bf_points4(g, b, pt, bp[pt]); }
   cd acc = {0.0, 0.0};
+  const cd ik2inv = cdiv({1.0, 0.0}, k2);          // 1/k^2, once per pair
   for (int ip = 0; ip < 8; ip++)
     for (int jp = 0; jp < 8; jp++) {
       const double R0 = ap[ip][0] - bp[jp][0], R1 = ap[ip][1] - bp[jp][1], R2 = ap[ip][2] - bp[jp][2];
-      const double r = sqrt(R0 * R0 + R1 * R1 + R2 * R2);
-      if (r < 1.0e-12) continue;
+      const double r2 = R0 * R0 + R1 * R1 + R2 * R2;
+      if (r2 < 1.0e-24) continue;                  // r < 1e-12
+      const double ir = rsqrt_nr(r2), r = r2 * ir;
       const double dot = ap[ip][3] * bp[jp][3] + ap[ip][4] * bp[jp][4] + ap[ip][5] * bp[jp][5];
       const double dd = ap[ip][6] * bp[jp][6];
-      const cd pe = {dot - cdiv({dd, 0.0}, k2).re, -cdiv({dd, 0.0}, k2).im};
+      const cd pe = {dot - dd * ik2inv.re, -dd * ik2inv.im};
       const cd ef = exprel3(cscale(ik, r));
-      acc = cadd(acc, cmul(pe, cscale(ef, 1.0 / (FOURPI * r))));
+      acc = cadd(acc, cmul(pe, cscale(ef, ir * (1.0 / FOURPI))));
     }
   // static part (GMatrixElements.cc:415-419)
   const double *G = table + 6 * j;
