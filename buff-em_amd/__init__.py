@@ -95,6 +95,7 @@ _SIGS = {
     "bfa_solve_refine": (I, [P, I64, P, I64, P, I, P, I, I64, D, I, P, P, P]),
     "bfa_profile_enable": (I, [I]),
     "bfa_profile_read": (I, [I, P, P, P]),
+    "bfa_lu_partition": (I, [I]),
     "bfa_device_malloc": (I, [P, ctypes.c_size_t]),
     "bfa_device_free": (I, [P]),
     "bfa_memcpy_htod": (I, [P, P, ctypes.c_size_t, P]),
@@ -467,6 +468,11 @@ def cgemm_sub(m, n, k, d_A, lda, d_B, ldb, d_C, ldc, stream=None):
     """C -= A @ B on the device (column-major complex64)."""
     _check(lib().bfa_cgemm_sub(int(m), int(n), int(k), _ptr(d_A), int(lda), _ptr(d_B), int(ldb), _ptr(d_C),
                                int(ldc), _stream_ptr(stream)), "bfa_cgemm_sub")
+
+
+def lu_partition(on=True):
+    """CU-partitioned look-ahead LU on this device (default on); off keeps every stream on every CU."""
+    _check(lib().bfa_lu_partition(1 if on else 0), "bfa_lu_partition")
 
 
 def profile_enable(on=True):

@@ -991,6 +991,7 @@ struct LUWork {
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   // CU-partitioned look-ahead: the panel on mask_cus CUs, the updates on the rest
   int mask_cus = -1;             // -1 undecided, 0 off
+  bool mask_off = false;         // bfa_lu_partition(0): shared-CU path
   hipStream_t s_pan = nullptr, s_bulk = nullptr;
   hipEvent_t evm[6] = {};
 };
@@ -1261,7 +1262,7 @@ int lu_factor(cplx<T> *A, int n, int64_t lda, int *ipiv, int *info, hipStream_t 
   LUWork *lw = nullptr;
   int rc = lu_work(lw);
   if (rc) return rc;
-  if (lu_mask_setup(lw) > 0 && masked_fits<T>(n, nb, lw))
+  if (!lw->mask_off && lu_mask_setup(lw) > 0 && masked_fits<T>(n, nb, lw))
     return lu_factor_masked<T>(A, n, lda, ipiv, info, st, nb, lw);
   if (!lw->side) {
     HIPCHECK(hipStreamCreateWithFlags(&lw->side, hipStreamNonBlocking));
@@ -1932,6 +1933,15 @@ extern "C" int bfa_cgemm_sub(int m, int n, int k, const void *d_A, int64_t lda, 
     return fail(BFA_ERR_ARG, "bfa_cgemm_sub: bad sizes");
   bfa::gemm_sub_c(m, n, k, d_A, lda, d_B, ldb, d_C, ldc, (hipStream_t)stream);
   HIPCHECK(hipGetLastError());
+  return BFA_OK;
+}
+
+extern "C" int bfa_lu_partition(int on)
+{
+  bfa::LUWork *lw = nullptr;
+  int rc = bfa::lu_work(lw);
+  if (rc) return rc;
+  lw->mask_off = on == 0;
   return BFA_OK;
 }
 

@@ -196,6 +196,13 @@ int  bfa_lu_factor_z(void *d_A, int n, int64_t lda, int *d_ipiv, int *info,
 int  bfa_lu_solve_z(const void *d_A, int n, int64_t lda, const int *d_ipiv,
                     void *d_B, int nrhs, int64_t ldb, void *stream);
 
+/* LU execution on this device (no reference counterpart): on = 1 (default)
+ * runs the look-ahead LU on CU-partitioned streams (the panel on 64 CUs,
+ * the trailing updates on the rest; BFA_LU_MASK sets the count, 0 disables);
+ * on = 0 keeps every stream on every CU, for applications that partition
+ * the GPU themselves.                                                     */
+int  bfa_lu_partition(int on);
+
 /* ---- precision form (no reference counterpart) -------------------------- */
 /* complex128 -> complex64 copy of an m x n column-major matrix             */
 int  bfa_convert_z2c(const void *d_A64, int m, int n, int64_t lda, void *d_A32,

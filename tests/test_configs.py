@@ -12,7 +12,8 @@
 - An exactly singular matrix reports the LAPACK-style first zero pivot.
 - A larger matrix (n = 14,000) factors through the CU-partitioned
   look-ahead, column slices of the bulk update included, with a backward
-  error at the precision's level.
+  error at the precision's level; the shared-CU look-ahead gives the same
+  factors to rounding.
 """
 import numpy as np
 import pytest
@@ -158,3 +159,31 @@ def test_lu_tall_matrix_backward_error(bfa, torch_cuda, f64):
     eta = (torch.linalg.norm(R) / (torch.linalg.norm(A) * torch.linalg.norm(Xm))).item()
     print(f"[lu] n={n} {'z' if f64 else 'c'} backward error {eta:.2e}", flush=True)
     assert eta < (1e-15 if f64 else 1e-6)
+
+
+def test_lu_shared_cu_path(bfa, torch_cuda):
+    """bfa_lu_partition(0): the look-ahead LU with every stream on every CU
+    (the path for matrices whose panel does not fit the panel CUs) gives the
+    partitioned path's factors to rounding, and the partition comes back on."""
+    torch = torch_cuda
+    n = 3000
+    gen = torch.Generator(device="cuda:0").manual_seed(3000)
+    A = torch.randn((n, n), dtype=torch.complex128, device="cuda:0", generator=gen)
+    A += n ** 0.5 * torch.eye(n, dtype=torch.complex128, device="cuda:0")
+    out = {}
+    try:
+        for on in (False, True):
+            bfa.lu_partition(on)
+            for f64 in (True, False):
+                LU = A.to(torch.complex128 if f64 else torch.complex64).clone()
+                piv = torch.zeros(n, dtype=torch.int32, device="cuda:0")
+                assert (bfa.lu_factor if f64 else bfa.lu_factor_c)(LU, n, piv) == 0
+                out[(on, f64)] = (LU.to(torch.complex128), piv.clone())
+    finally:
+        bfa.lu_partition(True)
+    for f64, tol in ((True, 1e-12), (False, 1e-4)):
+        (a, pa), (b, pb) = out[(False, f64)], out[(True, f64)]
+        assert torch.equal(pa, pb)
+        err = (torch.linalg.norm(a - b) / torch.linalg.norm(b)).item()
+        print(f"[lu] shared-CU vs partitioned {'z' if f64 else 'c'}: {err:.1e}", flush=True)
+        assert err < tol
