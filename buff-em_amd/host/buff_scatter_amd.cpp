@@ -10,7 +10,12 @@
 //                    [--eps RE IM]  (material of --vmsh / --kuhn)
 //                    --omega W [--omega W ...]
 //                    --pw EX EY EZ NX NY NZ [--pw ...]
-// Output: one line per (omega, incident field): omega  ifield  P_abs  P_scat
+//                    [--rytov T_EXPR T_ENV]
+// Output: one line per (omega, incident field): omega  ifield  P_abs  P_scat.
+// With --rytov, per omega also buff-neq's flux of this object: the EMT power
+// of the dressed Rytov matrix (GetFlux.cc:309-399) for the temperature
+// expression T_EXPR (SVTensor syntax, e.g. "300 + 40*x") and environment
+// temperature T_ENV, as the line  omega  rytov  P_abs  P_scat.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -27,6 +32,8 @@ int main(int argc, char **argv)
   double R = 1.0, er = 4.0, ei = 0.1;
   std::vector<double> omegas;
   std::vector<PlaneWave> fields;
+  const char *texpr = nullptr;
+  double tenv = 0.0;
   for (int i = 1; i < argc; i++) {
     auto need = [&](int k) {
       if (i + k >= argc) { fprintf(stderr, "missing value for %s\n", argv[i]); exit(2); }
@@ -42,11 +49,15 @@ int main(int argc, char **argv)
       for (int c = 0; c < 3; c++) p.E0[c] = atof(argv[++i]);
       for (int c = 0; c < 3; c++) p.nHat[c] = atof(argv[++i]);
       fields.push_back(p);
+    } else if (!strcmp(argv[i], "--rytov")) {
+      need(2);
+      texpr = argv[++i];
+      tenv = atof(argv[++i]);
     } else { fprintf(stderr, "unknown option %s\n", argv[i]); return 2; }
   }
-  if ((!vmsh && !geo && kuhn_n <= 0) || omegas.empty() || fields.empty()) {
+  if ((!vmsh && !geo && kuhn_n <= 0) || omegas.empty() || (fields.empty() && !texpr)) {
     fprintf(stderr, "usage: %s (--geometry FILE.buffgeo | --vmsh FILE | --kuhn n R) [--eps RE IM] --omega W "
-                    "--pw EX EY EZ NX NY NZ\n", argv[0]);
+                    "--pw EX EY EZ NX NY NZ [--rytov T_EXPR T_ENV]\n", argv[0]);
     return 2;
   }
   try {
@@ -55,7 +66,16 @@ int main(int argc, char **argv)
                    : vmsh ? new SWGGeometry(vmsh, mat) : new SWGGeometry(kuhn_n, R, 0.05, 0x5EED, mat);
     HMatrix *M = G->AllocateVIEMatrix();
     HVector *J = G->AllocateRHSVector();
+    bfa_material T{};
+    if (texpr) check(bfa_material_expr(texpr, &T), "temperature expression");
+    HMatrix *DR = nullptr;
     for (double w : omegas) {
+      if (texpr) {
+        DR = G->ComputeDressedRytovMatrix(w, &T, tenv, DR);
+        std::vector<double> P = G->GetPFTMatrix(DR, w);
+        printf("%.6e rytov %.12e %.12e\n", w, P[0], P[1]);
+      }
+      if (fields.empty()) continue;
       M = G->AssembleVIEMatrix(w, M);
       int info = M->LUFactorize();
       if (info) { fprintf(stderr, "singular VIE matrix at omega=%g (info %d)\n", w, info); return 3; }
@@ -67,9 +87,11 @@ int main(int argc, char **argv)
         printf("%.6e %zu %.12e %.12e\n", w, f, P[0], P[1]);
       }
     }
+    delete DR;
     delete J;
     delete M;
     delete G;
+    if (texpr) bfa_material_release(&T);
   } catch (const Error &e) {
     fprintf(stderr, "error: %s\n", e.what());
     return 1;

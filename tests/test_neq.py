@@ -8,6 +8,8 @@ GPU:
   Im(eps-1) (eps-1) V^-1 (RelDeltaTheta = 1, also without a temperature
   field, GetOverlapIntegrand's default); a temperature gradient matches
   a numpy restatement of the Rytov integrand.
+- The C++ drop-in's buff-neq calls (buff_scatter_amd --rytov) reproduce the
+  ctypes path's EMT power of the dressed Rytov matrix.
 - LUSolve(HMatrix*) with many right-hand sides (the blocked TRSM + GEMM
   form) against numpy (complex128) and LAPACK cgetrf/cgetrs (complex64).
 """
@@ -267,3 +269,32 @@ def test_emt_power_dr_is_the_j_path_for_a_rank_one_dr(bfa, torch_cuda, geo):
     p = g.emt_power_dr(w, dDR)
     print(f"[neq] EMT(J J^H) {p[1]:.6e} J path {pj[0][1]:.6e}", flush=True)
     assert abs(p[1] - pj[0][1]) <= 1e-12 * abs(pj[0][1])
+
+
+def test_cpp_driver_rytov_flux(bfa, torch_cuda, geo):
+    """buff_scatter_amd --rytov (the C++ drop-in: ComputeDressedRytovMatrix,
+    GetPFTMatrix(DR)) gives the ctypes path's EMT power of the dressed Rytov
+    matrix on the same mesh and material."""
+    import os
+    import subprocess
+    from conftest import ROOT
+    V, T, g, o = geo
+    w, eps, texpr, t_env = 0.8, 4.0 + 0.6j, "300 + 40*x - 15*z", 280.0
+    drv = os.path.join(ROOT, "buff-em_amd", "lib", "buff_scatter_amd")
+    print("[neq] driver --rytov n=3", flush=True)
+    r = subprocess.run([drv, "--kuhn", "3", "1.0", "--eps", str(eps.real), str(eps.imag), "--omega", str(w),
+                        "--rytov", texpr, str(t_env)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    line = [l.split() for l in r.stdout.strip().splitlines() if l.split()[1] == "rytov"]
+    assert len(line) == 1
+    got = np.array([float(line[0][2]), float(line[0][3])])
+    # the drop-in builds its static table at the default Taylor-Duffy order (12)
+    g12 = bfa.Geometry(V, T)
+    g12.upload()
+    g12.build_static(12)
+    Sinv, DR, work = _neq_bufs(torch_cuda, g12)
+    g12.dressed_rytov(w, bfa.material("iso", eps=eps), Sinv, DR, work, temperature=bfa.material_expr(texpr),
+                      t_env=t_env)
+    want = g12.emt_power_dr(w, DR)
+    print(f"[neq] driver {got} ctypes {want}", flush=True)
+    assert np.all(np.abs(got - want) <= 1e-9 * np.abs(want))
