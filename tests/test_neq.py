@@ -8,6 +8,11 @@ GPU:
   Im(eps-1) (eps-1) V^-1 (RelDeltaTheta = 1, also without a temperature
   field, GetOverlapIntegrand's default); a temperature gradient matches
   a numpy restatement of the Rytov integrand.
+- S^-1 equals the inverse of the oracle's Gram matrix; the dressed Rytov
+  matrix W (S^-1 R S^-1)^H equals the oracle's step-by-step restatement
+  (constant eps with every input from the oracle; sincos eps with a
+  temperature gradient); its EMT power equals the oracle's, and for
+  DR = J J^H the J-vector EMT power.
 - The C++ drop-in's buff-neq calls (buff_scatter_amd --rytov) reproduce the
   ctypes path's EMT power of the dressed Rytov matrix.
 - LUSolve(HMatrix*) with many right-hand sides (the blocked TRSM + GEMM
