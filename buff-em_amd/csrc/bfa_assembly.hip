@@ -1895,15 +1895,17 @@ __device__ __forceinline__ int find_inst(const AsmInst *__restrict__ d, int n, i
   return lo;
 }
 
+// REALK: the host guarantees omega_im == 0 for every frequency replayed
+// through this launch (one kernel holding both variants spills 592 B/lane,
+// the real-k one alone 144 B/lane, as the single-matrix k_far_tp<true>)
+template <bool REALK>
 __global__ __launch_bounds__(FT_THREADS) void kg_far(const AsmInst *__restrict__ d, int ni, const FreqState *fs)
 {
   __shared__ FarTpShared sh;
   const AsmInst &q = d[find_inst(d, ni, blockIdx.x, GF_FAR)];
   int ti, tj;
   tile_coords(blockIdx.x - q.far0, q.ntiles, ti, tj);
-  const double kre = fs->omega_re, kim = fs->omega_im;
-  if (kim == 0.0) far_tp_tile<true>(q.g, kre, 0.0, (cd *)q.M, q.ldm, ti, tj, sh);
-  else far_tp_tile<false>(q.g, kre, kim, (cd *)q.M, q.ldm, ti, tj, sh);
+  far_tp_tile<REALK>(q.g, fs->omega_re, REALK ? 0.0 : fs->omega_im, (cd *)q.M, q.ldm, ti, tj, sh);
 }
 
 __global__ __launch_bounds__(256) void kg_near(const AsmInst *__restrict__ d, int ni, const FreqState *fs)
@@ -1995,9 +1997,11 @@ __global__ __launch_bounds__(64) void kg_emt_final(const AsmInst *__restrict__ d
 }
 
 namespace bfa {
-void launch_far_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, hipStream_t st)
+void launch_far_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, bool realk, hipStream_t st)
 {
-  if (nblk > 0) hipLaunchKernelGGL(kg_far, dim3(nblk), dim3(FT_THREADS), 0, st, d, ni, fs);
+  if (nblk <= 0) return;
+  if (realk) hipLaunchKernelGGL(kg_far<true>, dim3(nblk), dim3(FT_THREADS), 0, st, d, ni, fs);
+  else hipLaunchKernelGGL(kg_far<false>, dim3(nblk), dim3(FT_THREADS), 0, st, d, ni, fs);
 }
 void launch_near_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, hipStream_t st)
 {
