@@ -644,8 +644,6 @@ struct FarTpShared {
   int slot[2][2 * FT2];          // (BF local * 2 + side) -> distinct tet
 };
 
-// 4-point product-rule moments of one tetrahedron pair (signed weights);
-// out[i * stride] for i < 16
 // 1/sqrt(x): the hardware estimate (v_rsq_f64, ~23 correct bits) refined by
 // two Newton steps y <- y + y/2 (1 - x y^2) to full double precision;
 // replaces a correctly rounded sqrt and a division in the pair loop
@@ -657,16 +655,23 @@ __device__ __forceinline__ double rsqrt_nr(double x)
   return y;
 }
 
+// 4-point product-rule moments of one tetrahedron pair (signed weights):
+// row tetrahedron us of the tile (points in sh), column tetrahedron's points
+// ys and centroid cT in registers; out[i * stride] for i < 16
 template <bool REALK>
-__device__ __forceinline__ void tp_moments(const double *xs, const double *cS, const double *ys, const double *cT,
-                                           double kre, double kim, double *out, int stride)
+__device__ __forceinline__ void tp_moments(const FarTpShared &sh, int us, const double *cS, const double *ys,
+                                           const double *cT, double kre, double kim, double *out, int stride)
 {
   double m[16];
 #pragma unroll
   for (int i = 0; i < 16; i++) m[i] = 0.0;
 #pragma unroll 1
   for (int p = 0; p < 4; p++) {
-    const double *x = xs + 4 * p;
+    // row tetrahedron's point p read from LDS: a register array indexed by
+    // this rolled loop's counter would be placed in scratch memory
+    double x[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) x[e] = sh.pts[0][4 * p + e][us];
     const double xi0 = x[0] - cS[0], xi1 = x[1] - cS[1], xi2 = x[2] - cS[2];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -734,7 +739,7 @@ __device__ void far_tp_tile(const DevGeom &g, double kre, double kim, cd *M, int
       double xs[16], cS[3], ys[16], cT[3];
       tp_load(sh, 0, us, xs, cS);
       tp_load(sh, 1, ut, ys, cT);
-      tp_moments<REALK>(xs, cS, ys, cT, kre, kim, &sh.mom[0][pidx], FTT_MAX);
+      tp_moments<REALK>(sh, us, cS, ys, cT, kre, kim, &sh.mom[0][pidx], FTT_MAX);
     }
   }
   __syncthreads();
@@ -764,7 +769,7 @@ __device__ void far_tp_tile(const DevGeom &g, double kre, double kim, cd *M, int
           double xs[16], cS[3], ys[16], cT[3];
           tp_load(sh, 0, us, xs, cS);
           tp_load(sh, 1, ut, ys, cT);
-          tp_moments<REALK>(xs, cS, ys, cT, kre, kim, m, 1);
+          tp_moments<REALK>(sh, us, cS, ys, cT, kre, kim, m, 1);
         } else {
 #pragma unroll
           for (int i = 0; i < 16; i++) m[i] = sh.mom[i][us * nc + ut];
@@ -1504,15 +1509,18 @@ __device__ double emt_tetpair(const DevGeom &g, int ta, double pfa, const double
 // vertex, |omega| max(r_a, r_b) <= 5): the same regrouping as the far field
 // with the real kernel k Im g / 4pi = k sin(kr) / (4 pi r) (series below
 // |kr| = 0.02); other pairs run emt_tetpair's rule directly.
-__device__ __forceinline__ void emt_moments(const double *xs, const double *cS, const double *ys, const double *cT,
-                                            double k, double *out, int stride)
+// Row tetrahedron us's points are read from sh (as in tp_moments).
+__device__ __forceinline__ void emt_moments(const FarTpShared &sh, int us, const double *cS, const double *ys,
+                                            const double *cT, double k, double *out, int stride)
 {
   double m[8];
 #pragma unroll
   for (int i = 0; i < 8; i++) m[i] = 0.0;
 #pragma unroll 1
   for (int p = 0; p < 4; p++) {
-    const double *x = xs + 4 * p;
+    double x[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) x[e] = sh.pts[0][4 * p + e][us];
     const double xi0 = x[0] - cS[0], xi1 = x[1] - cS[1], xi2 = x[2] - cS[2];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
@@ -1564,7 +1572,7 @@ __device__ void emt_tp_tile(const DevGeom &g, double omega, double *Q, int64_t l
       double xs[16], cS[3], ys[16], cT[3];
       tp_load(sh, 0, us, xs, cS);
       tp_load(sh, 1, ut, ys, cT);
-      emt_moments(xs, cS, ys, cT, omega, &sh.mom[0][pidx], FTT_MAX);
+      emt_moments(sh, us, cS, ys, cT, omega, &sh.mom[0][pidx], FTT_MAX);
     }
   }
   __syncthreads();
@@ -1598,7 +1606,7 @@ __device__ void emt_tp_tile(const DevGeom &g, double omega, double *Q, int64_t l
             double xs[16], cS[3], ys[16], cT[3];
             tp_load(sh, 0, us, xs, cS);
             tp_load(sh, 1, ut, ys, cT);
-            emt_moments(xs, cS, ys, cT, omega, m, 1);
+            emt_moments(sh, us, cS, ys, cT, omega, m, 1);
           } else {
 #pragma unroll
             for (int i = 0; i < 8; i++) m[i] = sh.mom[i][us * nc + ut];
