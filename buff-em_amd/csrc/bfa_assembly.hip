@@ -1304,22 +1304,30 @@ __device__ __forceinline__ void near_entry(const DevGeom &g, double kre, double 
   const cd k = {kre, kim}, ik = {-kim, kre};
   const cd k2 = cmul(k, k);
   const cd ik2 = cmul(ik, ik);
-  double ap[8][7], bp[8][7];
-  for (int pt = 0; pt < 8; pt++) { bf_points4(g, a, pt, ap[pt]); bf_points4(g, b, pt, bp[pt]); }
+  // column BF's 8 points in registers (the inner loop is unrolled); the row
+  // BF's point is built per outer iteration, so no array is indexed by a
+  // loop counter the compiler keeps rolled (that would spill to scratch)
+  double bp[8][7];
+#pragma unroll
+  for (int pt = 0; pt < 8; pt++) bf_points4(g, b, pt, bp[pt]);
   cd acc = {0.0, 0.0};
   const cd ik2inv = cdiv({1.0, 0.0}, k2);          // 1/k^2, once per pair
-  for (int ip = 0; ip < 8; ip++)
+  for (int ip = 0; ip < 8; ip++) {
+    double ap[7];
+    bf_points4(g, a, ip, ap);
+#pragma unroll
     for (int jp = 0; jp < 8; jp++) {
-      const double R0 = ap[ip][0] - bp[jp][0], R1 = ap[ip][1] - bp[jp][1], R2 = ap[ip][2] - bp[jp][2];
+      const double R0 = ap[0] - bp[jp][0], R1 = ap[1] - bp[jp][1], R2 = ap[2] - bp[jp][2];
       const double r2 = R0 * R0 + R1 * R1 + R2 * R2;
       if (r2 < 1.0e-24) continue;                  // r < 1e-12
       const double ir = rsqrt_nr(r2), r = r2 * ir;
-      const double dot = ap[ip][3] * bp[jp][3] + ap[ip][4] * bp[jp][4] + ap[ip][5] * bp[jp][5];
-      const double dd = ap[ip][6] * bp[jp][6];
+      const double dot = ap[3] * bp[jp][3] + ap[4] * bp[jp][4] + ap[5] * bp[jp][5];
+      const double dd = ap[6] * bp[jp][6];
       const cd pe = {dot - dd * ik2inv.re, -dd * ik2inv.im};
       const cd ef = exprel3(cscale(ik, r));
       acc = cadd(acc, cmul(pe, cscale(ef, ir * (1.0 / FOURPI))));
     }
+  }
   // static part (GMatrixElements.cc:415-419)
   const double *G = table + 6 * j;
   const cd t0 = {G[0] - cdiv({G[1], 0.0}, k2).re, -cdiv({G[1], 0.0}, k2).im};
