@@ -243,6 +243,18 @@ class Geometry:
                            cols.ctypes.data, table.ctypes.data, Mcol.ctypes.data)
         return Mcol.T.copy()
 
+    def assemble_colmajor(self, omega, mat, table):
+        """Dense VIE matrix M[r, c] as a Fortran-ordered array (no transposed copy:
+        LAPACK can factor it in place, which matters at N ~ 20k)."""
+        rowptr, cols = self.touch_pairs()
+        table = np.ascontiguousarray(table, float)
+        N = self.N
+        Mcol = np.zeros((N, N), dtype=np.complex128)       # Mcol[c, r] = M(r, c)
+        omega = complex(omega)
+        lib().orc_assemble(self.h, omega.real, omega.imag, ctypes.byref(mat), rowptr.ctypes.data,
+                           cols.ctypes.data, table.ctypes.data, Mcol.ctypes.data)
+        return Mcol.T
+
     def assemble_rows(self, omega, rows, table=None):
         """Upper-triangle entries (b >= a) of the given rows, zero static records if table is None."""
         rowptr, cols = self.touch_pairs()
@@ -314,10 +326,11 @@ def dressed_rytov(G, Sinv, Vb, R):
     return lu_solve(M3, M1.conj().T)
 
 
-def lu_solve(M, B):
-    """LAPACK zgetrf + zgetrs (the reference's HMatrix::LUFactorize/LUSolve)."""
+def lu_solve(M, B, overwrite=False):
+    """LAPACK zgetrf + zgetrs (the reference's HMatrix::LUFactorize/LUSolve).
+    overwrite=True factors a Fortran-ordered M in place."""
     import scipy.linalg as sl
-    lu, piv = sl.lu_factor(M, check_finite=False)
+    lu, piv = sl.lu_factor(M, check_finite=False, overwrite_a=overwrite)
     return sl.lu_solve((lu, piv), B, check_finite=False)
 
 
@@ -347,6 +360,8 @@ def fields(geom, V, T, J, omega, X, pws=()):
         Et = np.zeros(3, complex)
         Ht = np.zeros(3, complex)
         for a in range(N):
+            if J[a] == 0:
+                continue                    # no current in this BF: no field
             fv = Vt[F["V"][a]]
             cen = fv.mean(0)
             rad = max(np.linalg.norm(Vt[i] - cen) for i in (F["QP"][a], *F["V"][a], F["QM"][a]))
