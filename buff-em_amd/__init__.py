@@ -86,6 +86,7 @@ _SIGS = {
     "bfa_assemble_vie_far32": (I, [P, D, D, P, P, I64, P]),
     "bfa_lu_factor_z": (I, [P, I, I64, P, P, P]),
     "bfa_lu_solve_z": (I, [P, I, I64, P, P, I, I64, P]),
+    "bfa_lu_solve_z_noalloc": (I, [P, I, I64, P, P, I, I64, P]),
     "bfa_emt_power": (I, [P, D, P, I, P, I64, P, P]),
     "bfa_zgemm_sub": (I, [I, I, I, P, I64, P, I64, P, I64, P]),
     "bfa_cgemm_sub": (I, [I, I, I, P, I64, P, I64, P, I64, P]),
@@ -456,6 +457,13 @@ def lu_factor(d_A, n, d_ipiv, lda=None, stream=None):
 def lu_solve(d_A, n, d_ipiv, d_B, nrhs, lda=None, ldb=None, stream=None):
     _check(lib().bfa_lu_solve_z(_ptr(d_A), int(n), int(lda or n), _ptr(d_ipiv), _ptr(d_B), int(nrhs),
                                 int(ldb or n), _stream_ptr(stream)), "bfa_lu_solve_z")
+
+
+def lu_solve_noalloc(d_A, n, d_ipiv, d_B, nrhs, lda=None, ldb=None, stream=None):
+    """lu_solve with the library's static scratch array: no device allocation
+    (n <= 20480, nrhs <= 8; concurrent calls on one GPU race)."""
+    _check(lib().bfa_lu_solve_z_noalloc(_ptr(d_A), int(n), int(lda or n), _ptr(d_ipiv), _ptr(d_B), int(nrhs),
+                                        int(ldb or n), _stream_ptr(stream)), "bfa_lu_solve_z_noalloc")
 
 
 def zgemm_sub(m, n, k, d_A, lda, d_B, ldb, d_C, ldc, stream=None):

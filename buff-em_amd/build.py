@@ -16,7 +16,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "lib", "libbuffem_amd.so")
 
-SOURCES = ["bfa_host.cpp", "bfa_fibbi.cpp", "bfa_material.cpp", "bfa_assembly.hip", "bfa_lu.hip", "bfa_forms.hip", "bfa_batch.hip", "bfa_neq.hip", "bfa_mlp.hip", "bfa_surrogate.hip"]
+# every C++ and HIP source of the library
+SOURCES = sorted(f for f in os.listdir(CSRC) if f.endswith((".cpp", ".hip")))
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-value", "-Wno-unused-result",

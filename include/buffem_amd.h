@@ -196,6 +196,19 @@ int  bfa_lu_factor_z(void *d_A, int n, int64_t lda, int *d_ipiv, int *info,
 int  bfa_lu_solve_z(const void *d_A, int n, int64_t lda, const int *d_ipiv,
                     void *d_B, int nrhs, int64_t ldb, void *stream);
 
+/* bfa_lu_solve_z without any device allocation (no reference counterpart):
+ * for callers that cannot call hipMalloc -- code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation.  Same kernels and
+ * results as bfa_lu_solve_z; its scratch is one static device array per GPU
+ * (47.6 MiB, kept until the process ends), so n <= BFA_NOALLOC_MAX_N and
+ * nrhs <= BFA_NOALLOC_MAX_NRHS (larger sizes return BFA_ERR_ARG), and calls
+ * that overlap on one GPU race on that array.  Synchronous like
+ * bfa_lu_solve_z.                                                          */
+#define BFA_NOALLOC_MAX_N    20480
+#define BFA_NOALLOC_MAX_NRHS 8
+int  bfa_lu_solve_z_noalloc(const void *d_A, int n, int64_t lda, const int *d_ipiv,
+                            void *d_B, int nrhs, int64_t ldb, void *stream);
+
 /* LU execution on this device (no reference counterpart): on = 1 (default)
  * runs the look-ahead LU on CU-partitioned streams (the panel on 64 CUs,
  * the trailing updates on the rest; BFA_LU_MASK sets the count, 0 disables);
