@@ -66,6 +66,19 @@ def build(jobs=4, verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(drv + ".tmp", drv)
+    # programs written against the source-compatible drop-in (host/libbuff.h)
+    exdir = os.path.join(HERE, "host", "examples")
+    for src in sorted(os.listdir(exdir)):
+        if not src.endswith(".cc"):
+            continue
+        exe = os.path.join(HERE, "lib", src[:-3] + "_dropin")
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(HERE, "host"),
+               os.path.join(exdir, src), "-o", exe + ".tmp", "-L" + os.path.dirname(LIB), "-lbuffem_amd",
+               "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"drop-in example build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(exe + ".tmp", exe)
     return LIB
 
 
