@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
+PEAK_FP16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 
 # HBM bytes per launch of the roofline kernel (its launches on the LU's update
@@ -225,8 +226,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=12, help="subcubes per axis (12 = the N=19,872 workload)")
     ap.add_argument("--td-order", type=int, default=10)
-    ap.add_argument("--forms", default="fp64,mixed,graph",
-                    help="comma list of fp64, mixed, graph (mixed in grouped kernels replayed from a hipGraph)")
+    ap.add_argument("--forms", default="fp64,mixed,fast,graph",
+                    help="comma list of fp64, mixed (complex64 partial-pivoting LU + FP64 refinement), fast "
+                         "(block-local pivoting, split-FP16 trailing updates + FP64 refinement), graph (mixed "
+                         "in grouped kernels replayed from a hipGraph)")
     ap.add_argument("--workload", default="single", choices=["single", "batch"],
                     help="single: one N=19,872 sphere; batch: 48 uneven spheres x 8 frequencies")
     ap.add_argument("--batch-count", type=int, default=48)
@@ -273,7 +276,7 @@ def main():
 
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
-    dM32 = torch.empty((N, N), dtype=torch.complex64, device=dev) if "mixed" in forms else None
+    dM32 = torch.empty((N, N), dtype=torch.complex64, device=dev) if ("mixed" in forms or "fast" in forms) else None
     refine = {"iters": 0, "resid": 0.0}
 
     def step(form, phase, record):
@@ -285,6 +288,9 @@ def main():
             ev[1].record()
         if form == "fp64":
             info = bfa.lu_factor(dM, N, dp)
+        elif form == "fast":
+            bfa.convert_z2c(dM, N, N, dM32)
+            info = bfa.lu_factor_c_local(dM32, N, dp)
         else:
             bfa.convert_z2c(dM, N, N, dM32)
             info = bfa.lu_factor_c(dM32, N, dp)
@@ -295,8 +301,10 @@ def main():
         if form == "fp64":
             bfa.lu_solve(dM, N, dp, dB, len(spec))
         else:
-            it, res = bfa.solve_refine(dM, dM32, dp, N, dB, len(spec), tol=args.refine_tol, maxit=20)
+            it, res = bfa.solve_refine(dM, dM32, dp, N, dB, len(spec), tol=args.refine_tol, maxit=30)
             refine["iters"], refine["resid"] = it, res
+            if res > args.refine_tol:
+                raise RuntimeError(f"{form}: refinement stalled at residual {res:.2e} after {it} iterations")
         if record:
             ev[3].record()
             torch.cuda.synchronize()
@@ -326,13 +334,15 @@ def main():
         prof_big = bfa.profile_read(2 if form == "fp64" else 3)
         prof_slice = bfa.profile_read(4 if form == "fp64" else 5)
         prof = bfa.profile_read(0 if form == "fp64" else 1)
+        prof_split = bfa.profile_read(6)
         for k in phase:
             phase[k] /= args.steps
         P = g.emt_power(omega, pw, dB)
         results[form] = {"s_per_freq": t_total / args.steps, "phases_s": dict(phase), "gemm": prof,
-                         "gemm_big": prof_big, "gemm_slice": prof_slice, "J": dB.cpu().numpy().T.copy(),
-                         "P": P.copy(), "refine": dict(refine) if form == "mixed" else None}
-        log(f"{form}: {t_total / args.steps:.4f} s/frequency  phases {phase}  refine {refine if form == 'mixed' else ''}")
+                         "gemm_big": prof_big, "gemm_slice": prof_slice, "gemm_split": prof_split,
+                         "J": dB.cpu().numpy().T.copy(),
+                         "P": P.copy(), "refine": dict(refine) if form != "fp64" else None}
+        log(f"{form}: {t_total / args.steps:.4f} s/frequency  phases {phase}  refine {refine if form != 'fp64' else ''}")
 
     lu_flops = 8.0 / 3.0 * N ** 3
     ref = results.get("fp64")
@@ -351,12 +361,69 @@ def main():
     # HIP events in the fp64 / mixed forms (the graph form runs the same
     # kernels grouped)
     rk = results["mixed" if best == "graph" and "mixed" in results else best]
+    if best == "fast":
+        roofline = fast_roofline(rk, N, args.steps)
+    else:
+        roofline = plain_roofline(rk, best, N, args.steps, lu_flops)
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu, J_cpu = cpu_baseline(g, V, T, N, omega, eps, spec, log=log)
+        # the solutions of the timed GPU forms against the CPU oracle's
+        cpu["solution_rel_err"] = {f: float(np.linalg.norm(v["J"] - J_cpu) / np.linalg.norm(J_cpu))
+                                   for f, v in results.items() if "J" in v}
+    forms_out = {f: {"s_per_freq": v["s_per_freq"], "phases_s": v["phases_s"],
+                     "P_abs": v["P"][:, 0].tolist(), "P_scat": v["P"][:, 1].tolist(),
+                     "gemm_tflops": (v["gemm"][1] / (v["gemm"][0] * 1e-3) / 1e12) if v["gemm"][0] > 0 else None,
+                     "refine": v["refine"], "power_rel_err_vs_fp64": errs.get(f)} for f, v in results.items()}
+    dtypes = {"fp64": "complex128 (f64)", "mixed": "complex64 LU (f32 MFMA) + f64 refinement",
+              "graph": "complex64 LU (f32 MFMA) + f64 refinement",
+              "fast": "complex64 LU (split-f16 MFMA updates, block-local pivots) + f64 refinement"}
+    out = {
+        "metric": "assemble+factor+solve s/frequency at N~20k (Kuhn sphere n=12); mixed vs FP64 GPU at rel err <=1e-8",
+        "value": r["s_per_freq"], "unit": "s/frequency", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": r["s_per_freq"] * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": dtypes[best],
+        "fp64_s_per_freq": ref["s_per_freq"] if ref else None,
+        "speedup_vs_fp64": ref["s_per_freq"] / r["s_per_freq"] if ref else None,
+        "data": "synthetic",
+        "config": {"workload": "kuhn_sphere_n12_diag_eps_2rhs", "N": N, "tets": g.nt, "form": best,
+                   "omega": omega, "eps": "diag(4,5,6)+0.05i", "nrhs": len(spec), "td_order": args.td_order},
+        "power_rel_err_vs_fp64": err,
+        "speedup_mixed_over_fp64": (results["fp64"]["s_per_freq"] / results["mixed"]["s_per_freq"]
+                                    if "fp64" in results and "mixed" in results else None),
+        "forms": forms_out, "cold_static_s": t_cold,
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def fast_roofline(rk, N, steps):
+    """The fast form's dominant kernel, the split-FP16 trailing update: 3 FP16
+    products per real product (24 mnk FP16 flops per complex m x n x k update)
+    on the 2.5 PF dense FP16 matrix cores, from HIP events around each launch."""
+    ms, fl, nl = rk["gemm_split"]
+    f16 = 3.0 * fl                        # fl counts 8 mnk per complex update
+    achieved = f16 / (ms * 1e-3) / 1e12 if ms > 0 else None
+    return {
+        "kernel": "k_gemm_hs (LU trailing update: split-FP16 complex product on v_mfma_f32_16x16x32_f16)",
+        "bound": "mfma", "unit": "TFLOP/s", "achieved": achieved, "peak": PEAK_FP16_TFLOPS,
+        "frac": achieved / PEAK_FP16_TFLOPS if achieved else None,
+        "traffic": MEASURED_TRAFFIC.get(("k_gemm_hs", N)),
+        "per_launch_flops": f16 / max(nl, 1), "avg_launch_ms": ms / max(nl, 1),
+        "launches_per_step": nl / steps,
+        # the same launches as a complex64 GEMM (8 mnk flops per update)
+        "complex_gemm_equiv_tflops": fl / (ms * 1e-3) / 1e12 if ms > 0 else None,
+    }
+
+
+def plain_roofline(rk, best, N, steps, lu_flops):
     gemm_ms, gemm_flops, gemm_launches = rk["gemm_big"]
     all_ms, all_flops, all_launches = rk["gemm"]
     peak = PEAK_FP64_TFLOPS if best == "fp64" else PEAK_FP32_TFLOPS
     gemm_tflops = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else None
     sl_ms, sl_flops, sl_launches = rk["gemm_slice"]
-    roofline = {
+    args_steps = steps
+    return {
         "kernel": ("rocBLAS zgemm (LU trailing update, complex128 on f64 MFMA, on the 192 update CUs)"
                    if best == "fp64" else
                    "rocBLAS cgemm (LU trailing update, complex64 on f32 MFMA, on the 192 update CUs)"),
@@ -370,41 +437,14 @@ def main():
         "frac_of_cu_share": gemm_tflops / peak / (192 / 256) if gemm_tflops else None,
         "per_launch_flops": gemm_flops / max(gemm_launches, 1),
         "avg_launch_ms": gemm_ms / max(gemm_launches, 1),
-        "launches_per_step": gemm_launches / args.steps,
+        "launches_per_step": gemm_launches / args_steps,
         # column slices of the bulk update run on the 64 panel CUs beside
-        "panel_cu_slices": {"launches_per_step": sl_launches / args.steps,
+        "panel_cu_slices": {"launches_per_step": sl_launches / args_steps,
                             "tflops": sl_flops / (sl_ms * 1e-3) / 1e12 if sl_ms > 0 else None},
         "all_gemm_launches_tflops": all_flops / (all_ms * 1e-3) / 1e12 if all_ms > 0 else None,
         "lu_tflops_effective": (lu_flops / rk["phases_s"]["factor"] / 1e12
                                 if rk["phases_s"].get("factor", 0) > 0 else None),
     }
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu, J_cpu = cpu_baseline(g, V, T, N, omega, eps, spec, log=log)
-        # the solutions of the timed GPU forms against the CPU oracle's
-        cpu["solution_rel_err"] = {f: float(np.linalg.norm(v["J"] - J_cpu) / np.linalg.norm(J_cpu))
-                                   for f, v in results.items() if "J" in v}
-    forms_out = {f: {"s_per_freq": v["s_per_freq"], "phases_s": v["phases_s"],
-                     "P_abs": v["P"][:, 0].tolist(), "P_scat": v["P"][:, 1].tolist(),
-                     "gemm_tflops": (v["gemm"][1] / (v["gemm"][0] * 1e-3) / 1e12) if v["gemm"][0] > 0 else None,
-                     "refine": v["refine"], "power_rel_err_vs_fp64": errs.get(f)} for f, v in results.items()}
-    out = {
-        "metric": "assemble+factor+solve s/frequency at N~20k (Kuhn sphere n=12); mixed vs FP64 GPU at rel err <=1e-8",
-        "value": r["s_per_freq"], "unit": "s/frequency", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": r["s_per_freq"] * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "complex128 (f64)" if best == "fp64" else "complex64 LU (f32 MFMA) + f64 refinement",
-        "fp64_s_per_freq": ref["s_per_freq"] if ref else None,
-        "speedup_vs_fp64": ref["s_per_freq"] / r["s_per_freq"] if ref else None,
-        "data": "synthetic",
-        "config": {"workload": "kuhn_sphere_n12_diag_eps_2rhs", "N": N, "tets": g.nt, "form": best,
-                   "omega": omega, "eps": "diag(4,5,6)+0.05i", "nrhs": len(spec), "td_order": args.td_order},
-        "mixed_vs_fp64_power_rel_err": err,
-        "speedup_mixed_over_fp64": (results["fp64"]["s_per_freq"] / results["mixed"]["s_per_freq"]
-                                    if "fp64" in results and "mixed" in results else None),
-        "forms": forms_out, "cold_static_s": t_cold,
-        "roofline": roofline, "cpu_baseline": cpu,
-    }
-    print(json.dumps(out))
 
 
 if __name__ == "__main__":

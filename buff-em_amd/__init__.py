@@ -19,6 +19,7 @@ T[c, r] = M(r, c), i.e. M = T.T.
 There is no CPU fallback: every compute call goes to the HIP library and
 raises if it is missing or fails.
 """
+import atexit
 import ctypes
 import os
 
@@ -87,6 +88,9 @@ _SIGS = {
     "bfa_lu_factor_z": (I, [P, I, I64, P, P, P]),
     "bfa_lu_solve_z": (I, [P, I, I64, P, P, I, I64, P]),
     "bfa_lu_solve_z_noalloc": (I, [P, I, I64, P, P, I, I64, P]),
+    "bfa_lu_factor_c_local": (I, [P, I, I64, P, P, P]),
+    "bfa_shutdown": (I, []),
+    "bfa_cgemm_sub_split": (I, [I, I, I, P, I64, P, I64, P, I64, P]),
     "bfa_emt_power": (I, [P, D, P, I, P, I64, P, P]),
     "bfa_zgemm_sub": (I, [I, I, I, P, I64, P, I64, P, I64, P]),
     "bfa_cgemm_sub": (I, [I, I, I, P, I64, P, I64, P, I64, P]),
@@ -140,6 +144,9 @@ def lib():
             f.restype = res
             f.argtypes = args
         _lib = L
+        # release the library's device objects before the GPU runtime's own
+        # teardown (atexit handlers run last-registered first)
+        atexit.register(L.bfa_shutdown)
     return _lib
 
 
@@ -505,6 +512,20 @@ def lu_factor_c(d_A, n, d_ipiv, lda=None, stream=None):
     _check(lib().bfa_lu_factor_c(_ptr(d_A), int(n), int(lda or n), _ptr(d_ipiv), ctypes.byref(info),
                                  _stream_ptr(stream)), "bfa_lu_factor_c")
     return info.value
+
+
+def lu_factor_c_local(d_A, n, d_ipiv, lda=None, stream=None):
+    """complex64 LU, pivoting within 256-row diagonal blocks, split-FP16 trailing updates."""
+    info = I()
+    _check(lib().bfa_lu_factor_c_local(_ptr(d_A), int(n), int(lda or n), _ptr(d_ipiv), ctypes.byref(info),
+                                       _stream_ptr(stream)), "bfa_lu_factor_c_local")
+    return info.value
+
+
+def cgemm_sub_split(m, n, k, d_A, lda, d_B, ldb, d_C, ldc, stream=None):
+    """C -= A @ B (complex64, k <= 256) by split-FP16 MFMA products."""
+    _check(lib().bfa_cgemm_sub_split(int(m), int(n), int(k), _ptr(d_A), int(lda), _ptr(d_B), int(ldb), _ptr(d_C),
+                                     int(ldc), _stream_ptr(stream)), "bfa_cgemm_sub_split")
 
 
 def lu_solve_c(d_A, n, d_ipiv, d_B, nrhs, lda=None, ldb=None, stream=None):

@@ -77,11 +77,17 @@ template <> struct mf<float> {
 #define GBM 64
 #define GBN 64
 #define GBK 16
+// LDS row stride of the GEMM tiles: 16 elements of padding per k row.  An
+// MFMA operand read takes k row lane / 16 and row lane % 16, so with
+// unpadded rows of 64 or 128 elements (a multiple of 64 banks x 4 bytes)
+// the four lane groups address the same banks; the padding moves each
+// group to its own banks
+#define GPAD 16
 
 template <typename T>
 __device__ __forceinline__ void gemm64_body(int m, int n, int k, const cplx<T> *__restrict__ A, int64_t lda,
                                             const cplx<T> *__restrict__ B, int64_t ldb, cplx<T> *C, int64_t ldc,
-                                            int row0, int col0, T (&As)[2][GBK][GBM], T (&Bs)[2][GBK][GBN])
+                                            int row0, int col0, T (&As)[2][GBK][GBM + GPAD], T (&Bs)[2][GBK][GBN + GPAD])
 {
   typedef typename mf<T>::acc acc_t;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -167,8 +173,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_gemm_sub(int m, int n, int k, const cplx<T> *__restrict__ A, int64_t lda,
                                                   const cplx<T> *__restrict__ B, int64_t ldb, cplx<T> *C, int64_t ldc)
 {
-  __shared__ T As[2][GBK][GBM];
-  __shared__ T Bs[2][GBK][GBN];
+  __shared__ T As[2][GBK][GBM + GPAD];
+  __shared__ T Bs[2][GBK][GBN + GPAD];
   gemm64_body<T>(m, n, k, A, lda, B, ldb, C, ldc, blockIdx.x * GBM, blockIdx.y * GBN, As, Bs);
 }
 
@@ -187,10 +193,41 @@ __device__ __forceinline__ int xcd_remap(int bid, int ntile)
   return (ntile % 8 == 0) ? (bid % 8) * (ntile / 8) + bid / 8 : bid;
 }
 
+// Epilogue of the 128 x 128 tiles: C += (cr, ci), the accumulators holding
+// the product transposed (B's value as the MFMA's first operand), so that
+// register j of block (rb, cb) is row 16 rb + l16 and column 16 cb +
+// row(j, lq) of the wave's sub-tile: 16 lanes write 16 consecutive rows of
+// one column.  Each row block's loads are issued before its stores.
+template <typename T, int CB>
+__device__ __forceinline__ void gemm_epilogue(int m, int n, cplx<T> *C, int64_t ldc, int row, int col,
+                                              int lq, const typename mf<T>::acc (&cr)[4][CB],
+                                              const typename mf<T>::acc (&ci)[4][CB])
+{
+#pragma unroll
+  for (int rb = 0; rb < 4; rb++) {
+    const int r = row + rb * 16;
+    cplx<T> v[CB][4];
+#pragma unroll
+    for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int c = col + cb * 16 + mf<T>::row(j, lq);
+        v[cb][j] = (r < m && c < n) ? C[(size_t)c * ldc + r] : cplx<T>{0, 0};
+      }
+#pragma unroll
+    for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int c = col + cb * 16 + mf<T>::row(j, lq);
+        if (r < m && c < n) C[(size_t)c * ldc + r] = {v[cb][j].re + cr[rb][cb][j], v[cb][j].im + ci[rb][cb][j]};
+      }
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ void gemm128_body(int m, int n, int k, const cplx<T> *__restrict__ A, int64_t lda,
                                              const cplx<T> *__restrict__ B, int64_t ldb, cplx<T> *C, int64_t ldc,
-                                             int row0, int col0, T (&As)[2][2][G2K][G2M], T (&Bs)[2][2][G2K][G2N])
+                                             int row0, int col0, T (&As)[2][2][G2K][G2M + GPAD], T (&Bs)[2][2][G2K][G2N + GPAD])
 {
   typedef typename mf<T>::acc acc_t;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -249,35 +286,21 @@ __device__ __forceinline__ void gemm128_body(int m, int n, int k, const cplx<T> 
       for (int rb = 0; rb < 4; rb++)
 #pragma unroll
         for (int cb = 0; cb < 2; cb++) {
-          cr[rb][cb] = mf<T>::op(ar[rb], br[cb], cr[rb][cb]);
-          ci[rb][cb] = mf<T>::op(ar[rb], bi[cb], ci[rb][cb]);
+          cr[rb][cb] = mf<T>::op(br[cb], ar[rb], cr[rb][cb]);
+          ci[rb][cb] = mf<T>::op(bi[cb], ar[rb], ci[rb][cb]);
         }
 #pragma unroll
       for (int rb = 0; rb < 4; rb++)
 #pragma unroll
         for (int cb = 0; cb < 2; cb++) {
-          cr[rb][cb] = mf<T>::op(ai[rb], bi[cb], cr[rb][cb]);
-          ci[rb][cb] = mf<T>::op(-ai[rb], br[cb], ci[rb][cb]);
+          cr[rb][cb] = mf<T>::op(bi[cb], ai[rb], cr[rb][cb]);
+          ci[rb][cb] = mf<T>::op(br[cb], -ai[rb], ci[rb][cb]);
         }
     }
     if (st + 1 < nst) sstore(buf ^ 1);
     __syncthreads();
   }
-#pragma unroll
-  for (int rb = 0; rb < 4; rb++)
-#pragma unroll
-    for (int cb = 0; cb < 2; cb++) {
-      const int col = col0 + wc * 32 + cb * 16 + l16;
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int row = row0 + wr * 64 + rb * 16 + mf<T>::row(j, lq);
-        if (row < m && col < n) {
-          cplx<T> *c = &C[(size_t)col * ldc + row];
-          const cplx<T> v = *c;
-          *c = {v.re + cr[rb][cb][j], v.im + ci[rb][cb][j]};
-        }
-      }
-    }
+  gemm_epilogue<T, 2>(m, n, C, ldc, row0 + wr * 64 + l16, col0 + wc * 32, lq, cr, ci);
 }
 
 template <typename T>
@@ -285,8 +308,8 @@ __global__ __launch_bounds__(512) void k_gemm_sub2(int m, int n, int k, const cp
                                                    const cplx<T> *__restrict__ B, int64_t ldb, cplx<T> *C, int64_t ldc,
                                                    int tiles_m, int tiles_n)
 {
-  __shared__ T As[2][2][G2K][G2M];
-  __shared__ T Bs[2][2][G2K][G2N];
+  __shared__ T As[2][2][G2K][G2M + GPAD];
+  __shared__ T Bs[2][2][G2K][G2N + GPAD];
   const int t = xcd_remap(blockIdx.x, tiles_m * tiles_n);
   gemm128_body<T>(m, n, k, A, lda, B, ldb, C, ldc, (t % tiles_m) * G2M, (t / tiles_m) * G2N, As, Bs);
 }
@@ -326,9 +349,10 @@ static void prof_begin(hipStream_t st, hipEvent_t &a)
 // factorisation takes longer (317 against 297 ms).  BFA_GEMM=own or =blas
 // selects one kind for both precisions.  Streams being captured into a graph
 // always use the hand-written kernels.
+static rocblas_handle g_blas[64] = {};
 static rocblas_handle blas_handle(hipStream_t st, bool want)
 {
-  static rocblas_handle h[64] = {};
+  rocblas_handle *h = g_blas;
   static int mode = -1;     // 0: own kernels, 1: where the caller wants rocBLAS, 2: always
   if (mode < 0) {
     const char *e = getenv("BFA_GEMM");
@@ -917,8 +941,8 @@ __device__ __forceinline__ int find_prob(const P *__restrict__ d, int np, int bi
 template <typename T>
 __global__ __launch_bounds__(256) void kg_gemm64(const GemmProb *__restrict__ d, int np)
 {
-  __shared__ T As[2][GBK][GBM];
-  __shared__ T Bs[2][GBK][GBN];
+  __shared__ T As[2][GBK][GBM + GPAD];
+  __shared__ T Bs[2][GBK][GBN + GPAD];
   const GemmProb q = d[find_prob(d, np, blockIdx.x)];
   const int t = blockIdx.x - q.blk0;
   gemm64_body<T>(q.m, q.n, q.k, (const cplx<T> *)q.A, q.lda, (const cplx<T> *)q.B, q.ldb, (cplx<T> *)q.C, q.ldc,
@@ -928,8 +952,8 @@ __global__ __launch_bounds__(256) void kg_gemm64(const GemmProb *__restrict__ d,
 template <typename T>
 __global__ __launch_bounds__(512) void kg_gemm128(const GemmProb *__restrict__ d, int np)
 {
-  __shared__ T As[2][2][G2K][G2M];
-  __shared__ T Bs[2][2][G2K][G2N];
+  __shared__ T As[2][2][G2K][G2M + GPAD];
+  __shared__ T Bs[2][2][G2K][G2N + GPAD];
   const int bid = xcd_remap(blockIdx.x, gridDim.x);
   const GemmProb q = d[find_prob(d, np, bid)];
   const int t = bid - q.blk0;
@@ -1794,6 +1818,53 @@ int gemm_sub_c(int m, int n, int k, const void *A, int64_t lda, const void *B, i
 }
 int lu_factor_z(void *A, int n, int64_t lda, int *ipiv, int *info, hipStream_t st, int nb)
 { return lu_factor<double>((cplx<double> *)A, n, lda, ipiv, info, st, nb); }
+
+// pieces of the complex64 LU for the block-local factorisation (bfa_lu_local.hip)
+bool profile_on() { return g_prof.on; }
+
+int lu_local_begin(hipStream_t st)
+{
+  LUWork *lw = nullptr;
+  int rc = lu_work(lw);
+  if (rc) return rc;
+  HIPCHECK(hipMemsetAsync(lw->pw.flags, 0, 8, st));
+  return BFA_OK;
+}
+int lu_local_panel_c(void *A, int m, int w, int64_t lda, int *ipiv, int piv_off, hipStream_t st)
+{
+  LUWork *lw = nullptr;
+  int rc = lu_work(lw);
+  if (rc) return rc;
+  return panel_rec<float>((cplx<float> *)A, m, w, lda, ipiv, piv_off, lw, st, lw->num_cu);
+}
+int lu_local_end(int *info, hipStream_t st)
+{
+  LUWork *lw = nullptr;
+  int rc = lu_work(lw);
+  if (rc) return rc;
+  int flags[2] = {0, 0};
+  HIPCHECK(hipMemcpyAsync(flags, lw->pw.flags, 8, hipMemcpyDeviceToHost, st));
+  HIPCHECK(hipStreamSynchronize(st));
+  if (flags[0]) return fail(BFA_ERR_HIP, "LU panel barrier timed out");
+  if (info) *info = flags[1];
+  return BFA_OK;
+}
+void laswp_c(void *A, int64_t lda, int c0, int c1, int k0, int kb, const int *ipiv, hipStream_t st)
+{ laswp<float>((cplx<float> *)A, lda, c0, c1, k0, kb, ipiv, st); }
+void trsm_lower_c(int kb, int n, const void *L, int64_t ldl, void *B, int64_t ldb, hipStream_t st)
+{ trsm_lower<float>(kb, n, (const cplx<float> *)L, ldl, (cplx<float> *)B, ldb, st); }
+void trsm_upper_c(int kb, int n, const void *U, int64_t ldu, void *B, int64_t ldb, hipStream_t st)
+{
+  if (kb <= 0 || n <= 0) return;
+  constexpr int SC = TfShape<float>::SC;
+  hipLaunchKernelGGL(k_trsm_uf<float>, dim3((n + SC - 1) / SC), dim3(256), 0, st, kb, n, (const cplx<float> *)U, ldu,
+                     (cplx<float> *)B, ldb);
+}
+int *lu_local_flags()
+{
+  LUWork *lw = nullptr;
+  return lu_work(lw) == BFA_OK ? lw->pw.flags : nullptr;
+}
 int lu_factor_c(void *A, int n, int64_t lda, int *ipiv, int *info, hipStream_t st, int nb)
 { return lu_factor<float>((cplx<float> *)A, n, lda, ipiv, info, st, nb); }
 int lu_solve_z(const void *A, int n, int64_t lda, const int *ipiv, void *B, int nrhs, int64_t ldb, hipStream_t st, void *s)
@@ -1936,6 +2007,46 @@ extern "C" int bfa_cgemm_sub(int m, int n, int k, const void *d_A, int64_t lda, 
   return BFA_OK;
 }
 
+namespace bfa {
+// Release every process-lifetime device object of the LU: work areas,
+// streams and events of the look-ahead, instrumentation events, library
+// handles.  A later LU call creates them again.
+void lu_release()
+{
+  for (int d = 0; d < 16; d++) {
+    LUWork &w = g_lu[d];
+    if (!w.mem && !w.side && !w.s_pan) continue;
+    if (hipSetDevice(w.device >= 0 ? w.device : d) != hipSuccess) continue;
+    (void)hipDeviceSynchronize();
+    if (w.mem) (void)hipFree(w.mem);
+    if (w.side) (void)hipStreamDestroy(w.side);
+    for (auto &e : w.ev) if (e) (void)hipEventDestroy(e);
+    if (w.s_pan) (void)hipStreamDestroy(w.s_pan);
+    if (w.s_bulk) (void)hipStreamDestroy(w.s_bulk);
+    for (auto &e : w.evm) if (e) (void)hipEventDestroy(e);
+    w = LUWork();
+  }
+  for (int d = 0; d < 64; d++)
+    if (g_blas[d]) {
+      (void)rocblas_destroy_handle(g_blas[d]);
+      g_blas[d] = nullptr;
+    }
+  for (hipEvent_t e : g_prof.pool) (void)hipEventDestroy(e);
+  g_prof = GemmProfile();
+}
+
+}  // namespace bfa
+
+extern "C" int bfa_shutdown(void)
+{
+  int dev = 0;
+  const bool have = hipGetDevice(&dev) == hipSuccess;
+  bfa::lu_release();
+  bfa::local_release();
+  if (have) (void)hipSetDevice(dev);
+  return BFA_OK;
+}
+
 extern "C" int bfa_lu_partition(int on)
 {
   bfa::LUWork *lw = nullptr;
@@ -1956,7 +2067,8 @@ extern "C" int bfa_profile_read(int which, double *ms, double *flops, int *launc
   // which: 0 / 1 = every complex128 / complex64 trailing-update launch;
   // 2 / 3 = the large updates (m, n >= 256) on all or the update CUs;
   // 4 / 5 = the large updates run as column slices on the panel CUs
-  if (which < 0 || which > 5) return fail(BFA_ERR_ARG, "bfa_profile_read: which must be 0..5");
+  if (which == 6) return bfa::profile_read_split(ms, flops, launches);
+  if (which < 0 || which > 6) return fail(BFA_ERR_ARG, "bfa_profile_read: which must be 0..6");
   const int type = which & 1;
   const int lo = which < 2 ? 3 * type : which < 4 ? 3 * type + 1 : 3 * type + 2;
   const int hi = which < 2 ? 3 * type + 2 : lo;

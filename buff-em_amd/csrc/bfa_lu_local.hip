@@ -1,0 +1,780 @@
+// bfa_lu_local.hip -- complex64 LU with block-local pivoting and trailing
+// updates on the FP16 matrix cores, the factorisation behind the fast
+// refinement form.
+//
+// No reference counterpart: the reference factors with LAPACK zgetrf
+// (HMatrix::LUFactorize, buff-scatter.cc:285).  This factorisation is only
+// a preconditioner for FP64 iterative refinement against the resident FP64
+// matrix (bfa_solve_refine), which restores FP64 accuracy; it trades the
+// partial-pivoting search over all rows -- one grid-wide barrier per column,
+// the latency-bound critical path of the complex64 LU -- for two changes:
+//
+//  - Pivoting within each 256-row diagonal block.  Step j factors the
+//    diagonal block with partial pivoting over its own rows (the panel
+//    recursion of the partial-pivoting LU, run on jb rows), swaps those rows
+//    in the other columns, forms U12 = L11^-1 A12 (unit-lower TRSM) and
+//    L21 = A21 U11^-1 (U11^-1 by a one-launch upper TRSM, then a product on
+//    the matrix cores) -- no synchronisation across workgroups anywhere.
+//    The diagonal block's own factorisation runs its 32-column leaves on one
+//    wavefront each.  The VIE matrices factor stably this way (growth ~1;
+//    partial pivoting leaves >99.5 % of the rows in place on them).
+//  - The trailing update A22 -= L21 U12 in split FP16: every FP32 operand x
+//    (scaled by a power of two per row of L21 / column of U12 so that the
+//    row's or column's largest magnitude lies in [2^14, 2^15)) is written as
+//    hi + lo with hi = fp16(x), lo = fp16(x - hi), and each real product is
+//    hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_f16 with FP32
+//    accumulation (4 real products per complex product, 12 MFMA per 16x16x32
+//    block).  hi + lo carries 22 significant bits; the dropped lo*lo term is
+//    below 2^-22 relative; entries under 2^-17 of their row's (column's)
+//    maximum lose relative precision in lo but stay within 2^-38 of that
+//    maximum absolutely.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdint>
+#include <utility>
+#include <vector>
+
+#include "bfa_internal.h"
+#include "bfa_lu.h"
+
+using namespace bfa;
+
+#define HIPCHECK(call)                                                              \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      return fail(e_ == hipErrorOutOfMemory ? BFA_ERR_OOM : BFA_ERR_HIP, "%s: %s (%s:%d)", \
+                  #call, hipGetErrorString(e_), __FILE__, __LINE__);                \
+  } while (0)
+
+namespace {
+
+struct c2 { float re, im; };
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+#define LB 256          // block width of the factorisation (= max k of the split operands)
+#define HT 128          // GEMM workgroup tile (rows and columns)
+#define HK 32           // k per stage = K of one MFMA
+
+// power-of-two scale bringing max into [2^14, 2^15): returns e with max * 2^e there
+__device__ __forceinline__ int split_exp(float mx)
+{
+  if (!(mx > 0.0f) || !isfinite(mx)) return 0;
+  return 14 - ilogbf(mx);
+}
+
+__device__ __forceinline__ void split_one(float x, _Float16 &hi, _Float16 &lo)
+{
+  hi = (_Float16)x;
+  lo = (_Float16)(x - (float)hi);
+}
+
+// ---------------------------------------------------------------------------
+// L21 (m x k, column-major) -> planes P[4][mp][kp] (re hi, re lo, im hi,
+// im lo; row-major, k contiguous) scaled by 2^e per row, rfac[r] = 2^-e.
+// Rows in [m, mp) and k in [k, kp) are zero.  Block of 32 rows; thread
+// (row = t % 32, chunk = t / 32) holds 32 consecutive k of its row.
+__global__ __launch_bounds__(256) void k_split_rows(const c2 *__restrict__ A, int64_t lda, int m, int k, int kp,
+                                                    int64_t mp, _Float16 *P, float *rfac)
+{
+  __shared__ float smax[8][32];
+  const int row = threadIdx.x & 31, kc = threadIdx.x >> 5;
+  const int r = blockIdx.x * 32 + row;
+  float re[32], im[32];
+  float mx = 0.0f;
+#pragma unroll
+  for (int j = 0; j < 32; j++) {
+    const int kk = kc * 32 + j;
+    c2 v = {0.0f, 0.0f};
+    if (r < m && kk < k) v = A[(size_t)kk * lda + r];
+    re[j] = v.re;
+    im[j] = v.im;
+    mx = fmaxf(mx, fmaxf(fabsf(v.re), fabsf(v.im)));
+  }
+  smax[kc][row] = mx;
+  __syncthreads();
+  mx = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 8; i++) mx = fmaxf(mx, smax[i][row]);
+  const int e = split_exp(mx);
+  if (kc == 0 && r < m) rfac[r] = ldexpf(1.0f, -e);
+  if (kc * 32 >= kp) return;
+  const float s = ldexpf(1.0f, e);
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    h8 rh, rl, ih, il;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      _Float16 a, b;
+      split_one(re[q * 8 + j] * s, a, b);
+      rh[j] = a; rl[j] = b;
+      split_one(im[q * 8 + j] * s, a, b);
+      ih[j] = a; il[j] = b;
+    }
+    const size_t o = (size_t)r * kp + kc * 32 + q * 8;
+    *(h8 *)(P + o) = rh;
+    *(h8 *)(P + (size_t)mp * kp + o) = rl;
+    *(h8 *)(P + 2 * (size_t)mp * kp + o) = ih;
+    *(h8 *)(P + 3 * (size_t)mp * kp + o) = il;
+  }
+}
+
+// U12 (k x n, column-major, k contiguous) -> planes P[4][np][kp] scaled by
+// 2^e per column, cfac[c] = 2^-e.  Thread (column = t / 8, chunk = t % 8)
+// holds 32 consecutive k of its column; the column maximum by shuffles.
+__global__ __launch_bounds__(256) void k_split_cols(const c2 *__restrict__ B, int64_t ldb, int k, int n, int kp,
+                                                    int64_t np, _Float16 *P, float *cfac)
+{
+  const int kc = threadIdx.x & 7;
+  const int c = blockIdx.x * 32 + (threadIdx.x >> 3);
+  float re[32], im[32];
+  float mx = 0.0f;
+#pragma unroll
+  for (int j = 0; j < 32; j++) {
+    const int kk = kc * 32 + j;
+    c2 v = {0.0f, 0.0f};
+    if (c < n && kk < k) v = B[(size_t)c * ldb + kk];
+    re[j] = v.re;
+    im[j] = v.im;
+    mx = fmaxf(mx, fmaxf(fabsf(v.re), fabsf(v.im)));
+  }
+  mx = fmaxf(mx, __shfl_xor(mx, 1));
+  mx = fmaxf(mx, __shfl_xor(mx, 2));
+  mx = fmaxf(mx, __shfl_xor(mx, 4));
+  const int e = split_exp(mx);
+  if (kc == 0 && c < n) cfac[c] = ldexpf(1.0f, -e);
+  if (kc * 32 >= kp) return;
+  const float s = ldexpf(1.0f, e);
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    h8 rh, rl, ih, il;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      _Float16 a, b;
+      split_one(re[q * 8 + j] * s, a, b);
+      rh[j] = a; rl[j] = b;
+      split_one(im[q * 8 + j] * s, a, b);
+      ih[j] = a; il[j] = b;
+    }
+    const size_t o = (size_t)c * kp + kc * 32 + q * 8;
+    *(h8 *)(P + o) = rh;
+    *(h8 *)(P + (size_t)np * kp + o) = rl;
+    *(h8 *)(P + 2 * (size_t)np * kp + o) = ih;
+    *(h8 *)(P + 3 * (size_t)np * kp + o) = il;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// C(m x n, complex64) -= A B (or C = A B) from the split planes.  Workgroup
+// tile 128 x 128 in 16 x 16 MFMA blocks; k staged 32 at a time through LDS
+// ([plane][row][32 halves], 16-byte chunks XOR-swizzled by row so that a
+// fragment read hits distinct banks), the next stage prefetched into
+// registers while the current one computes.  Fragment layout of v_mfma_f32_16x16x32_f16:
+// lane l supplies row l % 16 of its first operand and column l % 16 of its
+// second, 8 halves each at k offset 8 (l / 16) of the stage (any k order
+// common to both gives the same sum); the result block holds
+// D(4 (l / 16) + v, l % 16) in register v.  Here the first operand is B's
+// fragment (column c of the product), the second A's (row r), so D is the
+// product block transposed.
+//
+// Tile order: super-tiles of G x G tiles, dealt round-robin to the 8 XCDs
+// (the dispatcher deals workgroups to XCDs round-robin, so workgroup b runs
+// on XCD b % 8): XCD x works through super-tiles x, x + 8, ..., each one's
+// G^2 tiles together, so a super-tile's row slab of A and column slab of B
+// (G x 128 rows / columns, ~2 MB each at G = 8) stay in that XCD's L2 while
+// its CUs work through them.  Tiles outside the matrix exit at once.
+__device__ __forceinline__ bool super_tile(int bid, int tiles_m, int tiles_n, int G, int &ti, int &tj)
+{
+  const int xcd = bid & 7, l = bid >> 3, gg = G * G;
+  const int s = (l / gg) * 8 + xcd, w = l % gg;
+  const int sm = (tiles_m + G - 1) / G, sn = (tiles_n + G - 1) / G;
+  if (s >= sm * sn) return false;
+  ti = (s % sm) * G + w % G;
+  tj = (s / sm) * G + w / G;
+  return ti < tiles_m && tj < tiles_n;
+}
+
+__device__ __forceinline__ h8 hneg(h8 x)
+{
+  typedef unsigned u4 __attribute__((ext_vector_type(4)));
+  u4 v = __builtin_bit_cast(u4, x);
+  v ^= 0x80008000u;
+  return __builtin_bit_cast(h8, v);
+}
+
+// NW = 8 waves in 2 x 4 of 64 x 32 each, two LDS buffers (128 KB, one
+// barrier per stage), one workgroup of two waves per SIMD (180 VGPRs, no
+// spill).  Measured at 19,616 x 19,616 x 256: 3.96 ms (199 TF/s counted as
+// 8 mnk, 598 TF/s of FP16 products); four waves of 64 x 64 on one 64 KB
+// buffer with two workgroups per CU spilled and took 6.59 ms.
+template <int NW>
+__device__ __forceinline__ void gemm_hs_tile(int m, int n, int kp, const _Float16 *__restrict__ PA, int64_t mp,
+                                             const _Float16 *__restrict__ PB, int64_t np,
+                                             const float *__restrict__ rfac, const float *__restrict__ cfac, c2 *C,
+                                             int64_t ldc, int row0, int col0, int store,
+                                             _Float16 (*As)[4][HT][HK], _Float16 (*Bs)[4][HT][HK])
+{
+  constexpr bool DB = NW == 8;
+  constexpr int CB = NW == 4 ? 4 : 2;           // 16-column blocks per wave
+  constexpr int NT = 64 * NW;
+  constexpr int NL = 2048 / NT;                 // 16-byte chunks per thread per operand and stage
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave / (NW / 2), wc = wave % (NW / 2);
+  const int l16 = lane & 15, lq = lane >> 4;
+  const size_t pa = (size_t)mp * kp, pb = (size_t)np * kp;
+
+  h8 ga[NL], gb[NL];
+  auto gload = [&](int s) {
+#pragma unroll
+    for (int i = 0; i < NL; i++) {
+      const int q = tid + NT * i;
+      const int p = q >> 9, r = (q >> 2) & 127, ch = q & 3;
+      ga[i] = *(const h8 *)(PA + p * pa + (size_t)(row0 + r) * kp + s * HK + ch * 8);
+      gb[i] = *(const h8 *)(PB + p * pb + (size_t)(col0 + r) * kp + s * HK + ch * 8);
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < NL; i++) {
+      const int q = tid + NT * i;
+      const int p = q >> 9, r = (q >> 2) & 127, ch = q & 3;
+      const int sw = ch ^ ((r >> 2) & 3);
+      *(h8 *)&As[buf][p][r][sw * 8] = ga[i];
+      *(h8 *)&Bs[buf][p][r][sw * 8] = gb[i];
+    }
+  };
+
+  f4 ar[4][CB], ai[4][CB];
+#pragma unroll
+  for (int x = 0; x < 4; x++)
+#pragma unroll
+    for (int y = 0; y < CB; y++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) ar[x][y][v] = ai[x][y][v] = 0.0f;
+
+  const int nst = kp / HK;
+  gload(0);
+  if (DB) {
+    sstore(0);
+    __syncthreads();
+  }
+  for (int s = 0; s < nst; s++) {
+    const int buf = DB ? (s & 1) : 0;
+    if (!DB) {
+      sstore(0);
+      __syncthreads();
+    }
+    if (s + 1 < nst) gload(s + 1);
+    h8 b[CB][4];
+#pragma unroll
+    for (int cb = 0; cb < CB; cb++) {
+      const int c = wc * (16 * CB) + cb * 16 + l16;
+      const int sw = lq ^ ((c >> 2) & 3);
+#pragma unroll
+      for (int p = 0; p < 4; p++) b[cb][p] = *(const h8 *)&Bs[buf][p][c][sw * 8];
+    }
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++) {
+      const int r = wr * 64 + rb * 16 + l16;
+      const int sw = lq ^ ((r >> 2) & 3);
+      h8 a[4];
+#pragma unroll
+      for (int p = 0; p < 4; p++) a[p] = *(const h8 *)&As[buf][p][r][sw * 8];
+      const h8 n2 = hneg(a[2]), n3 = hneg(a[3]);
+      // Re += Ar Br - Ai Bi ; Im += Ar Bi + Ai Br, each real product as
+      // hi*hi + hi*lo + lo*hi; alternate accumulators between dependent MFMAs
+#pragma unroll
+      for (int cb = 0; cb < CB; cb++) {
+        ar[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b[cb][0], a[0], ar[rb][cb], 0, 0, 0);
+        ai[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b[cb][2], a[0], ai[rb][cb], 0, 0, 0);
+      }
+#pragma unroll
+      for (int cb = 0; cb < CB; cb++) {
+        ar[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b[cb][1], a[0], ar[rb][cb], 0, 0, 0);
+        ai[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b[cb][3], a[0], ai[rb][cb], 0, 0, 0);
+      }
+#pragma unroll
+      for (int cb = 0; cb < CB; cb++) {
+        ar[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b[cb][0], a[1], ar[rb][cb], 0, 0, 0);
+        ai[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b[cb][2], a[1], ai[rb][cb], 0, 0, 0);
+      }
+#pragma unroll
+      for (int cb = 0; cb < CB; cb++) {
+        ar[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b[cb][2], n2, ar[rb][cb], 0, 0, 0);
+        ai[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b[cb][0], a[2], ai[rb][cb], 0, 0, 0);
+      }
+#pragma unroll
+      for (int cb = 0; cb < CB; cb++) {
+        ar[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b[cb][3], n2, ar[rb][cb], 0, 0, 0);
+        ai[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b[cb][1], a[2], ai[rb][cb], 0, 0, 0);
+      }
+#pragma unroll
+      for (int cb = 0; cb < CB; cb++) {
+        ar[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b[cb][2], n3, ar[rb][cb], 0, 0, 0);
+        ai[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(b[cb][0], a[3], ai[rb][cb], 0, 0, 0);
+      }
+    }
+    if (DB && s + 1 < nst) sstore(buf ^ 1);
+    __syncthreads();
+  }
+  // C -= (A B) 2^-(e_row + e_col), or C = (A B) 2^-(e_row + e_col) (store).
+  // The accumulators hold the product transposed (B fragment as the MFMA's
+  // first operand): register v of block (rb, cb) is row l16, column
+  // 4 lq + v, so 16 lanes cover 16 consecutive rows of one column -- 128
+  // contiguous bytes per column and instruction.  All loads of C are issued
+  // before the first store.
+  float rf[4], cf[CB][4];
+#pragma unroll
+  for (int rb = 0; rb < 4; rb++) {
+    const int row = row0 + wr * 64 + rb * 16 + l16;
+    rf[rb] = row < m ? rfac[row] : 0.0f;
+  }
+#pragma unroll
+  for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int col = col0 + wc * (16 * CB) + cb * 16 + 4 * lq + v;
+      cf[cb][v] = col < n ? cfac[col] : 0.0f;
+    }
+  c2 x[4][CB][4];
+  if (!store) {
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+      for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int row = row0 + wr * 64 + rb * 16 + l16;
+          const int col = col0 + wc * (16 * CB) + cb * 16 + 4 * lq + v;
+          x[rb][cb][v] = (row < m && col < n) ? C[(size_t)col * ldc + row] : c2{0.0f, 0.0f};
+        }
+  }
+#pragma unroll
+  for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+    for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int row = row0 + wr * 64 + rb * 16 + l16;
+        const int col = col0 + wc * (16 * CB) + cb * 16 + 4 * lq + v;
+        if (row < m && col < n) {
+          const float f = rf[rb] * cf[cb][v];
+          c2 y = x[rb][cb][v];
+          if (store) {
+            y.re = ar[rb][cb][v] * f;
+            y.im = ai[rb][cb][v] * f;
+          } else {
+            y.re -= ar[rb][cb][v] * f;
+            y.im -= ai[rb][cb][v] * f;
+          }
+          C[(size_t)col * ldc + row] = y;
+        }
+      }
+}
+
+__global__ __launch_bounds__(512, 1) void k_gemm_hs(int m, int n, int kp, const _Float16 *__restrict__ PA, int64_t mp,
+                                                     const _Float16 *__restrict__ PB, int64_t np,
+                                                     const float *__restrict__ rfac, const float *__restrict__ cfac,
+                                                     c2 *C, int64_t ldc, int tiles_m, int tiles_n, int store, int G)
+{
+  __shared__ __align__(16) _Float16 As[2][4][HT][HK];
+  __shared__ __align__(16) _Float16 Bs[2][4][HT][HK];
+  int ti, tj;
+  if (!super_tile(blockIdx.x, tiles_m, tiles_n, G, ti, tj)) return;
+  gemm_hs_tile<8>(m, n, kp, PA, mp, PB, np, rfac, cfac, C, ldc, ti * HT, tj * HT, store, As, Bs);
+}
+
+// ---------------------------------------------------------------------------
+// Partial-pivoting factorisation of an m x w leaf (m <= 256, w <= 32) by one
+// workgroup of 256 threads, thread t holding row t of the leaf in registers.
+// Per column c: the candidate |re| + |im| of each row >= c, the argmax
+// (first row among equals, the izamax rule) by shuffles within each wave and
+// across the four waves through LDS; rows c and p trade places through LDS,
+// which also hands the pivot row to every thread; the rows below c scale by
+// the pivot and update their remaining columns.  Two barriers per column
+// (the LDS slots alternate between even and odd columns).  A zero pivot is
+// recorded (first one, 1-based global column) and the column is left as it
+// is (nothing below it is nonzero), as LAPACK's zgetf2 does.  Pivots are
+// written absolute (piv_off + local row).
+#define LEAF_W 32
+#define LEAF_M 256
+// one column of k_leaf_lu (C compile-time, so that x stays in registers)
+template <int C>
+__device__ __forceinline__ void leaf_step(c2 (&x)[LEAF_W], int t, int lane, int wv, int m, int *ipiv, int piv_off,
+                                          int *flags, float *sbest, int *srow, c2 *prow, c2 *crow)
+{
+  float best = (t >= C && t < m) ? fabsf(x[C].re) + fabsf(x[C].im) : -1.0f;
+  int br = t;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const float ob = __shfl_xor(best, off);
+    const int orr = __shfl_xor(br, off);
+    if (ob > best || (ob == best && orr < br)) { best = ob; br = orr; }
+  }
+  if (lane == 0) {
+    sbest[wv] = best;
+    srow[wv] = br;
+  }
+  __syncthreads();
+  best = sbest[0];
+  br = srow[0];
+#pragma unroll
+  for (int i = 1; i < 4; i++) {
+    const float ob = sbest[i];
+    const int orr = srow[i];
+    if (ob > best || (ob == best && orr < br)) { best = ob; br = orr; }
+  }
+  const int p = br;
+  if (t == 0) ipiv[C] = piv_off + p;
+  if (!(best > 0.0f)) {
+    if (t == 0 && flags[1] == 0) flags[1] = piv_off + C + 1;
+    return;
+  }
+  if (t == p)
+#pragma unroll
+    for (int j = 0; j < LEAF_W; j++) prow[j] = x[j];
+  if (t == C)
+#pragma unroll
+    for (int j = 0; j < LEAF_W; j++) crow[j] = x[j];
+  __syncthreads();
+  if (t == C) {
+#pragma unroll
+    for (int j = 0; j < LEAF_W; j++) x[j] = prow[j];
+  } else if (t == p) {
+#pragma unroll
+    for (int j = 0; j < LEAF_W; j++) x[j] = crow[j];
+  }
+  if (t > C && t < m) {
+    const c2 d = prow[C];
+    const float dd = d.re * d.re + d.im * d.im;
+    const c2 inv = {d.re / dd, -d.im / dd};
+    const c2 a = x[C];
+    const c2 l = {a.re * inv.re - a.im * inv.im, a.re * inv.im + a.im * inv.re};
+    x[C] = l;
+#pragma unroll
+    for (int j = C + 1; j < LEAF_W; j++) {
+      const c2 u = prow[j];
+      x[j].re -= l.re * u.re - l.im * u.im;
+      x[j].im -= l.re * u.im + l.im * u.re;
+    }
+  }
+}
+
+// columns C .. w-1 (the LDS slots alternate between even and odd columns)
+template <int C>
+__device__ __forceinline__ void leaf_steps(c2 (&x)[LEAF_W], int t, int lane, int wv, int m, int w, int *ipiv,
+                                           int piv_off, int *flags, float (*sbest)[4], int (*srow)[4],
+                                           c2 (*prow)[LEAF_W], c2 (*crow)[LEAF_W])
+{
+  if constexpr (C < LEAF_W) {
+    if (C >= w) return;
+    leaf_step<C>(x, t, lane, wv, m, ipiv, piv_off, flags, sbest[C & 1], srow[C & 1], prow[C & 1], crow[C & 1]);
+    leaf_steps<C + 1>(x, t, lane, wv, m, w, ipiv, piv_off, flags, sbest, srow, prow, crow);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_leaf_lu(c2 *A, int64_t lda, int m, int w, int *ipiv, int piv_off,
+                                                 int *flags)
+{
+  __shared__ float sbest[2][4];
+  __shared__ int srow[2][4];
+  __shared__ c2 prow[2][LEAF_W], crow[2][LEAF_W];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  c2 x[LEAF_W];
+#pragma unroll
+  for (int j = 0; j < LEAF_W; j++) x[j] = (t < m && j < w) ? A[(size_t)j * lda + t] : c2{0.0f, 0.0f};
+  leaf_steps<0>(x, t, lane, wv, m, w, ipiv, piv_off, flags, sbest, srow, prow, crow);
+  if (t < m)
+#pragma unroll
+    for (int j = 0; j < LEAF_W; j++)
+      if (j < w) A[(size_t)j * lda + t] = x[j];
+}
+
+// X = I (n x n, leading dimension n)
+__global__ void k_eye(c2 *X, int n)
+{
+  const int r = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+  if (r < n) X[(size_t)c * n + r] = {r == c ? 1.0f : 0.0f, 0.0f};
+}
+
+// ---------------------------------------------------------------------------
+// Split planes and scale factors of one product's operands
+struct SplitWS {
+  _Float16 *PA = nullptr, *PB = nullptr;
+  float *rf = nullptr, *cf = nullptr;
+  int64_t mp = 0;                  // rows of each plane (padded to the GEMM tile)
+};
+
+// per-device workspace: two sets of split planes (the trailing update's on
+// the caller's stream, the block column's L21 product on the look-ahead
+// stream), U11^-1, the look-ahead stream and its events
+struct LocalWork {
+  void *mem = nullptr;
+  size_t bytes = 0;
+  hipStream_t side = nullptr;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+};
+LocalWork g_lw[64];
+
+size_t al256(size_t x) { return (x + 255) / 256 * 256; }
+
+int local_work(int n, SplitWS *bulk, SplitWS *pan, c2 **Uinv, LocalWork **out)
+{
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(BFA_ERR_ARG, "device index %d unsupported", dev);
+  const int64_t p = ((int64_t)n + HT - 1) / HT * HT;
+  const size_t planes = al256((size_t)4 * p * LB * sizeof(_Float16)), fac = al256((size_t)p * 4);
+  const size_t set = 2 * planes + 2 * fac;
+  const size_t need = 2 * set + al256((size_t)LB * LB * sizeof(c2));
+  LocalWork &w = g_lw[dev];
+  if (w.bytes < need) {
+    if (w.mem) {
+      HIPCHECK(hipDeviceSynchronize());
+      HIPCHECK(hipFree(w.mem));
+      w.mem = nullptr;
+      w.bytes = 0;
+    }
+    HIPCHECK(hipMalloc(&w.mem, need));
+    w.bytes = need;
+  }
+  if (!w.side) {
+    HIPCHECK(hipStreamCreateWithFlags(&w.side, hipStreamNonBlocking));
+    for (auto &e : w.ev) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  char *b = (char *)w.mem;
+  for (SplitWS *s : {bulk, pan}) {
+    s->PA = (_Float16 *)b;
+    s->PB = (_Float16 *)(b + planes);
+    s->rf = (float *)(b + 2 * planes);
+    s->cf = (float *)(b + 2 * planes + fac);
+    s->mp = p;
+    b += set;
+  }
+  *Uinv = (c2 *)b;
+  *out = &w;
+  return BFA_OK;
+}
+
+// HIP events around the split GEMM launches while bfa_profile_enable is on
+struct SplitProfile {
+  std::vector<hipEvent_t> pool;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> rec;
+  double flops = 0.0;
+  size_t used = 0;
+};
+SplitProfile g_sp;
+
+hipEvent_t sp_event(hipStream_t st)
+{
+  if (g_sp.used >= g_sp.pool.size()) {
+    for (int i = 0; i < 256; i++) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) == hipSuccess) g_sp.pool.push_back(e);
+    }
+  }
+  if (g_sp.used >= g_sp.pool.size()) return nullptr;
+  hipEvent_t e = g_sp.pool[g_sp.used++];
+  (void)hipEventRecord(e, st);
+  return e;
+}
+
+// super-tile edge G (8, or smaller while fewer than 16 super-tiles would
+// leave XCDs idle) and the padded grid of super_tile()
+void super_grid(int tm, int tn, int *G, size_t *nblk)
+{
+  int g = 8;
+  while (g > 1 && (size_t)((tm + g - 1) / g) * ((tn + g - 1) / g) < 16) g /= 2;
+  const size_t ns = (size_t)((tm + g - 1) / g) * ((tn + g - 1) / g);
+  *G = g;
+  *nblk = (ns + 7) / 8 * 8 * g * g;
+}
+
+// split planes of A (m x k) by rows and B (k x n) by columns, k <= 256
+void split_ops(int m, int n, int k, const c2 *A, int64_t lda, const c2 *B, int64_t ldb, const SplitWS &w,
+               hipStream_t st)
+{
+  const int kp = (k + HK - 1) / HK * HK;
+  if (m > 0) hipLaunchKernelGGL(k_split_rows, dim3((unsigned)(w.mp / 32)), dim3(256), 0, st, A, lda, m, k, kp, w.mp,
+                                w.PA, w.rf);
+  if (n > 0) hipLaunchKernelGGL(k_split_cols, dim3((unsigned)(w.mp / 32)), dim3(256), 0, st, B, ldb, k, n, kp, w.mp,
+                                w.PB, w.cf);
+}
+
+// C(m x n) -= A B (store: C = A B) from split planes, the product's columns
+// starting at column c0 of the split B
+void gemm_planes(int m, int n, int k, c2 *C, int64_t ldc, const SplitWS &w, int c0, hipStream_t st, int store)
+{
+  if (m <= 0 || n <= 0 || k <= 0) return;
+  const int kp = (k + HK - 1) / HK * HK;
+  const int tm = (m + HT - 1) / HT, tn = (n + HT - 1) / HT;
+  const _Float16 *PB = w.PB + (size_t)c0 * kp;
+  const float *cf = w.cf + c0;
+  const bool prof = bfa::profile_on();
+  hipEvent_t e0 = prof ? sp_event(st) : nullptr;
+  int G = 1;
+  size_t nblk = 0;
+  super_grid(tm, tn, &G, &nblk);
+  hipLaunchKernelGGL(k_gemm_hs, dim3((unsigned)nblk), dim3(512), 0, st, m, n, kp,
+                     (const _Float16 *)w.PA, w.mp, PB, w.mp, (const float *)w.rf, cf, C, ldc, tm, tn, store, G);
+  if (prof) {
+    hipEvent_t e1 = sp_event(st);
+    if (e0 && e1) {
+      g_sp.rec.push_back({e0, e1});
+      g_sp.flops += 8.0 * m * (double)n * k;
+    }
+  }
+}
+
+// m x w panel with partial pivoting over its m rows (m <= 256): the panel
+// recursion of the partial-pivoting LU (left half, swaps and unit-lower TRSM
+// into the right half, its update, right half, swaps back into the left)
+// with the single-wave leaf kernel at width <= 32
+int panel_local(c2 *A, int m, int w, int64_t lda, int *ipiv, int piv_off, int *flags, hipStream_t st)
+{
+  if (w <= LEAF_W) {
+    hipLaunchKernelGGL(k_leaf_lu, dim3(1), dim3(LEAF_M), 0, st, A, lda, m, w, ipiv, piv_off, flags);
+    return BFA_OK;
+  }
+  const int w1 = w / 2, w2 = w - w1;
+  int rc = panel_local(A, m, w1, lda, ipiv, piv_off, flags, st);
+  if (rc) return rc;
+  c2 *Ar = A + (size_t)w1 * lda;
+  bfa::laswp_c(Ar - piv_off, lda, 0, w2, piv_off, w1, ipiv - piv_off, st);
+  bfa::trsm_lower_c(w1, w2, A, lda, Ar, lda, st);
+  rc = bfa::gemm_sub_c(m - w1, w2, w1, A + w1, lda, Ar, lda, Ar + w1, lda, st);
+  if (rc) return rc;
+  rc = panel_local(A + (size_t)w1 * lda + w1, m - w1, w2, lda, ipiv + w1, piv_off + w1, flags, st);
+  if (rc) return rc;
+  bfa::laswp_c(A - piv_off, lda, 0, w1, piv_off + w1, w2, ipiv - piv_off, st);
+  return BFA_OK;
+}
+
+// step j's block column: the jb x jb diagonal block with partial pivoting
+// over its rows, then L21 = A21 U11^-1 (U11^-1 by the one-launch upper TRSM
+// on the identity, the product on the matrix cores stored over A21)
+int block_column(c2 *A, int n, int64_t lda, int j, int jb, int *ipiv, int *flags, const SplitWS &w, c2 *Uinv,
+                 hipStream_t st)
+{
+  c2 *Ajj = A + (size_t)j * lda + j;
+  int rc = panel_local(Ajj, jb, jb, lda, ipiv + j, j, flags, st);
+  if (rc) return rc;
+  const int below = n - j - jb;
+  if (below <= 0) return BFA_OK;
+  hipLaunchKernelGGL(k_eye, dim3((jb + 255) / 256, jb), dim3(256), 0, st, Uinv, jb);
+  bfa::trsm_upper_c(jb, jb, Ajj, lda, Uinv, jb, st);
+  split_ops(below, jb, jb, Ajj + jb, lda, Uinv, jb, w, st);
+  gemm_planes(below, jb, jb, Ajj + jb, lda, w, 0, st, 1);
+  return BFA_OK;
+}
+
+}  // namespace
+
+void bfa::local_release()
+{
+  for (int d = 0; d < 64; d++) {
+    LocalWork &w = g_lw[d];
+    if (!w.mem && !w.side) continue;
+    if (hipSetDevice(d) == hipSuccess) {
+      (void)hipDeviceSynchronize();
+      if (w.mem) (void)hipFree(w.mem);
+      if (w.side) (void)hipStreamDestroy(w.side);
+      for (auto &e : w.ev) if (e) (void)hipEventDestroy(e);
+    }
+    w = LocalWork();
+  }
+  for (hipEvent_t e : g_sp.pool) (void)hipEventDestroy(e);
+  g_sp = SplitProfile();
+}
+
+int bfa::profile_read_split(double *ms, double *flops, int *launches)
+{
+  double t = 0.0;
+  for (auto &pr : g_sp.rec) {
+    HIPCHECK(hipEventSynchronize(pr.second));
+    float x = 0.0f;
+    HIPCHECK(hipEventElapsedTime(&x, pr.first, pr.second));
+    t += x;
+  }
+  if (ms) *ms = t;
+  if (flops) *flops = g_sp.flops;
+  if (launches) *launches = (int)g_sp.rec.size();
+  g_sp.rec.clear();
+  g_sp.flops = 0.0;
+  g_sp.used = 0;
+  return BFA_OK;
+}
+
+// ---------------------------------------------------------------------------
+extern "C" int bfa_cgemm_sub_split(int m, int n, int k, const void *d_A, int64_t lda, const void *d_B, int64_t ldb,
+                                   void *d_C, int64_t ldc, void *stream)
+{
+  if (m < 0 || n < 0 || k < 0 || k > LB || (m > 0 && (lda < m || ldc < m)) || (k > 0 && ldb < k))
+    return fail(BFA_ERR_ARG, "bfa_cgemm_sub_split: bad sizes (k <= %d)", LB);
+  if (m == 0 || n == 0 || k == 0) return BFA_OK;
+  SplitWS bulk, pan;
+  c2 *Uinv;
+  LocalWork *lw;
+  int rc = local_work(m > n ? m : n, &bulk, &pan, &Uinv, &lw);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  split_ops(m, n, k, (const c2 *)d_A, lda, (const c2 *)d_B, ldb, bulk, st);
+  gemm_planes(m, n, k, (c2 *)d_C, ldc, bulk, 0, st, 0);
+  HIPCHECK(hipGetLastError());
+  return BFA_OK;
+}
+
+// Look-ahead: the caller's stream st carries, per step, the row swaps, the
+// U12 TRSM, the splits and the trailing update -- the next block column
+// first; the side stream then factors that block column (diagonal block,
+// L21) while st updates the remaining columns.  The two touch disjoint
+// columns (the side stream only the next block column, st everything right
+// of it) and each has its own split planes.
+extern "C" int bfa_lu_factor_c_local(void *d_A, int n, int64_t lda, int *d_ipiv, int *info, void *stream)
+{
+  if (!d_A || !d_ipiv || n <= 0 || lda < n) return fail(BFA_ERR_ARG, "bfa_lu_factor_c_local: bad argument");
+  hipStream_t st = (hipStream_t)stream;
+  c2 *A = (c2 *)d_A;
+  SplitWS bulk, pan;
+  c2 *Uinv;
+  LocalWork *lw;
+  int rc = local_work(n, &bulk, &pan, &Uinv, &lw);
+  if (rc) return rc;
+  rc = bfa::lu_local_begin(st);
+  if (rc) return rc;
+  int *flags = bfa::lu_local_flags();
+  if (!flags) return fail(BFA_ERR_STATE, "bfa_lu_factor_c_local: LU work area");
+  hipStream_t sp = lw->side;
+  hipEvent_t ev_in = lw->ev[0], ev_col = lw->ev[1], ev_next = lw->ev[2];
+  // the side stream starts after everything queued on st (the matrix itself)
+  HIPCHECK(hipEventRecord(ev_in, st));
+  HIPCHECK(hipStreamWaitEvent(sp, ev_in, 0));
+  rc = block_column(A, n, lda, 0, n < LB ? n : LB, d_ipiv, flags, pan, Uinv, sp);
+  if (rc) return rc;
+  HIPCHECK(hipEventRecord(ev_col, sp));
+  for (int j = 0; j < n; j += LB) {
+    const int jb = n - j < LB ? n - j : LB;
+    c2 *Ajj = A + (size_t)j * lda + j;
+    HIPCHECK(hipStreamWaitEvent(st, ev_col, 0));         // block column j factored
+    bfa::laswp_c(A, lda, 0, j, j, jb, d_ipiv, st);
+    const int rest = n - j - jb;
+    if (rest <= 0) break;
+    bfa::laswp_c(A, lda, j + jb, n, j, jb, d_ipiv, st);
+    c2 *A12 = A + (size_t)(j + jb) * lda + j;
+    bfa::trsm_lower_c(jb, rest, Ajj, lda, A12, lda, st);
+    split_ops(rest, rest, jb, Ajj + jb, lda, A12, lda, bulk, st);
+    const int nb2 = rest < LB ? rest : LB;
+    gemm_planes(rest, nb2, jb, A12 + jb, lda, bulk, 0, st, 0);   // the next block column
+    HIPCHECK(hipEventRecord(ev_next, st));
+    HIPCHECK(hipStreamWaitEvent(sp, ev_next, 0));
+    rc = block_column(A, n, lda, j + jb, nb2, d_ipiv, flags, pan, Uinv, sp);
+    if (rc) return rc;
+    HIPCHECK(hipEventRecord(ev_col, sp));
+    gemm_planes(rest, rest - nb2, jb, A12 + (size_t)nb2 * lda + jb, lda, bulk, nb2, st, 0);
+  }
+  HIPCHECK(hipEventRecord(ev_col, sp));
+  HIPCHECK(hipStreamWaitEvent(st, ev_col, 0));
+  HIPCHECK(hipGetLastError());
+  return bfa::lu_local_end(info, st);
+}

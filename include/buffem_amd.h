@@ -224,6 +224,19 @@ int  bfa_convert_z2c(const void *d_A64, int m, int n, int64_t lda, void *d_A32,
 int  bfa_lu_factor_c(void *d_A, int n, int64_t lda, int *d_ipiv, int *info, void *stream);
 int  bfa_lu_solve_c(const void *d_A, int n, int64_t lda, const int *d_ipiv,
                     void *d_B, int nrhs, int64_t ldb, void *stream);
+/* complex64 LU with pivoting restricted to each 256-row diagonal block and
+ * the trailing updates in split FP16 on the matrix cores (each FP32 operand,
+ * scaled by a power of two per row / column, as hi + lo FP16 halves; three
+ * FP16 products per real product, FP32 accumulation): the preconditioner of
+ * the fast refinement form (bfa_solve_refine with these factors).  Not a
+ * partial-pivoting LU: meant for matrices that factor stably without
+ * pivoting across blocks (the VIE matrices do), with FP64 refinement
+ * restoring the accuracy.  d_ipiv and info as bfa_lu_factor_c.             */
+int  bfa_lu_factor_c_local(void *d_A, int n, int64_t lda, int *d_ipiv, int *info, void *stream);
+/* C(m x n) -= A(m x k) * B(k x n), complex64, k <= 256, by the split-FP16
+ * products of bfa_lu_factor_c_local's trailing update.                     */
+int  bfa_cgemm_sub_split(int m, int n, int k, const void *d_A, int64_t lda, const void *d_B,
+                         int64_t ldb, void *d_C, int64_t ldc, void *stream);
 /* Solve M64 x = b with the complex64 LU factors of M64 and FP64 iterative
  * refinement (residual b - M64 x in FP64 against the resident FP64 matrix)
  * until max_j ||r_j||/||b_j|| <= tol or maxit iterations.  d_B: RHS in,
@@ -359,9 +372,17 @@ int  bfa_surr_params(bfa_surrogate *s, float *out, const float *in);
  * which = 0 (complex128) or 1 (complex64), all launches; 2 / 3 the large
  * updates (m, n >= 256) on all CUs or, in the CU-partitioned LU, on the
  * update CUs; 4 / 5 the large updates run as column slices on the panel
- * CUs; reading 0 / 1 resets that type.                                    */
+ * CUs; reading 0 / 1 resets that type; 6 = the split-FP16 trailing updates
+ * of bfa_lu_factor_c_local (flops counted as 8mnk; reading resets).        */
 int  bfa_profile_enable(int on);
 int  bfa_profile_read(int which, double *ms, double *flops, int *launches);
+
+/* Release the library's process-lifetime device objects (LU work areas,
+ * the look-ahead's streams and events, instrumentation events, library
+ * handles) on every device; call before the process tears down its GPU
+ * runtime (the Python binding does so at exit).  Later calls re-create what
+ * they need.  No reference counterpart.                                    */
+int  bfa_shutdown(void);
 
 /* ---- device memory helpers (for hosts without a tensor library) -------- */
 int  bfa_device_malloc(void **ptr, size_t bytes);
