@@ -26,12 +26,10 @@ int lu_solve_apply_c(const void *A, int n, int64_t lda, void *B, int nrhs, int64
 int lu_solve_check(const void *scratch, int n, hipStream_t st);
 
 // complex64 pieces shared with the block-local LU (bfa_lu_local.hip):
-// clear the panel flags; partial-pivoting factorisation of an m x w panel
-// (pivots absolute, offset piv_off); read the flags back (timeout -> error,
-// first zero pivot -> info); row interchanges k0..k0+kb-1 on columns
-// [c0, c1); B(kb x n) := L^-1 B with L unit lower
+// clear the LU's device flags; read them back (timeout -> error, first zero
+// pivot -> info); row interchanges k0..k0+kb-1 on columns [c0, c1);
+// B(kb x n) := L^-1 B with L unit lower
 int lu_local_begin(hipStream_t st);
-int lu_local_panel_c(void *A, int m, int w, int64_t lda, int *ipiv, int piv_off, hipStream_t st);
 int lu_local_end(int *info, hipStream_t st);
 void laswp_c(void *A, int64_t lda, int c0, int c1, int k0, int kb, const int *ipiv, hipStream_t st);
 void trsm_lower_c(int kb, int n, const void *L, int64_t ldl, void *B, int64_t ldb, hipStream_t st);

@@ -1830,13 +1830,6 @@ int lu_local_begin(hipStream_t st)
   HIPCHECK(hipMemsetAsync(lw->pw.flags, 0, 8, st));
   return BFA_OK;
 }
-int lu_local_panel_c(void *A, int m, int w, int64_t lda, int *ipiv, int piv_off, hipStream_t st)
-{
-  LUWork *lw = nullptr;
-  int rc = lu_work(lw);
-  if (rc) return rc;
-  return panel_rec<float>((cplx<float> *)A, m, w, lda, ipiv, piv_off, lw, st, lw->num_cu);
-}
 int lu_local_end(int *info, hipStream_t st)
 {
   LUWork *lw = nullptr;

@@ -15,8 +15,8 @@
 //    in the other columns, forms U12 = L11^-1 A12 (unit-lower TRSM) and
 //    L21 = A21 U11^-1 (U11^-1 by a one-launch upper TRSM, then a product on
 //    the matrix cores) -- no synchronisation across workgroups anywhere.
-//    The diagonal block's own factorisation runs its 32-column leaves on one
-//    wavefront each.  The VIE matrices factor stably this way (growth ~1;
+//    The diagonal block's own factorisation is one launch of one workgroup
+//    (k_diag_lu).  The VIE matrices factor stably this way (growth ~1;
 //    partial pivoting leaves >99.5 % of the rows in place on them).
 //  - The trailing update A22 -= L21 U12 in split FP16: every FP32 operand x
 //    (scaled by a power of two per row of L21 / column of U12 so that the
@@ -30,6 +30,7 @@
 //    maximum absolutely.
 #include <hip/hip_runtime.h>
 #include <cmath>
+#include <cstdlib>
 #include <cstdint>
 #include <utility>
 #include <vector>
@@ -386,109 +387,235 @@ __global__ __launch_bounds__(512, 1) void k_gemm_hs(int m, int n, int kp, const 
 }
 
 // ---------------------------------------------------------------------------
-// Partial-pivoting factorisation of an m x w leaf (m <= 256, w <= 32) by one
-// workgroup of 256 threads, thread t holding row t of the leaf in registers.
-// Per column c: the candidate |re| + |im| of each row >= c, the argmax
-// (first row among equals, the izamax rule) by shuffles within each wave and
-// across the four waves through LDS; rows c and p trade places through LDS,
-// which also hands the pivot row to every thread; the rows below c scale by
-// the pivot and update their remaining columns.  Two barriers per column
-// (the LDS slots alternate between even and odd columns).  A zero pivot is
-// recorded (first one, 1-based global column) and the column is left as it
-// is (nothing below it is nonzero), as LAPACK's zgetf2 does.  Pivots are
-// written absolute (piv_off + local row).
+// The jb x jb diagonal block (jb <= 256) with partial pivoting over its rows
+// in one workgroup of 1024 threads.  The block stays in global memory (512
+// KB, L2-resident); panels of 32 columns, each:
+//   leaf    rows c0..jb-1 of the panel.  Thread t keeps the panel part of
+//           row t (its data row) in registers, shifted left by one column
+//           per step so that the current column is always x[0] (the loop
+//           body is the same for every column and stays small), and tracks
+//           the row's position: an interchange only swaps two positions.
+//           Per column d: the candidate |re| + |im| of each row at a
+//           position >= d as the key (bits of |re| + |im|) << 32 | ~position,
+//           whose maximum -- DPP within each wave, then the four rows' waves
+//           through LDS -- is the izamax pivot (largest magnitude, first
+//           position among equals); the pivot row's values go to every
+//           thread through LDS, the rows below scale and update.  A zero pivot is recorded (first one,
+//           1-based global column) and the column is left as it is, as
+//           LAPACK's zgetf2 does.  Each finished column value goes to LDS by
+//           data row; at the end every row lands at its final position in A
+//           and in LDS (L11 on top of L21), and perm[position] = data row;
+//   swaps   that permutation applied to the block's other columns (all
+//           loads, a barrier, all stores, 14 columns per thread at a time);
+//   U12     L11^-1 A12 for the panel's rows, a thread per column, into A and
+//           LDS;
+//   update  A22 -= L21 U12 in 16 x 16 tiles on v_mfma_f32_16x16x4f32 (U12's
+//           value as the first operand, so that register v of lane l holds
+//           row l % 16, column 4 (l / 16) + v of the tile), a wave per tile
+//           in turn.
+// Barriers separate the phases; a workgroup barrier also orders the global
+// stores of one phase before the loads of the next.  Pivots are written
+// absolute (piv_off + local row).
 #define LEAF_W 32
-#define LEAF_M 256
-// one column of k_leaf_lu (C compile-time, so that x stays in registers)
-template <int C>
-__device__ __forceinline__ void leaf_step(c2 (&x)[LEAF_W], int t, int lane, int wv, int m, int *ipiv, int piv_off,
-                                          int *flags, float *sbest, int *srow, c2 *prow, c2 *crow)
+#define DL_NT 1024
+#define DL_NW 16
+#define DL_M 256
+// One max-reduction step of a 64-bit key across lanes: DPP control CTRL on
+// the rows enabled by RM; a lane whose source is disabled or out of range
+// takes 0 (no candidate).
+template <int CTRL, int RM>
+__device__ __forceinline__ void key_max_step(unsigned &lo, unsigned &hi)
 {
-  float best = (t >= C && t < m) ? fabsf(x[C].re) + fabsf(x[C].im) : -1.0f;
-  int br = t;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const float ob = __shfl_xor(best, off);
-    const int orr = __shfl_xor(br, off);
-    if (ob > best || (ob == best && orr < br)) { best = ob; br = orr; }
-  }
-  if (lane == 0) {
-    sbest[wv] = best;
-    srow[wv] = br;
-  }
-  __syncthreads();
-  best = sbest[0];
-  br = srow[0];
-#pragma unroll
-  for (int i = 1; i < 4; i++) {
-    const float ob = sbest[i];
-    const int orr = srow[i];
-    if (ob > best || (ob == best && orr < br)) { best = ob; br = orr; }
-  }
-  const int p = br;
-  if (t == 0) ipiv[C] = piv_off + p;
-  if (!(best > 0.0f)) {
-    if (t == 0 && flags[1] == 0) flags[1] = piv_off + C + 1;
-    return;
-  }
-  if (t == p)
-#pragma unroll
-    for (int j = 0; j < LEAF_W; j++) prow[j] = x[j];
-  if (t == C)
-#pragma unroll
-    for (int j = 0; j < LEAF_W; j++) crow[j] = x[j];
-  __syncthreads();
-  if (t == C) {
-#pragma unroll
-    for (int j = 0; j < LEAF_W; j++) x[j] = prow[j];
-  } else if (t == p) {
-#pragma unroll
-    for (int j = 0; j < LEAF_W; j++) x[j] = crow[j];
-  }
-  if (t > C && t < m) {
-    const c2 d = prow[C];
-    const float dd = d.re * d.re + d.im * d.im;
-    const c2 inv = {d.re / dd, -d.im / dd};
-    const c2 a = x[C];
-    const c2 l = {a.re * inv.re - a.im * inv.im, a.re * inv.im + a.im * inv.re};
-    x[C] = l;
-#pragma unroll
-    for (int j = C + 1; j < LEAF_W; j++) {
-      const c2 u = prow[j];
-      x[j].re -= l.re * u.re - l.im * u.im;
-      x[j].im -= l.re * u.im + l.im * u.re;
-    }
+  const unsigned ol = __builtin_amdgcn_update_dpp(0u, lo, CTRL, RM, 0xF, false);
+  const unsigned oh = __builtin_amdgcn_update_dpp(0u, hi, CTRL, RM, 0xF, false);
+  if (oh > hi || (oh == hi && ol > lo)) {
+    hi = oh;
+    lo = ol;
   }
 }
 
-// columns C .. w-1 (the LDS slots alternate between even and odd columns)
-template <int C>
-__device__ __forceinline__ void leaf_steps(c2 (&x)[LEAF_W], int t, int lane, int wv, int m, int w, int *ipiv,
-                                           int piv_off, int *flags, float (*sbest)[4], int (*srow)[4],
-                                           c2 (*prow)[LEAF_W], c2 (*crow)[LEAF_W])
+// maximum of k over the wavefront (all lanes active), returned uniform:
+// quad_perm [1,0,3,2] and [2,3,0,1], row_half_mirror and row_mirror reduce
+// each row of 16 lanes; row_bcast:15 (rows 1, 3) and row_bcast:31 (rows 2,
+// 3) carry the rows' maxima up to lane 63
+__device__ __forceinline__ unsigned long long wave_max_key(unsigned long long k)
 {
-  if constexpr (C < LEAF_W) {
-    if (C >= w) return;
-    leaf_step<C>(x, t, lane, wv, m, ipiv, piv_off, flags, sbest[C & 1], srow[C & 1], prow[C & 1], crow[C & 1]);
-    leaf_steps<C + 1>(x, t, lane, wv, m, w, ipiv, piv_off, flags, sbest, srow, prow, crow);
-  }
+  unsigned lo = (unsigned)k, hi = (unsigned)(k >> 32);
+  key_max_step<0xB1, 0xF>(lo, hi);
+  key_max_step<0x4E, 0xF>(lo, hi);
+  key_max_step<0x141, 0xF>(lo, hi);
+  key_max_step<0x140, 0xF>(lo, hi);
+  key_max_step<0x142, 0xA>(lo, hi);
+  key_max_step<0x143, 0xC>(lo, hi);
+  lo = __builtin_amdgcn_readlane(lo, 63);
+  hi = __builtin_amdgcn_readlane(hi, 63);
+  return ((unsigned long long)hi << 32) | lo;
 }
 
-__global__ __launch_bounds__(256) void k_leaf_lu(c2 *A, int64_t lda, int m, int w, int *ipiv, int piv_off,
-                                                 int *flags)
+struct DiagShared {
+  c2 L[DL_M][LEAF_W + 1];   // the panel by position, rows c0..jb-1 (stride 33: 16 lanes on distinct banks)
+  c2 U[LEAF_W][DL_M];       // U12, columns counted from c0 + 32; during the leaf, the finished columns by data row
+  unsigned long long wkey[2][4];  // per-wave argmax keys of the rows' waves (slots alternate between columns)
+  c2 prow[2][LEAF_W];         // pivot row
+  int perm[DL_M];
+};
+
+__global__ __launch_bounds__(DL_NT) void k_diag_lu(c2 *A, int64_t lda, int jb, int *ipiv, int piv_off, int *flags)
 {
-  __shared__ float sbest[2][4];
-  __shared__ int srow[2][4];
-  __shared__ c2 prow[2][LEAF_W], crow[2][LEAF_W];
+  __shared__ DiagShared sh;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  c2 x[LEAF_W];
+  const int l16 = lane & 15, lq = lane >> 4;
+  for (int c0 = 0; c0 < jb; c0 += LEAF_W) {
+    const int pw = min(LEAF_W, jb - c0);
+    // leaf
+    const bool act = t >= c0 && t < jb;
+    int pos = t;
+    c2 x[LEAF_W];
 #pragma unroll
-  for (int j = 0; j < LEAF_W; j++) x[j] = (t < m && j < w) ? A[(size_t)j * lda + t] : c2{0.0f, 0.0f};
-  leaf_steps<0>(x, t, lane, wv, m, w, ipiv, piv_off, flags, sbest, srow, prow, crow);
-  if (t < m)
+    for (int j = 0; j < LEAF_W; j++) x[j] = (act && j < pw) ? A[(size_t)(c0 + j) * lda + t] : c2{0.0f, 0.0f};
+#pragma unroll 1
+    for (int c = 0; c < pw; c++) {
+      const int d = c0 + c, b = c & 1;
+      unsigned long long kk = 0ull;
+      if (act && pos >= d)
+        kk = ((unsigned long long)__float_as_uint(fabsf(x[0].re) + fabsf(x[0].im)) << 32) | (unsigned)~pos;
+      kk = wave_max_key(kk);
+      if (lane == 0 && wv < 4) sh.wkey[b][wv] = kk;
+      __syncthreads();
+      kk = sh.wkey[b][0];
 #pragma unroll
-    for (int j = 0; j < LEAF_W; j++)
-      if (j < w) A[(size_t)j * lda + t] = x[j];
+      for (int i = 1; i < 4; i++) kk = sh.wkey[b][i] > kk ? sh.wkey[b][i] : kk;
+      const float best = __uint_as_float((unsigned)(kk >> 32));
+      const int bp = (int)~(unsigned)kk;
+      if (t == 0) ipiv[d] = piv_off + bp;
+      if (best > 0.0f) {
+        if (act && pos == bp) {
+#pragma unroll
+          for (int j = 0; j < LEAF_W; j++) sh.prow[b][j] = x[j];
+          pos = d;
+        } else if (act && pos == d) {
+          pos = bp;
+        }
+        __syncthreads();
+        if (act && pos > d) {
+          const c2 dv = sh.prow[b][0];
+          const float dd = dv.re * dv.re + dv.im * dv.im;
+          const c2 inv = {dv.re / dd, -dv.im / dd};
+          const c2 l = {x[0].re * inv.re - x[0].im * inv.im, x[0].re * inv.im + x[0].im * inv.re};
+          x[0] = l;
+#pragma unroll
+          for (int j = 1; j < LEAF_W; j++) {
+            const c2 u = sh.prow[b][j];
+            x[j].re -= l.re * u.re - l.im * u.im;
+            x[j].im -= l.re * u.im + l.im * u.re;
+          }
+        }
+      } else {
+        if (t == 0 && flags[1] == 0) flags[1] = piv_off + d + 1;
+        __syncthreads();
+      }
+      if (act) sh.U[c][t - c0] = x[0];
+#pragma unroll
+      for (int j = 0; j < LEAF_W - 1; j++) x[j] = x[j + 1];
+      x[LEAF_W - 1] = c2{0.0f, 0.0f};
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int j = 0; j < LEAF_W; j++) x[j] = sh.U[j][t - c0];
+      sh.perm[pos] = t;
+#pragma unroll
+      for (int j = 0; j < LEAF_W; j++) {
+        sh.L[pos - c0][j] = x[j];
+        if (j < pw) A[(size_t)(c0 + j) * lda + pos] = x[j];
+      }
+    }
+    __syncthreads();
+    // swaps: rows r in [c0, jb) take row perm[r]'s data, in the columns
+    // outside the panel (cc < c0 left of it, cc >= c0 right of it)
+    {
+      const int r = t & (DL_M - 1), g = t >> 8;
+      const int src = (r >= c0 && r < jb) ? sh.perm[r] : r;
+      const int ncc = jb - pw;
+      for (int base = 0; base < ncc; base += 4 * 14) {
+        c2 v[14];
+#pragma unroll
+        for (int i = 0; i < 14; i++) {
+          const int cc = base + g + 4 * i;
+          const int col = cc < c0 ? cc : cc + pw;
+          v[i] = (src != r && cc < ncc) ? A[(size_t)col * lda + src] : c2{0.0f, 0.0f};
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 14; i++) {
+          const int cc = base + g + 4 * i;
+          const int col = cc < c0 ? cc : cc + pw;
+          if (src != r && cc < ncc) A[(size_t)col * lda + r] = v[i];
+        }
+      }
+    }
+    __syncthreads();
+    const int nr = jb - c0 - LEAF_W;  // rows and columns of A22 (a full panel when nr > 0)
+    if (nr <= 0) break;
+    // U12 = L11^-1 A12: thread t solves column c0 + 32 + t
+    if (t < nr) {
+      c2 *col = A + (size_t)(c0 + LEAF_W + t) * lda + c0;
+      c2 u[LEAF_W];
+#pragma unroll
+      for (int i = 0; i < LEAF_W; i++) u[i] = col[i];
+      // column-oriented: once u[k] is final, rows below subtract L(i, k)
+      // u[k]; the empty asm keeps the LDS reads of one k from being hoisted
+      // above the previous k (register pressure at 1024 threads)
+#pragma unroll
+      for (int k = 0; k < LEAF_W - 1; k++) {
+#pragma unroll
+        for (int i = k + 1; i < LEAF_W; i++) {
+          const c2 l = sh.L[i][k];
+          u[i].re -= l.re * u[k].re - l.im * u[k].im;
+          u[i].im -= l.re * u[k].im + l.im * u[k].re;
+        }
+        asm volatile("" ::: "memory");
+      }
+#pragma unroll
+      for (int i = 0; i < LEAF_W; i++) {
+        sh.U[i][t] = u[i];
+        col[i] = u[i];
+      }
+    }
+    __syncthreads();
+    // A22 -= L21 U12
+    const int nt = (nr + 15) / 16;
+    for (int tile = wv; tile < nt * nt; tile += DL_NW) {
+      const int tr = tile % nt, tc = tile / nt;
+      const int row = tr * 16 + l16;            // of A22
+      c2 *cc = A + (size_t)(c0 + LEAF_W) * lda + c0 + LEAF_W + row;
+      c2 cv[4];
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int cl = tc * 16 + 4 * lq + v;
+        cv[v] = (row < nr && cl < nr) ? cc[(size_t)cl * lda] : c2{0.0f, 0.0f};
+      }
+      f4 pr = {0.0f, 0.0f, 0.0f, 0.0f}, pi = {0.0f, 0.0f, 0.0f, 0.0f};
+      const int cu = tc * 16 + l16;             // U12 column of this lane's first operand
+      const bool uok = cu < nr, lok = row < nr;
+#pragma unroll
+      for (int k0 = 0; k0 < LEAF_W; k0 += 4) {
+        const int kk = k0 + lq;
+        const c2 uv = uok ? sh.U[kk][cu] : c2{0.0f, 0.0f};
+        const c2 lv = lok ? sh.L[LEAF_W + row][kk] : c2{0.0f, 0.0f};
+        pr = __builtin_amdgcn_mfma_f32_16x16x4f32(uv.re, lv.re, pr, 0, 0, 0);
+        pi = __builtin_amdgcn_mfma_f32_16x16x4f32(uv.re, lv.im, pi, 0, 0, 0);
+        pr = __builtin_amdgcn_mfma_f32_16x16x4f32(-uv.im, lv.im, pr, 0, 0, 0);
+        pi = __builtin_amdgcn_mfma_f32_16x16x4f32(uv.im, lv.re, pi, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int cl = tc * 16 + 4 * lq + v;
+        if (row < nr && cl < nr) cc[(size_t)cl * lda] = {cv[v].re - pr[v], cv[v].im - pi[v]};
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // X = I (n x n, leading dimension n)
@@ -627,30 +754,6 @@ void gemm_planes(int m, int n, int k, c2 *C, int64_t ldc, const SplitWS &w, int 
   }
 }
 
-// m x w panel with partial pivoting over its m rows (m <= 256): the panel
-// recursion of the partial-pivoting LU (left half, swaps and unit-lower TRSM
-// into the right half, its update, right half, swaps back into the left)
-// with the single-wave leaf kernel at width <= 32
-int panel_local(c2 *A, int m, int w, int64_t lda, int *ipiv, int piv_off, int *flags, hipStream_t st)
-{
-  if (w <= LEAF_W) {
-    hipLaunchKernelGGL(k_leaf_lu, dim3(1), dim3(LEAF_M), 0, st, A, lda, m, w, ipiv, piv_off, flags);
-    return BFA_OK;
-  }
-  const int w1 = w / 2, w2 = w - w1;
-  int rc = panel_local(A, m, w1, lda, ipiv, piv_off, flags, st);
-  if (rc) return rc;
-  c2 *Ar = A + (size_t)w1 * lda;
-  bfa::laswp_c(Ar - piv_off, lda, 0, w2, piv_off, w1, ipiv - piv_off, st);
-  bfa::trsm_lower_c(w1, w2, A, lda, Ar, lda, st);
-  rc = bfa::gemm_sub_c(m - w1, w2, w1, A + w1, lda, Ar, lda, Ar + w1, lda, st);
-  if (rc) return rc;
-  rc = panel_local(A + (size_t)w1 * lda + w1, m - w1, w2, lda, ipiv + w1, piv_off + w1, flags, st);
-  if (rc) return rc;
-  bfa::laswp_c(A - piv_off, lda, 0, w1, piv_off + w1, w2, ipiv - piv_off, st);
-  return BFA_OK;
-}
-
 // step j's block column: the jb x jb diagonal block with partial pivoting
 // over its rows, then L21 = A21 U11^-1 (U11^-1 by the one-launch upper TRSM
 // on the identity, the product on the matrix cores stored over A21)
@@ -658,8 +761,7 @@ int block_column(c2 *A, int n, int64_t lda, int j, int jb, int *ipiv, int *flags
                  hipStream_t st)
 {
   c2 *Ajj = A + (size_t)j * lda + j;
-  int rc = panel_local(Ajj, jb, jb, lda, ipiv + j, j, flags, st);
-  if (rc) return rc;
+  hipLaunchKernelGGL(k_diag_lu, dim3(1), dim3(DL_NT), 0, st, Ajj, lda, jb, ipiv + j, j, flags);
   const int below = n - j - jb;
   if (below <= 0) return BFA_OK;
   hipLaunchKernelGGL(k_eye, dim3((jb + 255) / 256, jb), dim3(256), 0, st, Uinv, jb);

@@ -8,7 +8,10 @@ cores, as the preconditioner of FP64 iterative refinement.
   and rows/columns spanning 1e-8..1e8 in magnitude (the per-row / per-column
   power-of-two scaling keeps every row's relative accuracy).
 - bfa_lu_factor_c_local on a random matrix (n = 3,000, not a multiple of
-  256): the solve's backward error at complex64 level.
+  256): the solve's backward error at complex64 level; on a matrix that
+  pivots in most columns (n = 600): the pivots equal the float64
+  restatement's (tests/blocklu.py) and P A = L U to the split products'
+  accuracy.
 - On VIE matrices (n = 5 and n = 12 Kuhn spheres, the second the N = 19,872
   headline workload): FP64 refinement with these factors reaches a 1e-10
   residual and its powers equal the FP64 path's to 1e-9.
@@ -16,6 +19,7 @@ cores, as the preconditioner of FP64 iterative refinement.
 import numpy as np
 import pytest
 
+from blocklu import block_local_lu
 from conftest import kuhn
 
 pytestmark = pytest.mark.gpu
@@ -76,6 +80,45 @@ def test_lu_local_random(bfa, torch_cuda):
     eta = (torch.linalg.norm(Am @ Xm - B.T) / (torch.linalg.norm(A) * torch.linalg.norm(Xm))).item()
     print(f"[lu local] n={n} backward error {eta:.2e}", flush=True)
     assert eta < 1e-6
+
+
+@pytest.mark.gpu
+def test_lu_local_pivots(bfa, torch_cuda):
+    """Pivot choices of the diagonal-block kernel against the float64
+    restatement on a matrix that pivots in most columns (entry magnitudes
+    spread over two decades), n = 600: blocks of 256, 256 and 88 rows (a
+    last panel of 24 columns).  Pivots must agree up to the first column
+    whose two best candidates lie within 1e-5 of each other; P A = L U to
+    the split products' accuracy."""
+    torch = torch_cuda
+    n = 600
+    rng = np.random.default_rng(600)
+    A = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * 10.0 ** rng.uniform(0, 2, (n, n))
+    ipiv_ref, gap, _ = block_local_lu(A)
+    dA = torch.from_numpy(np.asfortranarray(A).T.copy()).to(torch.complex64).to("cuda:0")   # column-major
+    piv = torch.zeros(n, dtype=torch.int32, device="cuda:0")
+    assert bfa.lu_factor_c_local(dA, n, piv) == 0
+    p = piv.cpu().numpy()
+    LU = dA.cpu().numpy().T.astype(np.complex128)
+    tie = np.flatnonzero(gap < 1e-5)
+    upto = int(tie[0]) if len(tie) else n
+    moved = int(np.sum(ipiv_ref[:upto] != np.arange(upto)))
+    print(f"[lu pivots] n={n}: pivots compared over {upto} columns ({moved} interchanges)", flush=True)
+    assert moved > upto // 2
+    assert np.array_equal(p[:upto], ipiv_ref[:upto])
+    PA = A.copy()
+    for i in range(n):
+        if p[i] != i:
+            PA[[i, p[i]], :] = PA[[p[i], i], :]
+    L = np.tril(LU, -1) + np.eye(n)
+    U = np.triu(LU)
+    err = np.linalg.norm(PA - L @ U) / np.linalg.norm(A)
+    # backward error of an LU with unit roundoff u: O(u) || |L||U| || / ||A||,
+    # u = 2^-22 for the split-FP16 products (|L||U| ~ 700 ||A|| here: the
+    # rows below a diagonal block are never pivots)
+    bound = 2.0 ** -22 * np.linalg.norm(np.abs(L) @ np.abs(U)) / np.linalg.norm(A)
+    print(f"[lu pivots] ||PA - LU|| / ||A|| = {err:.2e} (bound {bound:.2e})", flush=True)
+    assert err < bound
 
 
 @pytest.mark.parametrize("n_sub", [5, 12])
