@@ -391,28 +391,30 @@ __global__ __launch_bounds__(512, 1) void k_gemm_hs(int m, int n, int kp, const 
 // in one workgroup of 1024 threads.  The block stays in global memory (512
 // KB, L2-resident); panels of 32 columns, each:
 //   leaf    rows c0..jb-1 of the panel.  Thread t keeps the panel part of
-//           row t (its data row) in registers, shifted left by one column
-//           per step so that the current column is always x[0] (the loop
-//           body is the same for every column and stays small), and tracks
-//           the row's position: an interchange only swaps two positions.
-//           Per column d: the candidate |re| + |im| of each row at a
-//           position >= d as the key (bits of |re| + |im|) << 32 | ~position,
-//           whose maximum -- DPP within each wave, then the four rows' waves
-//           through LDS -- is the izamax pivot (largest magnitude, first
-//           position among equals); the pivot row's values go to every
-//           thread through LDS, the rows below scale and update.  A zero pivot is recorded (first one,
-//           1-based global column) and the column is left as it is, as
-//           LAPACK's zgetf2 does.  Each finished column value goes to LDS by
-//           data row; at the end every row lands at its final position in A
-//           and in LDS (L11 on top of L21), and perm[position] = data row;
-//   swaps   that permutation applied to the block's other columns (all
-//           loads, a barrier, all stores, 14 columns per thread at a time);
-//   U12     L11^-1 A12 for the panel's rows, a thread per column, into A and
-//           LDS;
+//           row t (its data row) in registers and tracks the row's position:
+//           an interchange only swaps two positions.  The columns run in
+//           windows of four held at x[0..3] (the window's loop body is the
+//           same for every window and stays small; x shifts by four between
+//           windows).  Per column d: the candidate |re| + |im| of each row at
+//           a position >= d as the key (bits of |re| + |im|) << 32 |
+//           ~position, whose maximum -- DPP within each wave, then the four
+//           rows' waves through LDS -- is the izamax pivot (largest
+//           magnitude, first position among equals); the pivot row's values
+//           go to every thread through LDS, the rows below scale and update.
+//           A zero pivot is recorded (first one, 1-based global column) and
+//           the column is left as it is, as LAPACK's zgetf2 does.  Each
+//           finished column value goes to LDS by data row; at the end every
+//           row lands at its final position in A and in LDS (L11 on top of
+//           L21), and perm[position] = data row;
+//   swaps   that permutation applied to the block's other columns for the
+//           positions the interchanges touched (all loads, a barrier, all
+//           stores);
+//   U12     L11^-1 A12 for the panel's rows in four blocks of eight rows,
+//           into A and LDS;
 //   update  A22 -= L21 U12 in 16 x 16 tiles on v_mfma_f32_16x16x4f32 (U12's
 //           value as the first operand, so that register v of lane l holds
-//           row l % 16, column 4 (l / 16) + v of the tile), a wave per tile
-//           in turn.
+//           row l % 16, column 4 (l / 16) + v of the tile), four tiles of a
+//           wave at a time.
 // Barriers separate the phases; a workgroup barrier also orders the global
 // stores of one phase before the loads of the next.  Pivots are written
 // absolute (piv_off + local row).
@@ -428,10 +430,10 @@ __device__ __forceinline__ void key_max_step(unsigned &lo, unsigned &hi)
 {
   const unsigned ol = __builtin_amdgcn_update_dpp(0u, lo, CTRL, RM, 0xF, false);
   const unsigned oh = __builtin_amdgcn_update_dpp(0u, hi, CTRL, RM, 0xF, false);
-  if (oh > hi || (oh == hi && ol > lo)) {
-    hi = oh;
-    lo = ol;
-  }
+  const unsigned long long o = ((unsigned long long)oh << 32) | ol, k = ((unsigned long long)hi << 32) | lo;
+  const bool take = o > k;
+  lo = take ? ol : lo;
+  hi = take ? oh : hi;
 }
 
 // maximum of k over the wavefront (all lanes active), returned uniform:
@@ -455,10 +457,79 @@ __device__ __forceinline__ unsigned long long wave_max_key(unsigned long long k)
 struct DiagShared {
   c2 L[DL_M][LEAF_W + 1];   // the panel by position, rows c0..jb-1 (stride 33: 16 lanes on distinct banks)
   c2 U[LEAF_W][DL_M];       // U12, columns counted from c0 + 32; during the leaf, the finished columns by data row
-  unsigned long long wkey[2][4];  // per-wave argmax keys of the rows' waves (slots alternate between columns)
+  unsigned long long wkey[2][4];  // argmax keys of the column's four waves (slots alternate between columns)
+  c2 colv[2][DL_M];           // the current column by data row
+  int piv[LEAF_W];            // the panel's pivots (local positions)
   c2 prow[2][LEAF_W];         // pivot row
   int perm[DL_M];
+  int mv[2 * LEAF_W], nmv;    // positions touched by the panel's interchanges
 };
+
+// Column 8W + U of the leaf: thread (row r, quarter q) holds columns
+// 8q..8q+7 of data row r in x; the four waves of quarter W hold the column.
+// Those waves form the argmax keys of the rows at positions >= d (a wave
+// maximum by DPP, one key per wave to LDS) and put the column's values in
+// LDS by data row; after the first barrier every thread takes the pivot,
+// the pivot row's threads put it in LDS, and after the second the rows
+// below scale column d and update their later columns.  No global memory
+// is touched (a barrier would wait for it); the pivots collect in LDS.
+// The slots alternate with the column's parity.
+template <int U>
+__device__ __forceinline__ void leaf_col(c2 (&x)[8], int &pos, bool act, int q, int r, int t, int lane, int wv,
+                                         int c0, int W, int *flags, int piv_off, DiagShared &sh)
+{
+  const int C = 8 * W + U, d = c0 + C, b = U & 1;
+  if (q == W) {
+    unsigned long long kk = 0ull;
+    if (act && pos >= d)
+      kk = ((unsigned long long)__float_as_uint(fabsf(x[U].re) + fabsf(x[U].im)) << 32) | (unsigned)~pos;
+    kk = wave_max_key(kk);
+    if (lane == 0) sh.wkey[b][wv & 3] = kk;
+    sh.colv[b][r] = x[U];
+  }
+  __syncthreads();
+  unsigned long long kk = sh.wkey[b][0];
+#pragma unroll
+  for (int i = 1; i < 4; i++) kk = sh.wkey[b][i] > kk ? sh.wkey[b][i] : kk;
+  const float best = __uint_as_float((unsigned)(kk >> 32));
+  const int bp = (int)~(unsigned)kk;
+  if (t == 0) {
+    sh.piv[C] = bp;
+    if (best > 0.0f && bp != d) {
+      const int k = sh.nmv;
+      sh.mv[k] = d;
+      sh.mv[k + 1] = bp;
+      sh.nmv = k + 2;
+    }
+  }
+  if (best > 0.0f) {
+    if (act && pos == bp) {
+#pragma unroll
+      for (int j = 0; j < 8; j++) sh.prow[b][8 * q + j] = x[j];
+      pos = d;
+    } else if (act && pos == d) {
+      pos = bp;
+    }
+    __syncthreads();
+    if (q >= W && act && pos > d) {
+      const c2 cand = sh.colv[b][r];
+      const c2 dv = sh.prow[b][C];
+      const float rr = 1.0f / (dv.re * dv.re + dv.im * dv.im);
+      const c2 inv = {dv.re * rr, -dv.im * rr};
+      const c2 l = {cand.re * inv.re - cand.im * inv.im, cand.re * inv.im + cand.im * inv.re};
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const c2 u = sh.prow[b][8 * q + j];
+        const c2 y = {fmaf(l.im, u.im, fmaf(-l.re, u.re, x[j].re)), fmaf(-l.im, u.re, fmaf(-l.re, u.im, x[j].im))};
+        if (q > W || j > U) x[j] = y;
+        else if (j == U) x[j] = l;
+      }
+    }
+  } else {
+    if (t == 0 && flags[1] == 0) flags[1] = piv_off + d + 1;
+    __syncthreads();
+  }
+}
 
 __global__ __launch_bounds__(DL_NT) void k_diag_lu(c2 *A, int64_t lda, int jb, int *ipiv, int piv_off, int *flags)
 {
@@ -467,151 +538,151 @@ __global__ __launch_bounds__(DL_NT) void k_diag_lu(c2 *A, int64_t lda, int jb, i
   const int l16 = lane & 15, lq = lane >> 4;
   for (int c0 = 0; c0 < jb; c0 += LEAF_W) {
     const int pw = min(LEAF_W, jb - c0);
-    // leaf
-    const bool act = t >= c0 && t < jb;
-    int pos = t;
-    c2 x[LEAF_W];
-#pragma unroll
-    for (int j = 0; j < LEAF_W; j++) x[j] = (act && j < pw) ? A[(size_t)(c0 + j) * lda + t] : c2{0.0f, 0.0f};
-#pragma unroll 1
-    for (int c = 0; c < pw; c++) {
-      const int d = c0 + c, b = c & 1;
-      unsigned long long kk = 0ull;
-      if (act && pos >= d)
-        kk = ((unsigned long long)__float_as_uint(fabsf(x[0].re) + fabsf(x[0].im)) << 32) | (unsigned)~pos;
-      kk = wave_max_key(kk);
-      if (lane == 0 && wv < 4) sh.wkey[b][wv] = kk;
-      __syncthreads();
-      kk = sh.wkey[b][0];
-#pragma unroll
-      for (int i = 1; i < 4; i++) kk = sh.wkey[b][i] > kk ? sh.wkey[b][i] : kk;
-      const float best = __uint_as_float((unsigned)(kk >> 32));
-      const int bp = (int)~(unsigned)kk;
-      if (t == 0) ipiv[d] = piv_off + bp;
-      if (best > 0.0f) {
-        if (act && pos == bp) {
-#pragma unroll
-          for (int j = 0; j < LEAF_W; j++) sh.prow[b][j] = x[j];
-          pos = d;
-        } else if (act && pos == d) {
-          pos = bp;
-        }
-        __syncthreads();
-        if (act && pos > d) {
-          const c2 dv = sh.prow[b][0];
-          const float dd = dv.re * dv.re + dv.im * dv.im;
-          const c2 inv = {dv.re / dd, -dv.im / dd};
-          const c2 l = {x[0].re * inv.re - x[0].im * inv.im, x[0].re * inv.im + x[0].im * inv.re};
-          x[0] = l;
-#pragma unroll
-          for (int j = 1; j < LEAF_W; j++) {
-            const c2 u = sh.prow[b][j];
-            x[j].re -= l.re * u.re - l.im * u.im;
-            x[j].im -= l.re * u.im + l.im * u.re;
-          }
-        }
-      } else {
-        if (t == 0 && flags[1] == 0) flags[1] = piv_off + d + 1;
-        __syncthreads();
-      }
-      if (act) sh.U[c][t - c0] = x[0];
-#pragma unroll
-      for (int j = 0; j < LEAF_W - 1; j++) x[j] = x[j + 1];
-      x[LEAF_W - 1] = c2{0.0f, 0.0f};
-    }
-    __syncthreads();
-    if (act) {
-#pragma unroll
-      for (int j = 0; j < LEAF_W; j++) x[j] = sh.U[j][t - c0];
-      sh.perm[pos] = t;
-#pragma unroll
-      for (int j = 0; j < LEAF_W; j++) {
-        sh.L[pos - c0][j] = x[j];
-        if (j < pw) A[(size_t)(c0 + j) * lda + pos] = x[j];
-      }
-    }
-    __syncthreads();
-    // swaps: rows r in [c0, jb) take row perm[r]'s data, in the columns
-    // outside the panel (cc < c0 left of it, cc >= c0 right of it)
+    // leaf: thread (row r = t % 256, quarter q = t / 256) holds columns
+    // 8q..8q+7 of data row r; windows of eight columns, one quarter at a time
+    if (t == 0) sh.nmv = 0;
     {
-      const int r = t & (DL_M - 1), g = t >> 8;
-      const int src = (r >= c0 && r < jb) ? sh.perm[r] : r;
-      const int ncc = jb - pw;
-      for (int base = 0; base < ncc; base += 4 * 14) {
-        c2 v[14];
+      const int r = t & (DL_M - 1), q = t >> 8;
+      const bool act = r >= c0 && r < jb;
+      int pos = r;
+      c2 x[8];
 #pragma unroll
-        for (int i = 0; i < 14; i++) {
-          const int cc = base + g + 4 * i;
-          const int col = cc < c0 ? cc : cc + pw;
-          v[i] = (src != r && cc < ncc) ? A[(size_t)col * lda + src] : c2{0.0f, 0.0f};
-        }
-        __syncthreads();
+      for (int j = 0; j < 8; j++)
+        x[j] = (act && 8 * q + j < pw) ? A[(size_t)(c0 + 8 * q + j) * lda + r] : c2{0.0f, 0.0f};
+#pragma unroll 1
+      for (int W = 0; 8 * W < pw; W++) {
+        const int left = pw - 8 * W;
+        leaf_col<0>(x, pos, act, q, r, t, lane, wv, c0, W, flags, piv_off, sh);
+        if (left > 1) leaf_col<1>(x, pos, act, q, r, t, lane, wv, c0, W, flags, piv_off, sh);
+        if (left > 2) leaf_col<2>(x, pos, act, q, r, t, lane, wv, c0, W, flags, piv_off, sh);
+        if (left > 3) leaf_col<3>(x, pos, act, q, r, t, lane, wv, c0, W, flags, piv_off, sh);
+        if (left > 4) leaf_col<4>(x, pos, act, q, r, t, lane, wv, c0, W, flags, piv_off, sh);
+        if (left > 5) leaf_col<5>(x, pos, act, q, r, t, lane, wv, c0, W, flags, piv_off, sh);
+        if (left > 6) leaf_col<6>(x, pos, act, q, r, t, lane, wv, c0, W, flags, piv_off, sh);
+        if (left > 7) leaf_col<7>(x, pos, act, q, r, t, lane, wv, c0, W, flags, piv_off, sh);
+      }
+      // every row to its final position, in A and in LDS; the pivots out
+      if (act) {
+        if (q == 0) sh.perm[pos] = r;
 #pragma unroll
-        for (int i = 0; i < 14; i++) {
-          const int cc = base + g + 4 * i;
-          const int col = cc < c0 ? cc : cc + pw;
-          if (src != r && cc < ncc) A[(size_t)col * lda + r] = v[i];
+        for (int j = 0; j < 8; j++) {
+          sh.L[pos - c0][8 * q + j] = x[j];
+          if (8 * q + j < pw) A[(size_t)(c0 + 8 * q + j) * lda + pos] = x[j];
         }
+      }
+      if (t < pw) ipiv[c0 + t] = piv_off + sh.piv[t];
+    }
+    __syncthreads();
+    // swaps: each position in the moved list (both rows of every interchange,
+    // repeats harmless) takes row perm[r]'s data in the columns outside the
+    // panel (cc < c0 left of it, cc >= c0 right of it); lane = list entry,
+    // wave = column group, so that one load touches one column
+    {
+      const int li = lane, cg = wv, ncc = jb - pw;
+      const int r = li < sh.nmv ? sh.mv[li] : 0;
+      const int src = li < sh.nmv ? sh.perm[r] : 0;
+      const bool mov = li < sh.nmv && src != r;
+      c2 v[14];
+#pragma unroll
+      for (int i = 0; i < 14; i++) {
+        const int cc = cg + 16 * i;
+        const int col = cc < c0 ? cc : cc + pw;
+        v[i] = (mov && cc < ncc) ? A[(size_t)col * lda + src] : c2{0.0f, 0.0f};
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < 14; i++) {
+        const int cc = cg + 16 * i;
+        const int col = cc < c0 ? cc : cc + pw;
+        if (mov && cc < ncc) A[(size_t)col * lda + r] = v[i];
       }
     }
     __syncthreads();
     const int nr = jb - c0 - LEAF_W;  // rows and columns of A22 (a full panel when nr > 0)
     if (nr <= 0) break;
-    // U12 = L11^-1 A12: thread t solves column c0 + 32 + t
-    if (t < nr) {
-      c2 *col = A + (size_t)(c0 + LEAF_W + t) * lda + c0;
-      c2 u[LEAF_W];
+    // U12 = L11^-1 A12: four threads per column, thread q holding rows
+    // 8q..8q+7; block q solved by its thread once blocks < q have been
+    // subtracted, then handed to the threads below through LDS
+    {
+      const int cl = t >> 2, q = t & 3;
+      const bool on = cl < nr;
+      c2 *col = A + (size_t)(c0 + LEAF_W + cl) * lda + c0 + 8 * q;
+      c2 a[8];
 #pragma unroll
-      for (int i = 0; i < LEAF_W; i++) u[i] = col[i];
-      // column-oriented: once u[k] is final, rows below subtract L(i, k)
-      // u[k]; the empty asm keeps the LDS reads of one k from being hoisted
-      // above the previous k (register pressure at 1024 threads)
+      for (int i = 0; i < 8; i++) a[i] = on ? col[i] : c2{0.0f, 0.0f};
 #pragma unroll
-      for (int k = 0; k < LEAF_W - 1; k++) {
+      for (int b = 0; b < 4; b++) {
+        if (on && q == b) {
 #pragma unroll
-        for (int i = k + 1; i < LEAF_W; i++) {
-          const c2 l = sh.L[i][k];
-          u[i].re -= l.re * u[k].re - l.im * u[k].im;
-          u[i].im -= l.re * u[k].im + l.im * u[k].re;
+          for (int i = 1; i < 8; i++)
+#pragma unroll
+            for (int k = 0; k < i; k++) {
+              const c2 l = sh.L[8 * b + i][8 * b + k];
+              a[i].re -= l.re * a[k].re - l.im * a[k].im;
+              a[i].im -= l.re * a[k].im + l.im * a[k].re;
+            }
+#pragma unroll
+          for (int i = 0; i < 8; i++) sh.U[8 * b + i][cl] = a[i];
         }
-        asm volatile("" ::: "memory");
-      }
+        __syncthreads();
+        if (on && q > b) {
 #pragma unroll
-      for (int i = 0; i < LEAF_W; i++) {
-        sh.U[i][t] = u[i];
-        col[i] = u[i];
+          for (int k = 0; k < 8; k++) {
+            const c2 u = sh.U[8 * b + k][cl];
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+              const c2 l = sh.L[8 * q + i][8 * b + k];
+              a[i].re -= l.re * u.re - l.im * u.im;
+              a[i].im -= l.re * u.im + l.im * u.re;
+            }
+          }
+        }
       }
+      if (on)
+#pragma unroll
+        for (int i = 0; i < 8; i++) col[i] = a[i];
     }
     __syncthreads();
-    // A22 -= L21 U12
+    // A22 -= L21 U12, four tiles of a wave at a time: their loads of A22
+    // first, then the products
     const int nt = (nr + 15) / 16;
-    for (int tile = wv; tile < nt * nt; tile += DL_NW) {
-      const int tr = tile % nt, tc = tile / nt;
-      const int row = tr * 16 + l16;            // of A22
-      c2 *cc = A + (size_t)(c0 + LEAF_W) * lda + c0 + LEAF_W + row;
-      c2 cv[4];
+    for (int tile0 = wv * 4; tile0 < nt * nt; tile0 += DL_NW * 4) {
+      c2 cv[4][4];
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int cl = tc * 16 + 4 * lq + v;
-        cv[v] = (row < nr && cl < nr) ? cc[(size_t)cl * lda] : c2{0.0f, 0.0f};
-      }
-      f4 pr = {0.0f, 0.0f, 0.0f, 0.0f}, pi = {0.0f, 0.0f, 0.0f, 0.0f};
-      const int cu = tc * 16 + l16;             // U12 column of this lane's first operand
-      const bool uok = cu < nr, lok = row < nr;
+      for (int g = 0; g < 4; g++) {
+        const int tile = tile0 + g, tr = tile % nt, tc = tile / nt;
+        const int row = tr * 16 + l16;
 #pragma unroll
-      for (int k0 = 0; k0 < LEAF_W; k0 += 4) {
-        const int kk = k0 + lq;
-        const c2 uv = uok ? sh.U[kk][cu] : c2{0.0f, 0.0f};
-        const c2 lv = lok ? sh.L[LEAF_W + row][kk] : c2{0.0f, 0.0f};
-        pr = __builtin_amdgcn_mfma_f32_16x16x4f32(uv.re, lv.re, pr, 0, 0, 0);
-        pi = __builtin_amdgcn_mfma_f32_16x16x4f32(uv.re, lv.im, pi, 0, 0, 0);
-        pr = __builtin_amdgcn_mfma_f32_16x16x4f32(-uv.im, lv.im, pr, 0, 0, 0);
-        pi = __builtin_amdgcn_mfma_f32_16x16x4f32(uv.im, lv.re, pi, 0, 0, 0);
+        for (int v = 0; v < 4; v++) {
+          const int cl = tc * 16 + 4 * lq + v;
+          cv[g][v] = (tile < nt * nt && row < nr && cl < nr)
+                         ? A[(size_t)(c0 + LEAF_W + cl) * lda + c0 + LEAF_W + row] : c2{0.0f, 0.0f};
+        }
       }
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int cl = tc * 16 + 4 * lq + v;
-        if (row < nr && cl < nr) cc[(size_t)cl * lda] = {cv[v].re - pr[v], cv[v].im - pi[v]};
+      for (int g = 0; g < 4; g++) {
+        const int tile = tile0 + g, tr = tile % nt, tc = tile / nt;
+        if (tile >= nt * nt) break;
+        const int row = tr * 16 + l16;
+        f4 pr = {0.0f, 0.0f, 0.0f, 0.0f}, pi = {0.0f, 0.0f, 0.0f, 0.0f};
+        const int cu = tc * 16 + l16;           // U12 column of this lane's first operand
+        const bool uok = cu < nr, lok = row < nr;
+#pragma unroll
+        for (int k0 = 0; k0 < LEAF_W; k0 += 4) {
+          const int kk = k0 + lq;
+          const c2 uv = uok ? sh.U[kk][cu] : c2{0.0f, 0.0f};
+          const c2 lv = lok ? sh.L[LEAF_W + row][kk] : c2{0.0f, 0.0f};
+          pr = __builtin_amdgcn_mfma_f32_16x16x4f32(uv.re, lv.re, pr, 0, 0, 0);
+          pi = __builtin_amdgcn_mfma_f32_16x16x4f32(uv.re, lv.im, pi, 0, 0, 0);
+          pr = __builtin_amdgcn_mfma_f32_16x16x4f32(-uv.im, lv.im, pr, 0, 0, 0);
+          pi = __builtin_amdgcn_mfma_f32_16x16x4f32(uv.im, lv.re, pi, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          const int cl = tc * 16 + 4 * lq + v;
+          if (row < nr && cl < nr)
+            A[(size_t)(c0 + LEAF_W + cl) * lda + c0 + LEAF_W + row] = {cv[g][v].re - pr[v], cv[g][v].im - pi[v]};
+        }
       }
     }
     __syncthreads();

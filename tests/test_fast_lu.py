@@ -9,9 +9,9 @@ cores, as the preconditioner of FP64 iterative refinement.
   power-of-two scaling keeps every row's relative accuracy).
 - bfa_lu_factor_c_local on a random matrix (n = 3,000, not a multiple of
   256): the solve's backward error at complex64 level; on a matrix that
-  pivots in most columns (n = 600): the pivots equal the float64
-  restatement's (tests/blocklu.py) and P A = L U to the split products'
-  accuracy.
+  pivots in most columns (n = 600): the first block's pivots equal the
+  float64 restatement's (tests/blocklu.py), every pivot dominates its
+  column's candidates, and P A = L U to the split products' accuracy.
 - On VIE matrices (n = 5 and n = 12 Kuhn spheres, the second the N = 19,872
   headline workload): FP64 refinement with these factors reaches a 1e-10
   residual and its powers equal the FP64 path's to 1e-9.
@@ -84,12 +84,18 @@ def test_lu_local_random(bfa, torch_cuda):
 
 @pytest.mark.gpu
 def test_lu_local_pivots(bfa, torch_cuda):
-    """Pivot choices of the diagonal-block kernel against the float64
-    restatement on a matrix that pivots in most columns (entry magnitudes
-    spread over two decades), n = 600: blocks of 256, 256 and 88 rows (a
-    last panel of 24 columns).  Pivots must agree up to the first column
-    whose two best candidates lie within 1e-5 of each other; P A = L U to
-    the split products' accuracy."""
+    """Pivot choices of the diagonal-block kernel on a matrix that pivots in
+    most columns (entry magnitudes spread over two decades), n = 600: blocks
+    of 256, 256 and 88 rows (a last panel of 24 columns).
+    - First block (FP32 arithmetic only): the pivots equal the float64
+      restatement's up to the first column whose two best candidates lie
+      within 1e-5 of each other.
+    - Every column: the chosen pivot dominates the column's other candidates
+      (izamax's rule, |re| + |im|) as the kernel's own factors show them
+      (candidate at step d = L(i, d) U(d, d)); later blocks see operands
+      rounded by the split products, so their pivots are not compared with
+      the float64 ones.
+    - P A = L U to the split products' accuracy."""
     torch = torch_cuda
     n = 600
     rng = np.random.default_rng(600)
@@ -100,18 +106,28 @@ def test_lu_local_pivots(bfa, torch_cuda):
     assert bfa.lu_factor_c_local(dA, n, piv) == 0
     p = piv.cpu().numpy()
     LU = dA.cpu().numpy().T.astype(np.complex128)
-    tie = np.flatnonzero(gap < 1e-5)
-    upto = int(tie[0]) if len(tie) else n
+    tie = np.flatnonzero(gap[:256] < 1e-5)
+    upto = int(tie[0]) if len(tie) else 256
     moved = int(np.sum(ipiv_ref[:upto] != np.arange(upto)))
-    print(f"[lu pivots] n={n}: pivots compared over {upto} columns ({moved} interchanges)", flush=True)
+    print(f"[lu pivots] n={n}: first block's pivots compared over {upto} columns ({moved} interchanges)", flush=True)
     assert moved > upto // 2
     assert np.array_equal(p[:upto], ipiv_ref[:upto])
+    assert all(i // 256 == p[i] // 256 and p[i] >= i for i in range(n))
+    L = np.tril(LU, -1) + np.eye(n)
+    U = np.triu(LU)
+    abs1 = lambda z: np.abs(z.real) + np.abs(z.imag)
+    worst = 0.0
+    for d in range(n):
+        e = (d // 256) * 256 + 256
+        cand = abs1(L[d + 1:e, d] * U[d, d])
+        if len(cand):
+            worst = max(worst, cand.max() / abs1(U[d, d]))
+    print(f"[lu pivots] largest candidate / pivot over all columns: {worst:.6f}", flush=True)
+    assert worst <= 1.0 + 1e-5
     PA = A.copy()
     for i in range(n):
         if p[i] != i:
             PA[[i, p[i]], :] = PA[[p[i], i], :]
-    L = np.tril(LU, -1) + np.eye(n)
-    U = np.triu(LU)
     err = np.linalg.norm(PA - L @ U) / np.linalg.norm(A)
     # backward error of an LU with unit roundoff u: O(u) || |L||U| || / ||A||,
     # u = 2^-22 for the split-FP16 products (|L||U| ~ 700 ||A|| here: the
