@@ -183,7 +183,7 @@ __global__ __launch_bounds__(256) void k_split_cols(const c2 *__restrict__ B, in
 // (the dispatcher deals workgroups to XCDs round-robin, so workgroup b runs
 // on XCD b % 8): XCD x works through super-tiles x, x + 8, ..., each one's
 // G^2 tiles together, so a super-tile's row slab of A and column slab of B
-// (G x 128 rows / columns, ~2 MB each at G = 8) stay in that XCD's L2 while
+// (G x 128 rows / columns, 1 MB each at G = 4) stay in that XCD's L2 while
 // its CUs work through them.  Tiles outside the matrix exit at once.
 __device__ __forceinline__ bool super_tile(int bid, int tiles_m, int tiles_n, int G, int &ti, int &tj)
 {
@@ -205,10 +205,16 @@ __device__ __forceinline__ h8 hneg(h8 x)
 }
 
 // NW = 8 waves in 2 x 4 of 64 x 32 each, two LDS buffers (128 KB, one
-// barrier per stage), one workgroup of two waves per SIMD (180 VGPRs, no
-// spill).  Measured at 19,616 x 19,616 x 256: 3.96 ms (199 TF/s counted as
-// 8 mnk, 598 TF/s of FP16 products); four waves of 64 x 64 on one 64 KB
-// buffer with two workgroups per CU spilled and took 6.59 ms.
+// barrier per stage), one workgroup of two waves per SIMD (181 VGPRs, no
+// spill).  Measured at 19,616 x 19,360 x 256 (the largest trailing update)
+// with G = 4: 2.86 ms, 818 TF/s of FP16 products (33 % of the 2.5 PF/s
+// peak).  Variants measured there: without the C loads 2.58 ms; without
+// operand loads after the first stage 2.84 ms; without LDS fragment reads
+// 2.95 ms; row blocks interleaved in pairs 2.90 ms; four waves of 64 x 64
+// fed by direct global-to-LDS loads (one wave per SIMD) 3.61 ms -- neither
+// LDS nor operand traffic limits it; the C read-modify-write (16 B per
+// entry per 24 x 256 FP16 products, near the 312 flop/B ridge) and the
+// unoverlapped epilogue of one workgroup per CU do.
 template <int NW>
 __device__ __forceinline__ void gemm_hs_tile(int m, int n, int kp, const _Float16 *__restrict__ PA, int64_t mp,
                                              const _Float16 *__restrict__ PB, int64_t np,
@@ -778,11 +784,13 @@ hipEvent_t sp_event(hipStream_t st)
   return e;
 }
 
-// super-tile edge G (8, or smaller while fewer than 16 super-tiles would
-// leave XCDs idle) and the padded grid of super_tile()
+// super-tile edge G (4, or smaller while fewer than 16 super-tiles would
+// leave XCDs idle) and the padded grid of super_tile().  Measured at
+// 19,616 x 19,360 x 256: G = 1, 2 or 4 2.86 ms, G = 8 3.03 ms, G = 16
+// 3.16 ms (an 8 x 8 super-tile's operand slabs fill an XCD's 4 MB L2).
 void super_grid(int tm, int tn, int *G, size_t *nblk)
 {
-  int g = 8;
+  int g = 4;
   while (g > 1 && (size_t)((tm + g - 1) / g) * ((tn + g - 1) / g) < 16) g /= 2;
   const size_t ns = (size_t)((tm + g - 1) / g) * ((tn + g - 1) / g);
   *G = g;
