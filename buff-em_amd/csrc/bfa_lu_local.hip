@@ -710,28 +710,45 @@ struct SplitWS {
   int64_t mp = 0;                  // rows of each plane (padded to the GEMM tile)
 };
 
+// split planes of L11^-1 (the A operand of U12 = L11^-1 A12), two of them
+// so that the look-ahead streams fill one while the caller's stream reads
+// the other
+struct LinvPlanes {
+  _Float16 *P = nullptr;
+  float *rf = nullptr;
+};
+
 // per-device workspace: two sets of split planes (the trailing update's on
 // the caller's stream, the block column's L21 product on the look-ahead
-// stream), U11^-1, the look-ahead stream and its events
+// stream), U11^-1 and L11^-1, L11^-1's planes, the two look-ahead streams
+// and their events
 struct LocalWork {
   void *mem = nullptr;
   size_t bytes = 0;
-  hipStream_t side = nullptr;
-  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  hipStream_t side = nullptr, side2 = nullptr;
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 LocalWork g_lw[64];
 
+struct LocalPtrs {
+  SplitWS bulk, pan;
+  c2 *Uinv = nullptr, *Linv = nullptr;
+  LinvPlanes lp[2];
+  LocalWork *lw = nullptr;
+};
+
 size_t al256(size_t x) { return (x + 255) / 256 * 256; }
 
-int local_work(int n, SplitWS *bulk, SplitWS *pan, c2 **Uinv, LocalWork **out)
+int local_work(int n, LocalPtrs *o)
 {
   int dev = 0;
   HIPCHECK(hipGetDevice(&dev));
   if (dev < 0 || dev >= 64) return fail(BFA_ERR_ARG, "device index %d unsupported", dev);
   const int64_t p = ((int64_t)n + HT - 1) / HT * HT;
   const size_t planes = al256((size_t)4 * p * LB * sizeof(_Float16)), fac = al256((size_t)p * 4);
-  const size_t set = 2 * planes + 2 * fac;
-  const size_t need = 2 * set + al256((size_t)LB * LB * sizeof(c2));
+  const size_t set = 2 * planes + 2 * fac, tri = al256((size_t)LB * LB * sizeof(c2));
+  const size_t lpl = al256((size_t)4 * LB * LB * sizeof(_Float16)) + al256((size_t)LB * 4);
+  const size_t need = 2 * set + 2 * tri + 2 * lpl;
   LocalWork &w = g_lw[dev];
   if (w.bytes < need) {
     if (w.mem) {
@@ -745,10 +762,11 @@ int local_work(int n, SplitWS *bulk, SplitWS *pan, c2 **Uinv, LocalWork **out)
   }
   if (!w.side) {
     HIPCHECK(hipStreamCreateWithFlags(&w.side, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&w.side2, hipStreamNonBlocking));
     for (auto &e : w.ev) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   char *b = (char *)w.mem;
-  for (SplitWS *s : {bulk, pan}) {
+  for (SplitWS *s : {&o->bulk, &o->pan}) {
     s->PA = (_Float16 *)b;
     s->PB = (_Float16 *)(b + planes);
     s->rf = (float *)(b + 2 * planes);
@@ -756,8 +774,15 @@ int local_work(int n, SplitWS *bulk, SplitWS *pan, c2 **Uinv, LocalWork **out)
     s->mp = p;
     b += set;
   }
-  *Uinv = (c2 *)b;
-  *out = &w;
+  o->Uinv = (c2 *)b;
+  o->Linv = (c2 *)(b + tri);
+  b += 2 * tri;
+  for (auto &l : o->lp) {
+    l.P = (_Float16 *)b;
+    l.rf = (float *)(b + al256((size_t)4 * LB * LB * sizeof(_Float16)));
+    b += lpl;
+  }
+  o->lw = &w;
   return BFA_OK;
 }
 
@@ -808,22 +833,22 @@ void split_ops(int m, int n, int k, const c2 *A, int64_t lda, const c2 *B, int64
                                 w.PB, w.cf);
 }
 
-// C(m x n) -= A B (store: C = A B) from split planes, the product's columns
-// starting at column c0 of the split B
-void gemm_planes(int m, int n, int k, c2 *C, int64_t ldc, const SplitWS &w, int c0, hipStream_t st, int store)
+// C(m x n) -= A B (store: C = A B) from split planes: A's planes PA (rows
+// padded to mpA) with row factors rf, B's planes PB (columns padded to npB)
+// with column factors cf, the product's columns starting at column c0 of B
+void gemm_ops(int m, int n, int k, c2 *C, int64_t ldc, const _Float16 *PA, const float *rf, int64_t mpA,
+              const _Float16 *PB, const float *cf, int64_t npB, int c0, hipStream_t st, int store)
 {
   if (m <= 0 || n <= 0 || k <= 0) return;
   const int kp = (k + HK - 1) / HK * HK;
   const int tm = (m + HT - 1) / HT, tn = (n + HT - 1) / HT;
-  const _Float16 *PB = w.PB + (size_t)c0 * kp;
-  const float *cf = w.cf + c0;
   const bool prof = bfa::profile_on();
   hipEvent_t e0 = prof ? sp_event(st) : nullptr;
   int G = 1;
   size_t nblk = 0;
   super_grid(tm, tn, &G, &nblk);
-  hipLaunchKernelGGL(k_gemm_hs, dim3((unsigned)nblk), dim3(512), 0, st, m, n, kp,
-                     (const _Float16 *)w.PA, w.mp, PB, w.mp, (const float *)w.rf, cf, C, ldc, tm, tn, store, G);
+  hipLaunchKernelGGL(k_gemm_hs, dim3((unsigned)nblk), dim3(512), 0, st, m, n, kp, PA, mpA, PB + (size_t)c0 * kp, npB,
+                     rf, cf + c0, C, ldc, tm, tn, store, G);
   if (prof) {
     hipEvent_t e1 = sp_event(st);
     if (e0 && e1) {
@@ -833,20 +858,38 @@ void gemm_planes(int m, int n, int k, c2 *C, int64_t ldc, const SplitWS &w, int 
   }
 }
 
-// step j's block column: the jb x jb diagonal block with partial pivoting
-// over its rows, then L21 = A21 U11^-1 (U11^-1 by the one-launch upper TRSM
-// on the identity, the product on the matrix cores stored over A21)
-int block_column(c2 *A, int n, int64_t lda, int j, int jb, int *ipiv, int *flags, const SplitWS &w, c2 *Uinv,
-                 hipStream_t st)
+// the same with both operands in one set of planes
+void gemm_planes(int m, int n, int k, c2 *C, int64_t ldc, const SplitWS &w, int c0, hipStream_t st, int store)
+{
+  gemm_ops(m, n, k, C, ldc, w.PA, w.rf, w.mp, w.PB, w.cf, w.mp, c0, st, store);
+}
+
+// step j's block column on the look-ahead stream st: the jb x jb diagonal
+// block with partial pivoting over its rows, then L21 = A21 U11^-1 (U11^-1
+// by the one-launch upper TRSM on the identity, the product on the matrix
+// cores stored over A21).  Beside it, on st2 once the diagonal block is
+// done: L11^-1 (the unit-lower TRSM on the identity) split into planes lp,
+// the A operand of the caller's U12 = L11^-1 A12.  ev_diag and ev_linv are
+// scratch events; st waits for st2 at the end.
+int block_column(c2 *A, int n, int64_t lda, int j, int jb, int *ipiv, int *flags, const LocalPtrs &P,
+                 const LinvPlanes &lp, hipStream_t st, hipStream_t st2, hipEvent_t ev_diag, hipEvent_t ev_linv)
 {
   c2 *Ajj = A + (size_t)j * lda + j;
   hipLaunchKernelGGL(k_diag_lu, dim3(1), dim3(DL_NT), 0, st, Ajj, lda, jb, ipiv + j, j, flags);
   const int below = n - j - jb;
   if (below <= 0) return BFA_OK;
-  hipLaunchKernelGGL(k_eye, dim3((jb + 255) / 256, jb), dim3(256), 0, st, Uinv, jb);
-  bfa::trsm_upper_c(jb, jb, Ajj, lda, Uinv, jb, st);
-  split_ops(below, jb, jb, Ajj + jb, lda, Uinv, jb, w, st);
-  gemm_planes(below, jb, jb, Ajj + jb, lda, w, 0, st, 1);
+  HIPCHECK(hipEventRecord(ev_diag, st));
+  HIPCHECK(hipStreamWaitEvent(st2, ev_diag, 0));
+  hipLaunchKernelGGL(k_eye, dim3((jb + 255) / 256, jb), dim3(256), 0, st2, P.Linv, jb);
+  bfa::trsm_lower_c(jb, jb, Ajj, lda, P.Linv, jb, st2);
+  hipLaunchKernelGGL(k_split_rows, dim3(LB / 32), dim3(256), 0, st2, (const c2 *)P.Linv, (int64_t)jb, jb, jb,
+                     (jb + HK - 1) / HK * HK, (int64_t)LB, lp.P, lp.rf);
+  HIPCHECK(hipEventRecord(ev_linv, st2));
+  hipLaunchKernelGGL(k_eye, dim3((jb + 255) / 256, jb), dim3(256), 0, st, P.Uinv, jb);
+  bfa::trsm_upper_c(jb, jb, Ajj, lda, P.Uinv, jb, st);
+  split_ops(below, jb, jb, Ajj + jb, lda, P.Uinv, jb, P.pan, st);
+  gemm_planes(below, jb, jb, Ajj + jb, lda, P.pan, 0, st, 1);
+  HIPCHECK(hipStreamWaitEvent(st, ev_linv, 0));
   return BFA_OK;
 }
 
@@ -861,6 +904,7 @@ void bfa::local_release()
       (void)hipDeviceSynchronize();
       if (w.mem) (void)hipFree(w.mem);
       if (w.side) (void)hipStreamDestroy(w.side);
+      if (w.side2) (void)hipStreamDestroy(w.side2);
       for (auto &e : w.ev) if (e) (void)hipEventDestroy(e);
     }
     w = LocalWork();
@@ -894,65 +938,69 @@ extern "C" int bfa_cgemm_sub_split(int m, int n, int k, const void *d_A, int64_t
   if (m < 0 || n < 0 || k < 0 || k > LB || (m > 0 && (lda < m || ldc < m)) || (k > 0 && ldb < k))
     return fail(BFA_ERR_ARG, "bfa_cgemm_sub_split: bad sizes (k <= %d)", LB);
   if (m == 0 || n == 0 || k == 0) return BFA_OK;
-  SplitWS bulk, pan;
-  c2 *Uinv;
-  LocalWork *lw;
-  int rc = local_work(m > n ? m : n, &bulk, &pan, &Uinv, &lw);
+  LocalPtrs P;
+  int rc = local_work(m > n ? m : n, &P);
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
-  split_ops(m, n, k, (const c2 *)d_A, lda, (const c2 *)d_B, ldb, bulk, st);
-  gemm_planes(m, n, k, (c2 *)d_C, ldc, bulk, 0, st, 0);
+  split_ops(m, n, k, (const c2 *)d_A, lda, (const c2 *)d_B, ldb, P.bulk, st);
+  gemm_planes(m, n, k, (c2 *)d_C, ldc, P.bulk, 0, st, 0);
   HIPCHECK(hipGetLastError());
   return BFA_OK;
 }
 
-// Look-ahead: the caller's stream st carries, per step, the row swaps, the
-// U12 TRSM, the splits and the trailing update -- the next block column
-// first; the side stream then factors that block column (diagonal block,
-// L21) while st updates the remaining columns.  The two touch disjoint
-// columns (the side stream only the next block column, st everything right
-// of it) and each has its own split planes.
+// Look-ahead: the caller's stream st carries, per step, the row swaps,
+// U12 = L11^-1 A12 (A12 split, the product stored over A12), the splits and
+// the trailing update -- the next block column first; the side streams then
+// factor that block column (diagonal block, L21, L11^-1's planes) while st
+// updates the remaining columns.  They touch disjoint columns (the side
+// streams only the next block column, st everything right of it) and each
+// has its own split planes; L11^-1's planes alternate between two sets.
 extern "C" int bfa_lu_factor_c_local(void *d_A, int n, int64_t lda, int *d_ipiv, int *info, void *stream)
 {
   if (!d_A || !d_ipiv || n <= 0 || lda < n) return fail(BFA_ERR_ARG, "bfa_lu_factor_c_local: bad argument");
   hipStream_t st = (hipStream_t)stream;
   c2 *A = (c2 *)d_A;
-  SplitWS bulk, pan;
-  c2 *Uinv;
-  LocalWork *lw;
-  int rc = local_work(n, &bulk, &pan, &Uinv, &lw);
+  LocalPtrs P;
+  int rc = local_work(n, &P);
   if (rc) return rc;
   rc = bfa::lu_local_begin(st);
   if (rc) return rc;
   int *flags = bfa::lu_local_flags();
   if (!flags) return fail(BFA_ERR_STATE, "bfa_lu_factor_c_local: LU work area");
-  hipStream_t sp = lw->side;
-  hipEvent_t ev_in = lw->ev[0], ev_col = lw->ev[1], ev_next = lw->ev[2];
-  // the side stream starts after everything queued on st (the matrix itself)
+  LocalWork *lw = P.lw;
+  hipStream_t sp = lw->side, sp2 = lw->side2;
+  hipEvent_t ev_in = lw->ev[0], ev_col = lw->ev[1], ev_next = lw->ev[2], ev_diag = lw->ev[3], ev_linv = lw->ev[4];
+  // the side streams start after everything queued on st (the matrix itself)
   HIPCHECK(hipEventRecord(ev_in, st));
   HIPCHECK(hipStreamWaitEvent(sp, ev_in, 0));
-  rc = block_column(A, n, lda, 0, n < LB ? n : LB, d_ipiv, flags, pan, Uinv, sp);
+  HIPCHECK(hipStreamWaitEvent(sp2, ev_in, 0));
+  rc = block_column(A, n, lda, 0, n < LB ? n : LB, d_ipiv, flags, P, P.lp[0], sp, sp2, ev_diag, ev_linv);
   if (rc) return rc;
   HIPCHECK(hipEventRecord(ev_col, sp));
   for (int j = 0; j < n; j += LB) {
     const int jb = n - j < LB ? n - j : LB;
-    c2 *Ajj = A + (size_t)j * lda + j;
     HIPCHECK(hipStreamWaitEvent(st, ev_col, 0));         // block column j factored
     bfa::laswp_c(A, lda, 0, j, j, jb, d_ipiv, st);
     const int rest = n - j - jb;
     if (rest <= 0) break;
     bfa::laswp_c(A, lda, j + jb, n, j, jb, d_ipiv, st);
     c2 *A12 = A + (size_t)(j + jb) * lda + j;
-    bfa::trsm_lower_c(jb, rest, Ajj, lda, A12, lda, st);
-    split_ops(rest, rest, jb, Ajj + jb, lda, A12, lda, bulk, st);
+    const LinvPlanes &lp = P.lp[(j / LB) & 1];
+    const int kp = (jb + HK - 1) / HK * HK;
+    hipLaunchKernelGGL(k_split_cols, dim3((unsigned)(P.bulk.mp / 32)), dim3(256), 0, st, (const c2 *)A12, lda, jb,
+                       rest, kp, P.bulk.mp, P.bulk.PB, P.bulk.cf);
+    gemm_ops(jb, rest, jb, A12, lda, lp.P, lp.rf, LB, P.bulk.PB, P.bulk.cf, P.bulk.mp, 0, st, 1);   // U12
+    const c2 *L21 = A + (size_t)j * lda + j + jb;
+    split_ops(rest, rest, jb, L21, lda, A12, lda, P.bulk, st);
     const int nb2 = rest < LB ? rest : LB;
-    gemm_planes(rest, nb2, jb, A12 + jb, lda, bulk, 0, st, 0);   // the next block column
+    gemm_planes(rest, nb2, jb, A12 + jb, lda, P.bulk, 0, st, 0);   // the next block column
     HIPCHECK(hipEventRecord(ev_next, st));
     HIPCHECK(hipStreamWaitEvent(sp, ev_next, 0));
-    rc = block_column(A, n, lda, j + jb, nb2, d_ipiv, flags, pan, Uinv, sp);
+    rc = block_column(A, n, lda, j + jb, nb2, d_ipiv, flags, P, P.lp[((j + jb) / LB) & 1], sp, sp2, ev_diag,
+                      ev_linv);
     if (rc) return rc;
     HIPCHECK(hipEventRecord(ev_col, sp));
-    gemm_planes(rest, rest - nb2, jb, A12 + (size_t)nb2 * lda + jb, lda, bulk, nb2, st, 0);
+    gemm_planes(rest, rest - nb2, jb, A12 + (size_t)nb2 * lda + jb, lda, P.bulk, nb2, st, 0);
   }
   HIPCHECK(hipEventRecord(ev_col, sp));
   HIPCHECK(hipStreamWaitEvent(st, ev_col, 0));
