@@ -868,9 +868,9 @@ void gemm_planes(int m, int n, int k, c2 *C, int64_t ldc, const SplitWS &w, int 
 // block with partial pivoting over its rows, then L21 = A21 U11^-1 (U11^-1
 // by the one-launch upper TRSM on the identity, the product on the matrix
 // cores stored over A21).  Beside it, on st2 once the diagonal block is
-// done: L11^-1 (the unit-lower TRSM on the identity) split into planes lp,
-// the A operand of the caller's U12 = L11^-1 A12.  ev_diag and ev_linv are
-// scratch events; st waits for st2 at the end.
+// done: A21's planes, then L11^-1 (the unit-lower TRSM on the identity)
+// split into planes lp, the A operand of the caller's U12 = L11^-1 A12.
+// ev_diag and ev_linv are scratch events; st waits for st2 at the end.
 int block_column(c2 *A, int n, int64_t lda, int j, int jb, int *ipiv, int *flags, const LocalPtrs &P,
                  const LinvPlanes &lp, hipStream_t st, hipStream_t st2, hipEvent_t ev_diag, hipEvent_t ev_linv)
 {
@@ -880,14 +880,23 @@ int block_column(c2 *A, int n, int64_t lda, int j, int jb, int *ipiv, int *flags
   if (below <= 0) return BFA_OK;
   HIPCHECK(hipEventRecord(ev_diag, st));
   HIPCHECK(hipStreamWaitEvent(st2, ev_diag, 0));
+  const int kp = (jb + HK - 1) / HK * HK;
+  // st2: A21's planes (they need only the diagonal block's row swaps, which
+  // do not reach below it), then L11^-1 and its planes
+  hipLaunchKernelGGL(k_split_rows, dim3((unsigned)(P.pan.mp / 32)), dim3(256), 0, st2, (const c2 *)(Ajj + jb), lda,
+                     below, jb, kp, P.pan.mp, P.pan.PA, P.pan.rf);
+  HIPCHECK(hipEventRecord(ev_diag, st2));                 // A21's planes done
   hipLaunchKernelGGL(k_eye, dim3((jb + 255) / 256, jb), dim3(256), 0, st2, P.Linv, jb);
   bfa::trsm_lower_c(jb, jb, Ajj, lda, P.Linv, jb, st2);
   hipLaunchKernelGGL(k_split_rows, dim3(LB / 32), dim3(256), 0, st2, (const c2 *)P.Linv, (int64_t)jb, jb, jb,
-                     (jb + HK - 1) / HK * HK, (int64_t)LB, lp.P, lp.rf);
+                     kp, (int64_t)LB, lp.P, lp.rf);
   HIPCHECK(hipEventRecord(ev_linv, st2));
+  // st: U11^-1, its planes, L21 = A21 U11^-1 over A21 once st2's planes of A21 exist
   hipLaunchKernelGGL(k_eye, dim3((jb + 255) / 256, jb), dim3(256), 0, st, P.Uinv, jb);
   bfa::trsm_upper_c(jb, jb, Ajj, lda, P.Uinv, jb, st);
-  split_ops(below, jb, jb, Ajj + jb, lda, P.Uinv, jb, P.pan, st);
+  hipLaunchKernelGGL(k_split_cols, dim3((unsigned)(P.pan.mp / 32)), dim3(256), 0, st, (const c2 *)P.Uinv,
+                     (int64_t)jb, jb, jb, kp, P.pan.mp, P.pan.PB, P.pan.cf);
+  HIPCHECK(hipStreamWaitEvent(st, ev_diag, 0));
   gemm_planes(below, jb, jb, Ajj + jb, lda, P.pan, 0, st, 1);
   HIPCHECK(hipStreamWaitEvent(st, ev_linv, 0));
   return BFA_OK;
@@ -980,9 +989,11 @@ extern "C" int bfa_lu_factor_c_local(void *d_A, int n, int64_t lda, int *d_ipiv,
   for (int j = 0; j < n; j += LB) {
     const int jb = n - j < LB ? n - j : LB;
     HIPCHECK(hipStreamWaitEvent(st, ev_col, 0));         // block column j factored
-    bfa::laswp_c(A, lda, 0, j, j, jb, d_ipiv, st);
     const int rest = n - j - jb;
-    if (rest <= 0) break;
+    if (rest <= 0) {
+      bfa::laswp_c(A, lda, 0, j, j, jb, d_ipiv, st);
+      break;
+    }
     bfa::laswp_c(A, lda, j + jb, n, j, jb, d_ipiv, st);
     c2 *A12 = A + (size_t)(j + jb) * lda + j;
     const LinvPlanes &lp = P.lp[(j / LB) & 1];
@@ -1000,6 +1011,8 @@ extern "C" int bfa_lu_factor_c_local(void *d_A, int n, int64_t lda, int *d_ipiv,
                       ev_linv);
     if (rc) return rc;
     HIPCHECK(hipEventRecord(ev_col, sp));
+    // the left columns' interchanges are off the look-ahead chain
+    bfa::laswp_c(A, lda, 0, j, j, jb, d_ipiv, st);
     gemm_planes(rest, rest - nb2, jb, A12 + (size_t)nb2 * lda + jb, lda, P.bulk, nb2, st, 0);
   }
   HIPCHECK(hipEventRecord(ev_col, sp));
