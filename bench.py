@@ -235,6 +235,9 @@ def main():
     ap.add_argument("--batch-count", type=int, default=48)
     ap.add_argument("--refine-tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 1 GiB device-to-device copy (1 GiB read, 1 GiB written), against which "
+                         "counter runs scale FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--quiet", action="store_true")
     args = ap.parse_args()
 
@@ -249,6 +252,13 @@ def main():
 
     import torch
     bfa = load_bfa()
+    if args.copy_calibration:
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        torch.cuda.empty_cache()
     if args.workload == "batch":
         return bench_batch(args, bfa, torch, log)
     dev = torch.device("cuda:0")

@@ -512,11 +512,10 @@ static void trsm(int kb, int n, const cplx<T> *L, int64_t ldl, cplx<T> *B, int64
 #define TF_MAXK 256
 template <typename T> struct TfShape { static constexpr int SC = 32768 / (TF_MAXK * (int)sizeof(cplx<T>)); };
 
-template <typename T>
+template <typename T, int SC = TfShape<T>::SC>
 __device__ __forceinline__ void trsm_lf_body(int kb, int n, const cplx<T> *__restrict__ L, int64_t ldl, cplx<T> *B,
-                                             int64_t ldb, int c0, cplx<T> (&X)[TfShape<T>::SC][TF_MAXK])
+                                             int64_t ldb, int c0, cplx<T> (&X)[SC][TF_MAXK])
 {
-  constexpr int SC = TfShape<T>::SC;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nc = min(SC, n - c0);
   for (int idx = tid; idx < kb * SC; idx += blockDim.x) {
@@ -574,22 +573,21 @@ __device__ __forceinline__ void trsm_lf_body(int kb, int n, const cplx<T> *__res
   }
 }
 
-template <typename T>
+template <typename T, int SC = TfShape<T>::SC>
 __global__ __launch_bounds__(256) void k_trsm_lf(int kb, int n, const cplx<T> *__restrict__ L, int64_t ldl, cplx<T> *B,
                                                  int64_t ldb)
 {
-  __shared__ cplx<T> X[TfShape<T>::SC][TF_MAXK];
-  trsm_lf_body<T>(kb, n, L, ldl, B, ldb, blockIdx.x * TfShape<T>::SC, X);
+  __shared__ cplx<T> X[SC][TF_MAXK];
+  trsm_lf_body<T, SC>(kb, n, L, ldl, B, ldb, blockIdx.x * SC, X);
 }
 
 // B(kb x n) := U^-1 B, U upper with its diagonal, kb <= TF_MAXK: the mirror
 // of trsm_lf_body (32-row blocks from the bottom; in the diagonal block lane
 // k divides by U(k,k) when its value is final, then passes it on)
-template <typename T>
+template <typename T, int SC = TfShape<T>::SC>
 __global__ __launch_bounds__(256) void k_trsm_uf(int kb, int n, const cplx<T> *__restrict__ U, int64_t ldu, cplx<T> *B,
                                                  int64_t ldb)
 {
-  constexpr int SC = TfShape<T>::SC;
   __shared__ cplx<T> X[SC][TF_MAXK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int c0 = blockIdx.x * SC, nc = min(SC, n - c0);
@@ -1844,13 +1842,21 @@ int lu_local_end(int *info, hipStream_t st)
 }
 void laswp_c(void *A, int64_t lda, int c0, int c1, int k0, int kb, const int *ipiv, hipStream_t st)
 { laswp<float>((cplx<float> *)A, lda, c0, c1, k0, kb, ipiv, st); }
+// Narrow column strips (4 right-hand sides per workgroup): these two serve
+// the block-local LU's triangular inverses (256 right-hand sides), whose
+// one-launch latency is the chain of eight 32-row blocks; a narrower strip
+// makes each block's update 4x shorter and the grid 4x wider
 void trsm_lower_c(int kb, int n, const void *L, int64_t ldl, void *B, int64_t ldb, hipStream_t st)
-{ trsm_lower<float>(kb, n, (const cplx<float> *)L, ldl, (cplx<float> *)B, ldb, st); }
+{
+  if (kb <= 0 || n <= 0) return;
+  if (kb > TF_MAXK) { trsm_lower<float>(kb, n, (const cplx<float> *)L, ldl, (cplx<float> *)B, ldb, st); return; }
+  hipLaunchKernelGGL((k_trsm_lf<float, 4>), dim3((n + 3) / 4), dim3(256), 0, st, kb, n, (const cplx<float> *)L, ldl,
+                     (cplx<float> *)B, ldb);
+}
 void trsm_upper_c(int kb, int n, const void *U, int64_t ldu, void *B, int64_t ldb, hipStream_t st)
 {
   if (kb <= 0 || n <= 0) return;
-  constexpr int SC = TfShape<float>::SC;
-  hipLaunchKernelGGL(k_trsm_uf<float>, dim3((n + SC - 1) / SC), dim3(256), 0, st, kb, n, (const cplx<float> *)U, ldu,
+  hipLaunchKernelGGL((k_trsm_uf<float, 4>), dim3((n + 3) / 4), dim3(256), 0, st, kb, n, (const cplx<float> *)U, ldu,
                      (cplx<float> *)B, ldb);
 }
 int *lu_local_flags()
