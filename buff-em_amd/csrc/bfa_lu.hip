@@ -1842,21 +1842,22 @@ int lu_local_end(int *info, hipStream_t st)
 }
 void laswp_c(void *A, int64_t lda, int c0, int c1, int k0, int kb, const int *ipiv, hipStream_t st)
 { laswp<float>((cplx<float> *)A, lda, c0, c1, k0, kb, ipiv, st); }
-// Narrow column strips (4 right-hand sides per workgroup): these two serve
+// Narrow column strips (2 right-hand sides per workgroup): these two serve
 // the block-local LU's triangular inverses (256 right-hand sides), whose
 // one-launch latency is the chain of eight 32-row blocks; a narrower strip
-// makes each block's update 4x shorter and the grid 4x wider
+// makes each block's update shorter and the grid wider (U^-1 of a 256 x 256
+// block: 146 us with 16 columns per workgroup, 86 with 4, 76 with 2)
 void trsm_lower_c(int kb, int n, const void *L, int64_t ldl, void *B, int64_t ldb, hipStream_t st)
 {
   if (kb <= 0 || n <= 0) return;
   if (kb > TF_MAXK) { trsm_lower<float>(kb, n, (const cplx<float> *)L, ldl, (cplx<float> *)B, ldb, st); return; }
-  hipLaunchKernelGGL((k_trsm_lf<float, 4>), dim3((n + 3) / 4), dim3(256), 0, st, kb, n, (const cplx<float> *)L, ldl,
+  hipLaunchKernelGGL((k_trsm_lf<float, 2>), dim3((n + 1) / 2), dim3(256), 0, st, kb, n, (const cplx<float> *)L, ldl,
                      (cplx<float> *)B, ldb);
 }
 void trsm_upper_c(int kb, int n, const void *U, int64_t ldu, void *B, int64_t ldb, hipStream_t st)
 {
   if (kb <= 0 || n <= 0) return;
-  hipLaunchKernelGGL((k_trsm_uf<float, 4>), dim3((n + 3) / 4), dim3(256), 0, st, kb, n, (const cplx<float> *)U, ldu,
+  hipLaunchKernelGGL((k_trsm_uf<float, 2>), dim3((n + 1) / 2), dim3(256), 0, st, kb, n, (const cplx<float> *)U, ldu,
                      (cplx<float> *)B, ldb);
 }
 int *lu_local_flags()
