@@ -417,10 +417,12 @@ __global__ __launch_bounds__(512, 1) void k_gemm_hs(int m, int n, int kp, const 
 //           stores);
 //   U12     L11^-1 A12 for the panel's rows in four blocks of eight rows,
 //           into A and LDS;
-//   update  A22 -= L21 U12 in 16 x 16 tiles on v_mfma_f32_16x16x4f32 (U12's
-//           value as the first operand, so that register v of lane l holds
-//           row l % 16, column 4 (l / 16) + v of the tile), four tiles of a
-//           wave at a time.
+//   update  L21 and U12 split into FP16 hi + lo planes (each row of L21 /
+//           column of U12 scaled by a power of two, as in the trailing
+//           update), then A22 -= L21 U12 in 16 x 16 tiles, twelve
+//           v_mfma_f32_16x16x32_f16 per tile (U12's fragment as the first
+//           operand, so that register v of lane l holds row l % 16, column
+//           4 (l / 16) + v of the tile), four tiles of a wave at a time.
 // Barriers separate the phases; a workgroup barrier also orders the global
 // stores of one phase before the loads of the next.  Pivots are written
 // absolute (piv_off + local row).
@@ -460,9 +462,21 @@ __device__ __forceinline__ unsigned long long wave_max_key(unsigned long long k)
   return ((unsigned long long)hi << 32) | lo;
 }
 
+// 16-byte chunk position of a split plane row (64 bytes = 4 chunks):
+// chunk ^ swizzle(row), so that 16 lanes reading one chunk of 16
+// consecutive rows hit distinct banks
+__device__ __forceinline__ int diag_swz(int r) { return ((r >> 1) ^ (r >> 3)) & 3; }
+
 struct DiagShared {
-  c2 L[DL_M][LEAF_W + 1];   // the panel by position, rows c0..jb-1 (stride 33: 16 lanes on distinct banks)
-  c2 U[LEAF_W][DL_M];       // U12, columns counted from c0 + 32; during the leaf, the finished columns by data row
+  union {
+    c2 L[DL_M][LEAF_W + 1];   // the panel by position, rows c0..jb-1 (stride 33: 16 lanes on distinct banks)
+    _Float16 LP[4][DL_M - LEAF_W][LEAF_W];   // then L21's split planes (re hi, re lo, im hi, im lo), 16-byte chunks swizzled
+  };
+  union {
+    c2 U[LEAF_W][DL_M];       // U12, columns counted from c0 + 32
+    _Float16 UP[4][DL_M - LEAF_W][LEAF_W];   // then U12's split planes by column
+  };
+  float rfs[DL_M], cfs[DL_M];   // 2^-e of L21's rows and U12's columns
   unsigned long long wkey[2][4];  // argmax keys of the column's four waves (slots alternate between columns)
   c2 colv[2][DL_M];           // the current column by data row
   int piv[LEAF_W];            // the panel's pivots (local positions)
@@ -606,11 +620,12 @@ __global__ __launch_bounds__(DL_NT) void k_diag_lu(c2 *A, int64_t lda, int jb, i
     __syncthreads();
     const int nr = jb - c0 - LEAF_W;  // rows and columns of A22 (a full panel when nr > 0)
     if (nr <= 0) break;
-    // U12 = L11^-1 A12: four threads per column, thread q holding rows
+    // U12 = L11^-1 A12: four threads per column, thread q (= wave / 4, so
+    // that the branches and L's reads are uniform per wave) holding rows
     // 8q..8q+7; block q solved by its thread once blocks < q have been
     // subtracted, then handed to the threads below through LDS
     {
-      const int cl = t >> 2, q = t & 3;
+      const int cl = t & (DL_M - 1), q = t >> 8;
       const bool on = cl < nr;
       c2 *col = A + (size_t)(c0 + LEAF_W + cl) * lda + c0 + 8 * q;
       c2 a[8];
@@ -649,13 +664,64 @@ __global__ __launch_bounds__(DL_NT) void k_diag_lu(c2 *A, int64_t lda, int jb, i
         for (int i = 0; i < 8; i++) col[i] = a[i];
     }
     __syncthreads();
-    // A22 -= L21 U12, four tiles of a wave at a time: their loads of A22
-    // first, then the products
-    const int nt = (nr + 15) / 16;
-    for (int tile0 = wv * 4; tile0 < nt * nt; tile0 += DL_NW * 4) {
-      c2 cv[4][4];
+    // L21 and U12 to split FP16 planes (each row of L21 / column of U12
+    // scaled by a power of two that brings its largest magnitude into
+    // [2^14, 2^15), x = hi + lo), written over their FP32 copies: four
+    // threads per row / column, eight k each, the maximum over the four by
+    // DPP within the quad
+    {
+      const int rr = t >> 2, kc = t & 3;
+      const bool lon = rr < nr;
+      c2 lv[8], uv[8];
+      float lm = 0.0f, um = 0.0f;
 #pragma unroll
-      for (int g = 0; g < 4; g++) {
+      for (int i = 0; i < 8; i++) {
+        lv[i] = lon ? sh.L[LEAF_W + rr][8 * kc + i] : c2{0.0f, 0.0f};
+        uv[i] = lon ? sh.U[8 * kc + i][rr] : c2{0.0f, 0.0f};
+        lm = fmaxf(lm, fmaxf(fabsf(lv[i].re), fabsf(lv[i].im)));
+        um = fmaxf(um, fmaxf(fabsf(uv[i].re), fabsf(uv[i].im)));
+      }
+      lm = fmaxf(lm, __shfl_xor(lm, 1));
+      lm = fmaxf(lm, __shfl_xor(lm, 2));
+      um = fmaxf(um, __shfl_xor(um, 1));
+      um = fmaxf(um, __shfl_xor(um, 2));
+      const int le = split_exp(lm), ue = split_exp(um);
+      __syncthreads();
+      if (rr < DL_M - LEAF_W) {
+        const float ls = ldexpf(1.0f, le), us = ldexpf(1.0f, ue);
+        if (kc == 0) {
+          sh.rfs[rr] = ldexpf(1.0f, -le);
+          sh.cfs[rr] = ldexpf(1.0f, -ue);
+        }
+        h8 l0, l1, l2, l3, u0, u1, u2, u3;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          _Float16 h, o;
+          split_one(lv[i].re * ls, h, o); l0[i] = h; l1[i] = o;
+          split_one(lv[i].im * ls, h, o); l2[i] = h; l3[i] = o;
+          split_one(uv[i].re * us, h, o); u0[i] = h; u1[i] = o;
+          split_one(uv[i].im * us, h, o); u2[i] = h; u3[i] = o;
+        }
+        const int pos = 8 * (kc ^ diag_swz(rr));
+        *(h8 *)&sh.LP[0][rr][pos] = l0;
+        *(h8 *)&sh.LP[1][rr][pos] = l1;
+        *(h8 *)&sh.LP[2][rr][pos] = l2;
+        *(h8 *)&sh.LP[3][rr][pos] = l3;
+        *(h8 *)&sh.UP[0][rr][pos] = u0;
+        *(h8 *)&sh.UP[1][rr][pos] = u1;
+        *(h8 *)&sh.UP[2][rr][pos] = u2;
+        *(h8 *)&sh.UP[3][rr][pos] = u3;
+      }
+    }
+    __syncthreads();
+    // A22 -= L21 U12 in 16 x 16 tiles, each k = 32 on twelve
+    // v_mfma_f32_16x16x32_f16 (hi*hi + hi*lo + lo*hi per real product), two
+    // tiles of a wave at a time: their loads of A22 first, then the products
+    const int nt = (nr + 15) / 16;
+    for (int tile0 = wv * 2; tile0 < nt * nt; tile0 += DL_NW * 2) {
+      c2 cv[2][4];
+#pragma unroll
+      for (int g = 0; g < 2; g++) {
         const int tile = tile0 + g, tr = tile % nt, tc = tile / nt;
         const int row = tr * 16 + l16;
 #pragma unroll
@@ -666,28 +732,39 @@ __global__ __launch_bounds__(DL_NT) void k_diag_lu(c2 *A, int64_t lda, int jb, i
         }
       }
 #pragma unroll
-      for (int g = 0; g < 4; g++) {
+      for (int g = 0; g < 2; g++) {
         const int tile = tile0 + g, tr = tile % nt, tc = tile / nt;
         if (tile >= nt * nt) break;
-        const int row = tr * 16 + l16;
-        f4 pr = {0.0f, 0.0f, 0.0f, 0.0f}, pi = {0.0f, 0.0f, 0.0f, 0.0f};
-        const int cu = tc * 16 + l16;           // U12 column of this lane's first operand
-        const bool uok = cu < nr, lok = row < nr;
+        const int row = tr * 16 + l16, cu = tc * 16 + l16;   // this lane's L21 row and U12 column
+        h8 a[4], bb[4];
 #pragma unroll
-        for (int k0 = 0; k0 < LEAF_W; k0 += 4) {
-          const int kk = k0 + lq;
-          const c2 uv = uok ? sh.U[kk][cu] : c2{0.0f, 0.0f};
-          const c2 lv = lok ? sh.L[LEAF_W + row][kk] : c2{0.0f, 0.0f};
-          pr = __builtin_amdgcn_mfma_f32_16x16x4f32(uv.re, lv.re, pr, 0, 0, 0);
-          pi = __builtin_amdgcn_mfma_f32_16x16x4f32(uv.re, lv.im, pi, 0, 0, 0);
-          pr = __builtin_amdgcn_mfma_f32_16x16x4f32(-uv.im, lv.im, pr, 0, 0, 0);
-          pi = __builtin_amdgcn_mfma_f32_16x16x4f32(uv.im, lv.re, pi, 0, 0, 0);
+        for (int q = 0; q < 4; q++) {
+          a[q] = *(const h8 *)&sh.LP[q][row][8 * (lq ^ diag_swz(row))];
+          bb[q] = *(const h8 *)&sh.UP[q][cu][8 * (lq ^ diag_swz(cu))];
         }
+        const h8 n2 = hneg(a[2]), n3 = hneg(a[3]);
+        f4 pr = {0.0f, 0.0f, 0.0f, 0.0f}, pi = {0.0f, 0.0f, 0.0f, 0.0f};
+        pr = __builtin_amdgcn_mfma_f32_16x16x32_f16(bb[0], a[0], pr, 0, 0, 0);
+        pi = __builtin_amdgcn_mfma_f32_16x16x32_f16(bb[2], a[0], pi, 0, 0, 0);
+        pr = __builtin_amdgcn_mfma_f32_16x16x32_f16(bb[1], a[0], pr, 0, 0, 0);
+        pi = __builtin_amdgcn_mfma_f32_16x16x32_f16(bb[3], a[0], pi, 0, 0, 0);
+        pr = __builtin_amdgcn_mfma_f32_16x16x32_f16(bb[0], a[1], pr, 0, 0, 0);
+        pi = __builtin_amdgcn_mfma_f32_16x16x32_f16(bb[2], a[1], pi, 0, 0, 0);
+        pr = __builtin_amdgcn_mfma_f32_16x16x32_f16(bb[2], n2, pr, 0, 0, 0);
+        pi = __builtin_amdgcn_mfma_f32_16x16x32_f16(bb[0], a[2], pi, 0, 0, 0);
+        pr = __builtin_amdgcn_mfma_f32_16x16x32_f16(bb[3], n2, pr, 0, 0, 0);
+        pi = __builtin_amdgcn_mfma_f32_16x16x32_f16(bb[1], a[2], pi, 0, 0, 0);
+        pr = __builtin_amdgcn_mfma_f32_16x16x32_f16(bb[2], n3, pr, 0, 0, 0);
+        pi = __builtin_amdgcn_mfma_f32_16x16x32_f16(bb[0], a[3], pi, 0, 0, 0);
+        const float rf = row < nr ? sh.rfs[row] : 0.0f;
 #pragma unroll
         for (int v = 0; v < 4; v++) {
           const int cl = tc * 16 + 4 * lq + v;
-          if (row < nr && cl < nr)
-            A[(size_t)(c0 + LEAF_W + cl) * lda + c0 + LEAF_W + row] = {cv[g][v].re - pr[v], cv[g][v].im - pi[v]};
+          if (row < nr && cl < nr) {
+            const float f = rf * sh.cfs[cl];
+            A[(size_t)(c0 + LEAF_W + cl) * lda + c0 + LEAF_W + row] = {cv[g][v].re - pr[v] * f,
+                                                                         cv[g][v].im - pi[v] * f};
+          }
         }
       }
     }
