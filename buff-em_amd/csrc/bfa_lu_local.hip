@@ -495,8 +495,8 @@ struct DiagShared {
 // is touched (a barrier would wait for it); the pivots collect in LDS.
 // The slots alternate with the column's parity.
 template <int U>
-__device__ __forceinline__ void leaf_col(c2 (&x)[8], int &pos, bool act, int q, int r, int t, int lane, int wv,
-                                         int c0, int W, int *flags, int piv_off, DiagShared &sh)
+__device__ __forceinline__ void leaf_col(c2 (&x)[8], int &pos, int &nmv, bool act, int q, int r, int t, int lane,
+                                         int wv, int c0, int W, int *flags, int piv_off, DiagShared &sh)
 {
   const int C = 8 * W + U, d = c0 + C, b = U & 1;
   if (q == W) {
@@ -516,10 +516,9 @@ __device__ __forceinline__ void leaf_col(c2 (&x)[8], int &pos, bool act, int q, 
   if (t == 0) {
     sh.piv[C] = bp;
     if (best > 0.0f && bp != d) {
-      const int k = sh.nmv;
-      sh.mv[k] = d;
-      sh.mv[k + 1] = bp;
-      sh.nmv = k + 2;
+      sh.mv[nmv] = d;
+      sh.mv[nmv + 1] = bp;
+      nmv += 2;
     }
   }
   if (best > 0.0f) {
@@ -560,11 +559,10 @@ __global__ __launch_bounds__(DL_NT) void k_diag_lu(c2 *A, int64_t lda, int jb, i
     const int pw = min(LEAF_W, jb - c0);
     // leaf: thread (row r = t % 256, quarter q = t / 256) holds columns
     // 8q..8q+7 of data row r; windows of eight columns, one quarter at a time
-    if (t == 0) sh.nmv = 0;
     {
       const int r = t & (DL_M - 1), q = t >> 8;
       const bool act = r >= c0 && r < jb;
-      int pos = r;
+      int pos = r, nmv = 0;          // nmv: thread 0's count of the moved list
       c2 x[8];
 #pragma unroll
       for (int j = 0; j < 8; j++)
@@ -572,14 +570,14 @@ __global__ __launch_bounds__(DL_NT) void k_diag_lu(c2 *A, int64_t lda, int jb, i
 #pragma unroll 1
       for (int W = 0; 8 * W < pw; W++) {
         const int left = pw - 8 * W;
-        leaf_col<0>(x, pos, act, q, r, t, lane, wv, c0, W, flags, piv_off, sh);
-        if (left > 1) leaf_col<1>(x, pos, act, q, r, t, lane, wv, c0, W, flags, piv_off, sh);
-        if (left > 2) leaf_col<2>(x, pos, act, q, r, t, lane, wv, c0, W, flags, piv_off, sh);
-        if (left > 3) leaf_col<3>(x, pos, act, q, r, t, lane, wv, c0, W, flags, piv_off, sh);
-        if (left > 4) leaf_col<4>(x, pos, act, q, r, t, lane, wv, c0, W, flags, piv_off, sh);
-        if (left > 5) leaf_col<5>(x, pos, act, q, r, t, lane, wv, c0, W, flags, piv_off, sh);
-        if (left > 6) leaf_col<6>(x, pos, act, q, r, t, lane, wv, c0, W, flags, piv_off, sh);
-        if (left > 7) leaf_col<7>(x, pos, act, q, r, t, lane, wv, c0, W, flags, piv_off, sh);
+        leaf_col<0>(x, pos, nmv, act, q, r, t, lane, wv, c0, W, flags, piv_off, sh);
+        if (left > 1) leaf_col<1>(x, pos, nmv, act, q, r, t, lane, wv, c0, W, flags, piv_off, sh);
+        if (left > 2) leaf_col<2>(x, pos, nmv, act, q, r, t, lane, wv, c0, W, flags, piv_off, sh);
+        if (left > 3) leaf_col<3>(x, pos, nmv, act, q, r, t, lane, wv, c0, W, flags, piv_off, sh);
+        if (left > 4) leaf_col<4>(x, pos, nmv, act, q, r, t, lane, wv, c0, W, flags, piv_off, sh);
+        if (left > 5) leaf_col<5>(x, pos, nmv, act, q, r, t, lane, wv, c0, W, flags, piv_off, sh);
+        if (left > 6) leaf_col<6>(x, pos, nmv, act, q, r, t, lane, wv, c0, W, flags, piv_off, sh);
+        if (left > 7) leaf_col<7>(x, pos, nmv, act, q, r, t, lane, wv, c0, W, flags, piv_off, sh);
       }
       // every row to its final position, in A and in LDS; the pivots out
       if (act) {
@@ -591,6 +589,7 @@ __global__ __launch_bounds__(DL_NT) void k_diag_lu(c2 *A, int64_t lda, int jb, i
         }
       }
       if (t < pw) ipiv[c0 + t] = piv_off + sh.piv[t];
+      if (t == 0) sh.nmv = nmv;
     }
     __syncthreads();
     // swaps: each position in the moved list (both rows of every interchange,
