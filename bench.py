@@ -29,11 +29,12 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_FP16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 
-# HBM bytes per launch of the roofline kernel (its launches on the LU's update
-# CUs) at N = 19,872, measured by two rocprofv3 --pmc passes (FETCH_SIZE x2,
-# WRITE_SIZE x1; the scale factors are read off a streaming kernel of known
-# size in the same run)
-MEASURED_TRAFFIC = {("rocblas_cgemm", 19872): 1.788e9}
+# HBM bytes per launch of the roofline kernel at N = 19,872, measured by two
+# rocprofv3 --pmc passes (FETCH_SIZE x2, WRITE_SIZE x1; the scale factors are
+# read off a copy of known size in the same run): rocBLAS cgemm's launches on
+# the LU's update CUs (mixed form), and the average over all k_gemm_hs
+# launches of the fast form's factorisation
+MEASURED_TRAFFIC = {("rocblas_cgemm", 19872): 1.788e9, ("k_gemm_hs", 19872): 0.771e9}
 
 
 def load_bfa():
@@ -345,11 +346,13 @@ def main():
         prof_slice = bfa.profile_read(4 if form == "fp64" else 5)
         prof = bfa.profile_read(0 if form == "fp64" else 1)
         prof_split = bfa.profile_read(6)
+        prof_split_bytes = bfa.profile_read(7)[1]
         for k in phase:
             phase[k] /= args.steps
         P = g.emt_power(omega, pw, dB)
         results[form] = {"s_per_freq": t_total / args.steps, "phases_s": dict(phase), "gemm": prof,
                          "gemm_big": prof_big, "gemm_slice": prof_slice, "gemm_split": prof_split,
+                         "gemm_split_bytes": prof_split_bytes,
                          "J": dB.cpu().numpy().T.copy(),
                          "P": P.copy(), "refine": dict(refine) if form != "fp64" else None}
         log(f"{form}: {t_total / args.steps:.4f} s/frequency  phases {phase}  refine {refine if form != 'fp64' else ''}")
@@ -418,8 +421,11 @@ def fast_roofline(rk, N, steps):
         "kernel": "k_gemm_hs (LU trailing update: split-FP16 complex product on v_mfma_f32_16x16x32_f16)",
         "bound": "mfma", "unit": "TFLOP/s", "achieved": achieved, "peak": PEAK_FP16_TFLOPS,
         "frac": achieved / PEAK_FP16_TFLOPS if achieved else None,
+        # HBM bytes per launch (rocprofv3 --pmc, all launches of the factorisation)
         "traffic": MEASURED_TRAFFIC.get(("k_gemm_hs", N)),
         "per_launch_flops": f16 / max(nl, 1), "avg_launch_ms": ms / max(nl, 1),
+        # A and B split planes read once, C read and written (written only where stored)
+        "per_launch_algorithmic_bytes": rk["gemm_split_bytes"] / max(nl, 1),
         "launches_per_step": nl / steps,
         # the same launches as a complex64 GEMM (8 mnk flops per update)
         "complex_gemm_equiv_tflops": fl / (ms * 1e-3) / 1e12 if ms > 0 else None,

@@ -44,4 +44,7 @@ bool profile_on();
 void lu_release();
 void local_release();
 int profile_read_split(double *ms, double *flops, int *launches);
+// the algorithmic bytes and the launch count of the split launches returned
+// by the last profile_read_split
+int profile_split_bytes(double *bytes, int *launches);
 }  // namespace bfa

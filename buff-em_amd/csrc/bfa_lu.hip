@@ -2066,9 +2066,15 @@ extern "C" int bfa_profile_read(int which, double *ms, double *flops, int *launc
 {
   // which: 0 / 1 = every complex128 / complex64 trailing-update launch;
   // 2 / 3 = the large updates (m, n >= 256) on all or the update CUs;
-  // 4 / 5 = the large updates run as column slices on the panel CUs
+  // 4 / 5 = the large updates run as column slices on the panel CUs;
+  // 6 = the fast form's split-FP16 launches (read and cleared); 7 = the
+  // algorithmic bytes of the launches the last read of 6 returned, in *flops
   if (which == 6) return bfa::profile_read_split(ms, flops, launches);
-  if (which < 0 || which > 6) return fail(BFA_ERR_ARG, "bfa_profile_read: which must be 0..6");
+  if (which == 7) {
+    if (ms) *ms = 0.0;
+    return bfa::profile_split_bytes(flops, launches);
+  }
+  if (which < 0 || which > 7) return fail(BFA_ERR_ARG, "bfa_profile_read: which must be 0..7");
   const int type = which & 1;
   const int lo = which < 2 ? 3 * type : which < 4 ? 3 * type + 1 : 3 * type + 2;
   const int hi = which < 2 ? 3 * type + 2 : lo;

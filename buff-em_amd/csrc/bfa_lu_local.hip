@@ -867,9 +867,12 @@ struct SplitProfile {
   std::vector<hipEvent_t> pool;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> rec;
   double flops = 0.0;
+  double bytes = 0.0;       // algorithmic: A and B planes once, C read + written (or written when stored)
   size_t used = 0;
 };
 SplitProfile g_sp;
+double g_sp_last_bytes = 0.0;
+int g_sp_last_launches = 0;
 
 hipEvent_t sp_event(hipStream_t st)
 {
@@ -930,6 +933,7 @@ void gemm_ops(int m, int n, int k, c2 *C, int64_t ldc, const _Float16 *PA, const
     if (e0 && e1) {
       g_sp.rec.push_back({e0, e1});
       g_sp.flops += 8.0 * m * (double)n * k;
+      g_sp.bytes += 8.0 * k * ((double)m + n) + (store ? 8.0 : 16.0) * m * (double)n;
     }
   }
 }
@@ -1010,9 +1014,19 @@ int bfa::profile_read_split(double *ms, double *flops, int *launches)
   if (ms) *ms = t;
   if (flops) *flops = g_sp.flops;
   if (launches) *launches = (int)g_sp.rec.size();
+  g_sp_last_bytes = g_sp.bytes;
+  g_sp_last_launches = (int)g_sp.rec.size();
   g_sp.rec.clear();
   g_sp.flops = 0.0;
+  g_sp.bytes = 0.0;
   g_sp.used = 0;
+  return BFA_OK;
+}
+
+int bfa::profile_split_bytes(double *bytes, int *launches)
+{
+  if (bytes) *bytes = g_sp_last_bytes;
+  if (launches) *launches = g_sp_last_launches;
   return BFA_OK;
 }
 

@@ -373,7 +373,10 @@ int  bfa_surr_params(bfa_surrogate *s, float *out, const float *in);
  * updates (m, n >= 256) on all CUs or, in the CU-partitioned LU, on the
  * update CUs; 4 / 5 the large updates run as column slices on the panel
  * CUs; reading 0 / 1 resets that type; 6 = the split-FP16 trailing updates
- * of bfa_lu_factor_c_local (flops counted as 8mnk; reading resets).        */
+ * of bfa_lu_factor_c_local (flops counted as 8mnk; reading resets); 7 = the
+ * algorithmic bytes (A and B operands once, C read and written, or written
+ * where the product is stored) of the launches the last read of 6
+ * returned, in *flops, with their count (ms = 0).                          */
 int  bfa_profile_enable(int on);
 int  bfa_profile_read(int which, double *ms, double *flops, int *launches);
 
