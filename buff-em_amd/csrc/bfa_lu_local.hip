@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <cstdint>
 #include <utility>
 #include <vector>
@@ -74,12 +75,14 @@ __device__ __forceinline__ void split_one(float x, _Float16 &hi, _Float16 &lo)
 // ---------------------------------------------------------------------------
 // L21 (m x k, column-major) -> planes P[4][mp][kp] (re hi, re lo, im hi,
 // im lo; row-major, k contiguous) scaled by 2^e per row, rfac[r] = 2^-e.
-// Rows in [m, mp) and k in [k, kp) are zero.  Block of 32 rows; thread
-// (row = t % 32, chunk = t / 32) holds 32 consecutive k of its row.
-__global__ __launch_bounds__(256) void k_split_rows(const c2 *__restrict__ A, int64_t lda, int m, int k, int kp,
+// Rows in [m, mp) and k in [k, kp) are zero.  Block of 32 rows and KC
+// chunks of 32 k (k <= 32 KC); thread (row = t % 32, chunk = t / 32) holds
+// 32 consecutive k of its row.
+template <int KC>
+__global__ __launch_bounds__(32 * KC) void k_split_rows(const c2 *__restrict__ A, int64_t lda, int m, int k, int kp,
                                                     int64_t mp, _Float16 *P, float *rfac)
 {
-  __shared__ float smax[8][32];
+  __shared__ float smax[KC][32];
   const int row = threadIdx.x & 31, kc = threadIdx.x >> 5;
   const int r = blockIdx.x * 32 + row;
   float re[32], im[32];
@@ -97,7 +100,7 @@ __global__ __launch_bounds__(256) void k_split_rows(const c2 *__restrict__ A, in
   __syncthreads();
   mx = 0.0f;
 #pragma unroll
-  for (int i = 0; i < 8; i++) mx = fmaxf(mx, smax[i][row]);
+  for (int i = 0; i < KC; i++) mx = fmaxf(mx, smax[i][row]);
   const int e = split_exp(mx);
   if (kc == 0 && r < m) rfac[r] = ldexpf(1.0f, -e);
   if (kc * 32 >= kp) return;
@@ -122,13 +125,15 @@ __global__ __launch_bounds__(256) void k_split_rows(const c2 *__restrict__ A, in
 }
 
 // U12 (k x n, column-major, k contiguous) -> planes P[4][np][kp] scaled by
-// 2^e per column, cfac[c] = 2^-e.  Thread (column = t / 8, chunk = t % 8)
-// holds 32 consecutive k of its column; the column maximum by shuffles.
-__global__ __launch_bounds__(256) void k_split_cols(const c2 *__restrict__ B, int64_t ldb, int k, int n, int kp,
+// 2^e per column, cfac[c] = 2^-e.  Thread (column = t / KC, chunk = t % KC)
+// holds 32 consecutive k of its column (k <= 32 KC); the column maximum by
+// shuffles.
+template <int KC>
+__global__ __launch_bounds__(32 * KC) void k_split_cols(const c2 *__restrict__ B, int64_t ldb, int k, int n, int kp,
                                                     int64_t np, _Float16 *P, float *cfac)
 {
-  const int kc = threadIdx.x & 7;
-  const int c = blockIdx.x * 32 + (threadIdx.x >> 3);
+  const int kc = threadIdx.x % KC;
+  const int c = blockIdx.x * 32 + threadIdx.x / KC;
   float re[32], im[32];
   float mx = 0.0f;
 #pragma unroll
@@ -143,6 +148,7 @@ __global__ __launch_bounds__(256) void k_split_cols(const c2 *__restrict__ B, in
   mx = fmaxf(mx, __shfl_xor(mx, 1));
   mx = fmaxf(mx, __shfl_xor(mx, 2));
   mx = fmaxf(mx, __shfl_xor(mx, 4));
+  if (KC > 8) mx = fmaxf(mx, __shfl_xor(mx, 8));
   const int e = split_exp(mx);
   if (kc == 0 && c < n) cfac[c] = ldexpf(1.0f, -e);
   if (kc * 32 >= kp) return;
@@ -802,14 +808,16 @@ struct LocalWork {
   void *mem = nullptr;
   size_t bytes = 0;
   hipStream_t side = nullptr, side2 = nullptr;
-  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[12] = {};
 };
 LocalWork g_lw[64];
 
 struct LocalPtrs {
   SplitWS bulk, pan;
+  SplitWS small;                   // planes of <= 256 x 256 operands (mp = LB)
   c2 *Uinv = nullptr, *Linv = nullptr;
   LinvPlanes lp[2];
+  LinvPlanes up[2];                // U11^-1's column planes (the B operand of L21)
   LocalWork *lw = nullptr;
 };
 
@@ -821,10 +829,13 @@ int local_work(int n, LocalPtrs *o)
   HIPCHECK(hipGetDevice(&dev));
   if (dev < 0 || dev >= 64) return fail(BFA_ERR_ARG, "device index %d unsupported", dev);
   const int64_t p = ((int64_t)n + HT - 1) / HT * HT;
+  // the caller's planes hold up to 2 LB of k (two block steps' update in one
+  // product), the side's LB
   const size_t planes = al256((size_t)4 * p * LB * sizeof(_Float16)), fac = al256((size_t)p * 4);
-  const size_t set = 2 * planes + 2 * fac, tri = al256((size_t)LB * LB * sizeof(c2));
+  const size_t set = 2 * planes + 2 * fac, bset = 4 * planes + 2 * fac, tri = al256((size_t)LB * LB * sizeof(c2));
   const size_t lpl = al256((size_t)4 * LB * LB * sizeof(_Float16)) + al256((size_t)LB * 4);
-  const size_t need = 2 * set + 2 * tri + 2 * lpl;
+  const size_t sset = 2 * lpl;
+  const size_t need = bset + set + 2 * tri + 4 * lpl + sset;
   LocalWork &w = g_lw[dev];
   if (w.bytes < need) {
     if (w.mem) {
@@ -842,22 +853,31 @@ int local_work(int n, LocalPtrs *o)
     for (auto &e : w.ev) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   char *b = (char *)w.mem;
-  for (SplitWS *s : {&o->bulk, &o->pan}) {
-    s->PA = (_Float16 *)b;
-    s->PB = (_Float16 *)(b + planes);
-    s->rf = (float *)(b + 2 * planes);
-    s->cf = (float *)(b + 2 * planes + fac);
-    s->mp = p;
-    b += set;
-  }
+  o->bulk.PA = (_Float16 *)b;
+  o->bulk.PB = (_Float16 *)(b + 2 * planes);
+  o->bulk.rf = (float *)(b + 4 * planes);
+  o->bulk.cf = (float *)(b + 4 * planes + fac);
+  o->bulk.mp = p;
+  b += bset;
+  o->pan.PA = (_Float16 *)b;
+  o->pan.PB = (_Float16 *)(b + planes);
+  o->pan.rf = (float *)(b + 2 * planes);
+  o->pan.cf = (float *)(b + 2 * planes + fac);
+  o->pan.mp = p;
+  b += set;
   o->Uinv = (c2 *)b;
   o->Linv = (c2 *)(b + tri);
   b += 2 * tri;
-  for (auto &l : o->lp) {
-    l.P = (_Float16 *)b;
-    l.rf = (float *)(b + al256((size_t)4 * LB * LB * sizeof(_Float16)));
+  for (LinvPlanes *l : {&o->lp[0], &o->lp[1], &o->up[0], &o->up[1]}) {
+    l->P = (_Float16 *)b;
+    l->rf = (float *)(b + al256((size_t)4 * LB * LB * sizeof(_Float16)));
     b += lpl;
   }
+  o->small.PA = (_Float16 *)b;
+  o->small.rf = (float *)(b + al256((size_t)4 * LB * LB * sizeof(_Float16)));
+  o->small.PB = (_Float16 *)(b + lpl);
+  o->small.cf = (float *)(b + lpl + al256((size_t)4 * LB * LB * sizeof(_Float16)));
+  o->small.mp = LB;
   o->lw = &w;
   return BFA_OK;
 }
@@ -901,15 +921,20 @@ void super_grid(int tm, int tn, int *G, size_t *nblk)
   *nblk = (ns + 7) / 8 * 8 * g * g;
 }
 
-// split planes of A (m x k) by rows and B (k x n) by columns, k <= 256
+// split planes of A (m x k) by rows and B (k x n) by columns, k <= 512 (the
+// caller's planes; the side's hold k <= 256)
 void split_ops(int m, int n, int k, const c2 *A, int64_t lda, const c2 *B, int64_t ldb, const SplitWS &w,
                hipStream_t st)
 {
   const int kp = (k + HK - 1) / HK * HK;
-  if (m > 0) hipLaunchKernelGGL(k_split_rows, dim3((unsigned)(w.mp / 32)), dim3(256), 0, st, A, lda, m, k, kp, w.mp,
-                                w.PA, w.rf);
-  if (n > 0) hipLaunchKernelGGL(k_split_cols, dim3((unsigned)(w.mp / 32)), dim3(256), 0, st, B, ldb, k, n, kp, w.mp,
-                                w.PB, w.cf);
+  const dim3 grid((unsigned)(w.mp / 32));
+  if (k <= LB) {
+    if (m > 0) hipLaunchKernelGGL(k_split_rows<8>, grid, dim3(256), 0, st, A, lda, m, k, kp, w.mp, w.PA, w.rf);
+    if (n > 0) hipLaunchKernelGGL(k_split_cols<8>, grid, dim3(256), 0, st, B, ldb, k, n, kp, w.mp, w.PB, w.cf);
+  } else {
+    if (m > 0) hipLaunchKernelGGL(k_split_rows<16>, grid, dim3(512), 0, st, A, lda, m, k, kp, w.mp, w.PA, w.rf);
+    if (n > 0) hipLaunchKernelGGL(k_split_cols<16>, grid, dim3(512), 0, st, B, ldb, k, n, kp, w.mp, w.PB, w.cf);
+  }
 }
 
 // C(m x n) -= A B (store: C = A B) from split planes: A's planes PA (rows
@@ -963,18 +988,18 @@ int block_column(c2 *A, int n, int64_t lda, int j, int jb, int *ipiv, int *flags
   const int kp = (jb + HK - 1) / HK * HK;
   // st2: A21's planes (they need only the diagonal block's row swaps, which
   // do not reach below it), then L11^-1 and its planes
-  hipLaunchKernelGGL(k_split_rows, dim3((unsigned)(P.pan.mp / 32)), dim3(256), 0, st2, (const c2 *)(Ajj + jb), lda,
+  hipLaunchKernelGGL(k_split_rows<8>, dim3((unsigned)(P.pan.mp / 32)), dim3(256), 0, st2, (const c2 *)(Ajj + jb), lda,
                      below, jb, kp, P.pan.mp, P.pan.PA, P.pan.rf);
   HIPCHECK(hipEventRecord(ev_diag, st2));                 // A21's planes done
   hipLaunchKernelGGL(k_eye, dim3((jb + 255) / 256, jb), dim3(256), 0, st2, P.Linv, jb);
   bfa::trsm_lower_c(jb, jb, Ajj, lda, P.Linv, jb, st2);
-  hipLaunchKernelGGL(k_split_rows, dim3(LB / 32), dim3(256), 0, st2, (const c2 *)P.Linv, (int64_t)jb, jb, jb,
+  hipLaunchKernelGGL(k_split_rows<8>, dim3(LB / 32), dim3(256), 0, st2, (const c2 *)P.Linv, (int64_t)jb, jb, jb,
                      kp, (int64_t)LB, lp.P, lp.rf);
   HIPCHECK(hipEventRecord(ev_linv, st2));
   // st: U11^-1, its planes, L21 = A21 U11^-1 over A21 once st2's planes of A21 exist
   hipLaunchKernelGGL(k_eye, dim3((jb + 255) / 256, jb), dim3(256), 0, st, P.Uinv, jb);
   bfa::trsm_upper_c(jb, jb, Ajj, lda, P.Uinv, jb, st);
-  hipLaunchKernelGGL(k_split_cols, dim3((unsigned)(P.pan.mp / 32)), dim3(256), 0, st, (const c2 *)P.Uinv,
+  hipLaunchKernelGGL(k_split_cols<8>, dim3((unsigned)(P.pan.mp / 32)), dim3(256), 0, st, (const c2 *)P.Uinv,
                      (int64_t)jb, jb, jb, kp, P.pan.mp, P.pan.PB, P.pan.cf);
   HIPCHECK(hipStreamWaitEvent(st, ev_diag, 0));
   gemm_planes(below, jb, jb, Ajj + jb, lda, P.pan, 0, st, 1);
@@ -1034,8 +1059,8 @@ int bfa::profile_split_bytes(double *bytes, int *launches)
 extern "C" int bfa_cgemm_sub_split(int m, int n, int k, const void *d_A, int64_t lda, const void *d_B, int64_t ldb,
                                    void *d_C, int64_t ldc, void *stream)
 {
-  if (m < 0 || n < 0 || k < 0 || k > LB || (m > 0 && (lda < m || ldc < m)) || (k > 0 && ldb < k))
-    return fail(BFA_ERR_ARG, "bfa_cgemm_sub_split: bad sizes (k <= %d)", LB);
+  if (m < 0 || n < 0 || k < 0 || k > 2 * LB || (m > 0 && (lda < m || ldc < m)) || (k > 0 && ldb < k))
+    return fail(BFA_ERR_ARG, "bfa_cgemm_sub_split: bad sizes (k <= %d)", 2 * LB);
   if (m == 0 || n == 0 || k == 0) return BFA_OK;
   LocalPtrs P;
   int rc = local_work(m > n ? m : n, &P);
@@ -1047,25 +1072,19 @@ extern "C" int bfa_cgemm_sub_split(int m, int n, int k, const void *d_A, int64_t
   return BFA_OK;
 }
 
-// Look-ahead: the caller's stream st carries, per step, the row swaps,
-// U12 = L11^-1 A12 (A12 split, the product stored over A12), the splits and
-// the trailing update -- the next block column first; the side streams then
-// factor that block column (diagonal block, L21, L11^-1's planes) while st
-// updates the remaining columns.  They touch disjoint columns (the side
-// streams only the next block column, st everything right of it) and each
-// has its own split planes; L11^-1's planes alternate between two sets.
-extern "C" int bfa_lu_factor_c_local(void *d_A, int n, int64_t lda, int *d_ipiv, int *info, void *stream)
+namespace {
+
+// Schedule 1 (look-ahead): the caller's stream st carries, per step, the row
+// swaps, U12 = L11^-1 A12 (A12 split, the product stored over A12), the
+// splits and the trailing update -- the next block column first; the side
+// streams then factor that block column (diagonal block, L21, L11^-1's
+// planes) while st updates the remaining columns.  They touch disjoint
+// columns (the side streams only the next block column, st everything right
+// of it) and each has its own split planes; L11^-1's planes alternate
+// between two sets.
+int sched_lookahead(c2 *A, int n, int64_t lda, int *d_ipiv, int *flags, const LocalPtrs &P, hipStream_t st)
 {
-  if (!d_A || !d_ipiv || n <= 0 || lda < n) return fail(BFA_ERR_ARG, "bfa_lu_factor_c_local: bad argument");
-  hipStream_t st = (hipStream_t)stream;
-  c2 *A = (c2 *)d_A;
-  LocalPtrs P;
-  int rc = local_work(n, &P);
-  if (rc) return rc;
-  rc = bfa::lu_local_begin(st);
-  if (rc) return rc;
-  int *flags = bfa::lu_local_flags();
-  if (!flags) return fail(BFA_ERR_STATE, "bfa_lu_factor_c_local: LU work area");
+  int rc = BFA_OK;
   LocalWork *lw = P.lw;
   hipStream_t sp = lw->side, sp2 = lw->side2;
   hipEvent_t ev_in = lw->ev[0], ev_col = lw->ev[1], ev_next = lw->ev[2], ev_diag = lw->ev[3], ev_linv = lw->ev[4];
@@ -1088,7 +1107,7 @@ extern "C" int bfa_lu_factor_c_local(void *d_A, int n, int64_t lda, int *d_ipiv,
     c2 *A12 = A + (size_t)(j + jb) * lda + j;
     const LinvPlanes &lp = P.lp[(j / LB) & 1];
     const int kp = (jb + HK - 1) / HK * HK;
-    hipLaunchKernelGGL(k_split_cols, dim3((unsigned)(P.bulk.mp / 32)), dim3(256), 0, st, (const c2 *)A12, lda, jb,
+    hipLaunchKernelGGL(k_split_cols<8>, dim3((unsigned)(P.bulk.mp / 32)), dim3(256), 0, st, (const c2 *)A12, lda, jb,
                        rest, kp, P.bulk.mp, P.bulk.PB, P.bulk.cf);
     gemm_ops(jb, rest, jb, A12, lda, lp.P, lp.rf, LB, P.bulk.PB, P.bulk.cf, P.bulk.mp, 0, st, 1);   // U12
     const c2 *L21 = A + (size_t)j * lda + j + jb;
@@ -1107,6 +1126,197 @@ extern "C" int bfa_lu_factor_c_local(void *d_A, int n, int64_t lda, int *d_ipiv,
   }
   HIPCHECK(hipEventRecord(ev_col, sp));
   HIPCHECK(hipStreamWaitEvent(st, ev_col, 0));
+  return BFA_OK;
+}
+
+// Schedule 2 (diagonal chain first).  The critical path of the look-ahead is
+// the chain from one diagonal block's factorisation to the next's; here it
+// carries only 256-row pieces.  With blocks j, j+1, j+2 (rows/columns
+// starting at j, j1, j2), per step:
+//   side2:  L11^-1 of block j and its row planes;
+//           L21 below block j+1: A(j2.., j) = A(j2.., j) U11^-1.
+//   side:   U11^-1 and its column planes; block j's interchanges in block
+//           column j+1 and U12 there, A(j, j1) = L11^-1 A(j, j1); L21 of
+//           block j+1's rows, A(j1, j) = A(j1, j) U11^-1; the diagonal block
+//           A(j1, j1) -= A(j1, j) A(j, j1); its factorisation.
+//   caller: block j's interchanges right of block j+1 and U12 there; the
+//           splits of L21 (all rows below block j) and U12 (all columns
+//           right of it); block column j+1 below its diagonal block; block
+//           column j+2; the left interchanges; the rest of the update.
+// The side stream waits for the caller's updates of block columns j+1 and
+// j+2 (events gc, nn) and the caller for the side's L21 and U12 pieces
+// (u1, l1, lr).  The pieces the streams work on at the same time are
+// disjoint: the side streams work in block columns j+1 (rows below block
+// j+1 on side2) and j+2 or, for the interchanges, rows of block j+1; the
+// caller in rows of block j and columns right of block j+2.  The planes of
+// L11^-1 and U11^-1 alternate between two sets by block parity; each set's
+// next writer is ordered after its last reader through these events.
+int sched_chain(c2 *A, int n, int64_t lda, int *ipiv, int *flags, const LocalPtrs &P, hipStream_t st)
+{
+  LocalWork *lw = P.lw;
+  hipStream_t sp = lw->side, sp2 = lw->side2;
+  hipEvent_t ev_in = lw->ev[0], ev_diag = lw->ev[1], ev_linv = lw->ev[2], ev_uinv = lw->ev[3], ev_u1 = lw->ev[4],
+             ev_lr = lw->ev[5], ev_l1 = lw->ev[6], ev_gc = lw->ev[7], ev_nn = lw->ev[8], ev_end = lw->ev[9];
+  HIPCHECK(hipEventRecord(ev_in, st));
+  HIPCHECK(hipStreamWaitEvent(sp, ev_in, 0));
+  HIPCHECK(hipStreamWaitEvent(sp2, ev_in, 0));
+  hipLaunchKernelGGL(k_diag_lu, dim3(1), dim3(DL_NT), 0, sp, A, lda, n < LB ? n : LB, ipiv, 0, flags);
+  HIPCHECK(hipEventRecord(ev_diag, sp));
+  const dim3 sgrid(LB / 32), bgrid((unsigned)(P.bulk.mp / 32)), pgrid((unsigned)(P.pan.mp / 32));
+  static int fuse = -1;
+  if (fuse < 0) {
+    const char *e = getenv("BFA_LU_LOCAL_FUSE");
+    fuse = !(e && strcmp(e, "0") == 0);
+  }
+  bool deferred = false;
+  for (int j = 0; j < n; j += LB) {
+    const int jb = n - j < LB ? n - j : LB;
+    const int j1 = j + jb, jb1 = n - j1 < LB ? n - j1 : LB;
+    if (jb1 <= 0) {                                      // the last block
+      HIPCHECK(hipStreamWaitEvent(st, ev_diag, 0));
+      bfa::laswp_c(A, lda, 0, j, j, jb, ipiv, st);
+      break;
+    }
+    const int j2 = j1 + jb1, jb2 = n - j2 <= 0 ? 0 : n - j2 < LB ? n - j2 : LB;
+    const int kp = (jb + HK - 1) / HK * HK;
+    const LinvPlanes &lp = P.lp[(j / LB) & 1], &up = P.up[(j / LB) & 1];
+    c2 *Ajj = A + (size_t)j * lda + j;
+    c2 *Aj_1 = A + (size_t)j1 * lda + j;                 // block (j, j+1)
+    c2 *Aj1_j = A + (size_t)j * lda + j1;                // block (j+1, j)
+    c2 *Aj1_1 = A + (size_t)j1 * lda + j1;               // block (j+1, j+1)
+    // side2: L11^-1 and its planes
+    HIPCHECK(hipStreamWaitEvent(sp2, ev_diag, 0));
+    hipLaunchKernelGGL(k_eye, dim3((jb + 255) / 256, jb), dim3(256), 0, sp2, P.Linv, jb);
+    bfa::trsm_lower_c(jb, jb, Ajj, lda, P.Linv, jb, sp2);
+    hipLaunchKernelGGL(k_split_rows<8>, sgrid, dim3(256), 0, sp2, (const c2 *)P.Linv, (int64_t)jb, jb, jb, kp,
+                       (int64_t)LB, lp.P, lp.rf);
+    HIPCHECK(hipEventRecord(ev_linv, sp2));
+    // side: U11^-1 and its planes
+    hipLaunchKernelGGL(k_eye, dim3((jb + 255) / 256, jb), dim3(256), 0, sp, P.Uinv, jb);
+    bfa::trsm_upper_c(jb, jb, Ajj, lda, P.Uinv, jb, sp);
+    hipLaunchKernelGGL(k_split_cols<8>, sgrid, dim3(256), 0, sp, (const c2 *)P.Uinv, (int64_t)jb, jb, jb, kp,
+                       (int64_t)LB, up.P, up.rf);
+    HIPCHECK(hipEventRecord(ev_uinv, sp));
+    // side: U12 of block column j+1 (after the caller's update of that
+    // column by step j-1)
+    HIPCHECK(hipStreamWaitEvent(sp, ev_linv, 0));
+    if (j > 0) HIPCHECK(hipStreamWaitEvent(sp, ev_nn, 0));
+    bfa::laswp_c(A, lda, j1, j2, j, jb, ipiv, sp);
+    hipLaunchKernelGGL(k_split_cols<8>, sgrid, dim3(256), 0, sp, (const c2 *)Aj_1, lda, jb, jb1, kp, (int64_t)LB,
+                       P.small.PB, P.small.cf);
+    gemm_ops(jb, jb1, jb, Aj_1, lda, lp.P, lp.rf, LB, P.small.PB, P.small.cf, LB, 0, sp, 1);
+    HIPCHECK(hipEventRecord(ev_u1, sp));
+    // side2: L21 below block j+1 (after the caller's update of block column
+    // j below block j by step j-1)
+    const int below2 = n - j2;
+    if (j > 0) HIPCHECK(hipStreamWaitEvent(sp2, ev_gc, 0));
+    if (below2 > 0) {
+      c2 *Aj2_j = A + (size_t)j * lda + j2;
+      hipLaunchKernelGGL(k_split_rows<8>, pgrid, dim3(256), 0, sp2, (const c2 *)Aj2_j, lda, below2, jb, kp, P.pan.mp,
+                         P.pan.PA, P.pan.rf);
+      HIPCHECK(hipStreamWaitEvent(sp2, ev_uinv, 0));
+      gemm_ops(below2, jb, jb, Aj2_j, lda, P.pan.PA, P.pan.rf, P.pan.mp, up.P, up.rf, LB, 0, sp2, 1);
+    }
+    HIPCHECK(hipEventRecord(ev_lr, sp2));
+    // side: L21 of block j+1's rows, block j+1's diagonal block, its factorisation
+    if (j > 0) HIPCHECK(hipStreamWaitEvent(sp, ev_gc, 0));
+    hipLaunchKernelGGL(k_split_rows<8>, sgrid, dim3(256), 0, sp, (const c2 *)Aj1_j, lda, jb1, jb, kp, (int64_t)LB,
+                       P.small.PA, P.small.rf);
+    gemm_ops(jb1, jb, jb, Aj1_j, lda, P.small.PA, P.small.rf, LB, up.P, up.rf, LB, 0, sp, 1);
+    HIPCHECK(hipEventRecord(ev_l1, sp));
+    hipLaunchKernelGGL(k_split_rows<8>, sgrid, dim3(256), 0, sp, (const c2 *)Aj1_j, lda, jb1, jb, kp, (int64_t)LB,
+                       P.small.PA, P.small.rf);
+    hipLaunchKernelGGL(k_split_cols<8>, sgrid, dim3(256), 0, sp, (const c2 *)Aj_1, lda, jb, jb1, kp, (int64_t)LB,
+                       P.small.PB, P.small.cf);
+    gemm_ops(jb1, jb1, jb, Aj1_1, lda, P.small.PA, P.small.rf, LB, P.small.PB, P.small.cf, LB, 0, sp, 0);
+    hipLaunchKernelGGL(k_diag_lu, dim3(1), dim3(DL_NT), 0, sp, Aj1_1, lda, jb1, ipiv + j1, j1, flags);
+    HIPCHECK(hipEventRecord(ev_diag, sp));
+    // caller: U12 right of block j+1
+    HIPCHECK(hipStreamWaitEvent(st, ev_linv, 0));
+    const int rest1 = n - j2;
+    if (rest1 > 0) {
+      c2 *Aj_2 = A + (size_t)j2 * lda + j;
+      bfa::laswp_c(A, lda, j2, n, j, jb, ipiv, st);
+      hipLaunchKernelGGL(k_split_cols<8>, bgrid, dim3(256), 0, st, (const c2 *)Aj_2, lda, jb, rest1, kp, P.bulk.mp,
+                         P.bulk.PB, P.bulk.cf);
+      gemm_ops(jb, rest1, jb, Aj_2, lda, lp.P, lp.rf, LB, P.bulk.PB, P.bulk.cf, P.bulk.mp, 0, st, 1);
+    }
+    // caller: the splits of L21 and U12, then the updates
+    HIPCHECK(hipStreamWaitEvent(st, ev_u1, 0));
+    HIPCHECK(hipStreamWaitEvent(st, ev_l1, 0));
+    HIPCHECK(hipStreamWaitEvent(st, ev_lr, 0));
+    const int below = n - j1;
+    split_ops(below, below, jb, Aj1_j, lda, Aj_1, lda, P.bulk, st);
+    if (below2 > 0)                                      // block column j+1 below its diagonal block
+      gemm_ops(below2, jb1, jb, A + (size_t)j1 * lda + j2, lda, P.bulk.PA + (size_t)jb1 * kp, P.bulk.rf + jb1,
+               P.bulk.mp, P.bulk.PB, P.bulk.cf, P.bulk.mp, 0, st, 0);
+    HIPCHECK(hipEventRecord(ev_gc, st));
+    // Pairs of steps share one trailing update of k = 2 LB (half the C
+    // traffic per flop): the first step of a pair updates block columns
+    // j+2 and j+3 and the rows of block j+1 only, and leaves rows below
+    // block j+1 right of block j+3 to the second step, which updates them
+    // with both steps' L21 and U12 in one product.  Nothing reads that
+    // region in between: the side streams work in block columns j+1..j+3
+    // and the caller's next step in rows of block j+1 and block column j+3.
+    const int j3 = j2 + jb2, jb3 = n - j3 <= 0 ? 0 : n - j3 < LB ? n - j3 : LB;
+    // (while the rest is wide enough that the caller's stream, not the side
+    // chain, sets the pace: measured at N = 19,872, pairs with fewer than
+    // 6,144 columns right of them were slower fused)
+    const bool first = fuse && !deferred && jb == LB && jb1 == LB && jb2 == LB && jb3 > 0 && n - j3 - jb3 >= 24 * LB;
+    const int nn = first ? jb2 + jb3 : jb2;
+    if (nn > 0)                                          // block column j+2 (and j+3)
+      gemm_ops(below, nn, jb, A + (size_t)j2 * lda + j1, lda, P.bulk.PA, P.bulk.rf, P.bulk.mp, P.bulk.PB,
+               P.bulk.cf, P.bulk.mp, jb1, st, 0);
+    HIPCHECK(hipEventRecord(ev_nn, st));
+    if (j > 0) bfa::laswp_c(A, lda, 0, j, j, jb, ipiv, st);
+    if (first) {                                         // rows of block j+1 right of block j+3
+      const int e = j3 + jb3;
+      gemm_ops(jb1, n - e, jb, A + (size_t)e * lda + j1, lda, P.bulk.PA, P.bulk.rf, P.bulk.mp, P.bulk.PB,
+               P.bulk.cf, P.bulk.mp, jb1 + nn, st, 0);
+      deferred = true;
+      continue;
+    }
+    const int rest2 = n - j2 - jb2;
+    if (deferred) {                                      // both steps' update below block j+1, right of j+2
+      const int a = j - LB;
+      split_ops(below, rest2, 2 * LB, A + (size_t)a * lda + j1, lda, A + (size_t)(j2 + jb2) * lda + a, lda, P.bulk,
+                st);
+      gemm_planes(below, rest2, 2 * LB, A + (size_t)(j2 + jb2) * lda + j1, lda, P.bulk, 0, st, 0);
+      deferred = false;
+    } else if (rest2 > 0)
+      gemm_ops(below, rest2, jb, A + (size_t)(j2 + jb2) * lda + j1, lda, P.bulk.PA, P.bulk.rf, P.bulk.mp,
+               P.bulk.PB, P.bulk.cf, P.bulk.mp, jb1 + jb2, st, 0);
+  }
+  HIPCHECK(hipEventRecord(ev_end, sp));
+  HIPCHECK(hipStreamWaitEvent(st, ev_end, 0));
+  HIPCHECK(hipEventRecord(ev_end, sp2));
+  HIPCHECK(hipStreamWaitEvent(st, ev_end, 0));
+  return BFA_OK;
+}
+
+}  // namespace
+
+// BFA_LU_LOCAL_SCHED=lookahead selects schedule 1; the default is the
+// diagonal chain (schedule 2)
+extern "C" int bfa_lu_factor_c_local(void *d_A, int n, int64_t lda, int *d_ipiv, int *info, void *stream)
+{
+  if (!d_A || !d_ipiv || n <= 0 || lda < n) return fail(BFA_ERR_ARG, "bfa_lu_factor_c_local: bad argument");
+  hipStream_t st = (hipStream_t)stream;
+  c2 *A = (c2 *)d_A;
+  LocalPtrs P;
+  int rc = local_work(n, &P);
+  if (rc) return rc;
+  rc = bfa::lu_local_begin(st);
+  if (rc) return rc;
+  int *flags = bfa::lu_local_flags();
+  if (!flags) return fail(BFA_ERR_STATE, "bfa_lu_factor_c_local: LU work area");
+  static int sched = -1;
+  if (sched < 0) {
+    const char *e = getenv("BFA_LU_LOCAL_SCHED");
+    sched = e && strcmp(e, "lookahead") == 0 ? 1 : 2;
+  }
+  rc = sched == 1 ? sched_lookahead(A, n, lda, d_ipiv, flags, P, st) : sched_chain(A, n, lda, d_ipiv, flags, P, st);
+  if (rc) return rc;
   HIPCHECK(hipGetLastError());
   return bfa::lu_local_end(info, st);
 }

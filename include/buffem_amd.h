@@ -233,7 +233,7 @@ int  bfa_lu_solve_c(const void *d_A, int n, int64_t lda, const int *d_ipiv,
  * pivoting across blocks (the VIE matrices do), with FP64 refinement
  * restoring the accuracy.  d_ipiv and info as bfa_lu_factor_c.             */
 int  bfa_lu_factor_c_local(void *d_A, int n, int64_t lda, int *d_ipiv, int *info, void *stream);
-/* C(m x n) -= A(m x k) * B(k x n), complex64, k <= 256, by the split-FP16
+/* C(m x n) -= A(m x k) * B(k x n), complex64, k <= 512, by the split-FP16
  * products of bfa_lu_factor_c_local's trailing update.                     */
 int  bfa_cgemm_sub_split(int m, int n, int k, const void *d_A, int64_t lda, const void *d_B,
                          int64_t ldb, void *d_C, int64_t ldc, void *stream);

@@ -4,11 +4,12 @@ cores, as the preconditioner of FP64 iterative refinement.
 
 - bfa_cgemm_sub_split (C -= A B from hi + lo FP16 halves of row- and
   column-scaled operands) against a complex128 product on the host:
-  normwise error at the split's 22-bit level, on ragged shapes, k = 1..256,
+  normwise error at the split's 22-bit level, on ragged shapes, k = 1..512,
   and rows/columns spanning 1e-8..1e8 in magnitude (the per-row / per-column
   power-of-two scaling keeps every row's relative accuracy).
-- bfa_lu_factor_c_local on a random matrix (n = 3,000, not a multiple of
-  256): the solve's backward error at complex64 level; on a matrix that
+- bfa_lu_factor_c_local on random matrices (n = 3,000 and 8,100, not
+  multiples of 256; the larger one through the paired k = 512 trailing
+  updates): the solve's backward error at complex64 level; on a matrix that
   pivots in most columns (n = 600): the first block's pivots equal the
   float64 restatement's (tests/blocklu.py), every pivot dominates its
   column's candidates, and P A = L U to the split products' accuracy.
@@ -33,7 +34,7 @@ def test_cgemm_split(bfa, torch_cuda):
     torch = torch_cuda
     rng = np.random.default_rng(5)
     shapes = [(16, 16, 32), (70, 33, 21), (200, 130, 256), (300, 260, 100), (129, 255, 1), (1000, 700, 256),
-              (2048, 384, 256)]
+              (2048, 384, 256), (500, 300, 300), (1100, 640, 512)]
     for (m, n, k) in shapes:
         A = (rng.standard_normal((m, k)) + 1j * rng.standard_normal((m, k))).astype(np.complex64)
         B = (rng.standard_normal((k, n)) + 1j * rng.standard_normal((k, n))).astype(np.complex64)
@@ -60,10 +61,13 @@ def test_cgemm_split(bfa, torch_cuda):
             assert errs[0] < max(4 * errs[1], 2e-6)
 
 
-def test_lu_local_random(bfa, torch_cuda):
+@pytest.mark.parametrize("n", [3000, 8100])
+def test_lu_local_random(bfa, torch_cuda, n):
+    """n = 8,100 takes the paired trailing updates (k = 512) in its first
+    steps, n = 3,000 only single ones."""
     torch = torch_cuda
-    n, nrhs = 3000, 3
-    gen = torch.Generator(device="cuda:0").manual_seed(3000)
+    nrhs = 3
+    gen = torch.Generator(device="cuda:0").manual_seed(n)
     A = torch.randn((n, n), dtype=torch.complex128, device="cuda:0", generator=gen)
     A += n ** 0.5 * torch.eye(n, dtype=torch.complex128, device="cuda:0")
     B = torch.randn((nrhs, n), dtype=torch.complex128, device="cuda:0", generator=gen)
