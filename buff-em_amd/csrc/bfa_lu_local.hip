@@ -784,6 +784,61 @@ __global__ void k_eye(c2 *X, int n)
   if (r < n) X[(size_t)c * n + r] = {r == c ? 1.0f : 0.0f, 0.0f};
 }
 
+// Row interchanges of one block (rows k0..k0+kb, kb <= LB, every pivot
+// inside the block, as the diagonal-block kernel chooses them) in columns
+// c0..c1.  LAPACK's sequence of interchanges is composed into one
+// permutation of the block's rows -- thread 0 walks only the rows that
+// were interchanged, found by ballots -- and each workgroup then moves the
+// rows that changed place in its 32 columns: all loads, a barrier, all
+// stores.  The columns of one workgroup are its own, so nothing crosses
+// workgroups.
+#define LSW_COLS 32
+__global__ __launch_bounds__(256) void k_laswp_local(c2 *A, int64_t lda, int c0, int c1, int k0, int kb,
+                                                     const int *__restrict__ ipiv)
+{
+  __shared__ int perm[LB], pv[LB];
+  __shared__ unsigned long long mask[LB / 64];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  int p = t < kb ? ipiv[k0 + t] - k0 : t;
+  if (p < 0 || p >= kb) p = t;                          // never for the block-local pivots
+  perm[t] = t;
+  pv[t] = p;
+  const unsigned long long b = __ballot(p != t);
+  if (lane == 0) mask[w] = b;
+  __syncthreads();
+  if (t == 0) {
+    for (int q = 0; q < LB / 64; q++) {
+      unsigned long long m = mask[q];
+      while (m) {
+        const int i = q * 64 + __builtin_ctzll(m);
+        m &= m - 1;
+        const int pp = pv[i], x = perm[i];
+        perm[i] = perm[pp];
+        perm[pp] = x;
+      }
+    }
+  }
+  __syncthreads();
+  const int src = perm[t];
+  const bool mov = t < kb && src != t;
+  const int cb = c0 + blockIdx.x * LSW_COLS;
+  c2 v[LSW_COLS];
+#pragma unroll
+  for (int i = 0; i < LSW_COLS; i++)
+    v[i] = (mov && cb + i < c1) ? A[(size_t)(cb + i) * lda + k0 + src] : c2{0.0f, 0.0f};
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < LSW_COLS; i++)
+    if (mov && cb + i < c1) A[(size_t)(cb + i) * lda + k0 + t] = v[i];
+}
+
+void laswp_local(c2 *A, int64_t lda, int c0, int c1, int k0, int kb, const int *ipiv, hipStream_t st)
+{
+  if (c1 <= c0 || kb <= 0) return;
+  hipLaunchKernelGGL(k_laswp_local, dim3((unsigned)((c1 - c0 + LSW_COLS - 1) / LSW_COLS)), dim3(256), 0, st, A, lda,
+                     c0, c1, k0, kb, ipiv);
+}
+
 // ---------------------------------------------------------------------------
 // Split planes and scale factors of one product's operands
 struct SplitWS {
@@ -1100,10 +1155,10 @@ int sched_lookahead(c2 *A, int n, int64_t lda, int *d_ipiv, int *flags, const Lo
     HIPCHECK(hipStreamWaitEvent(st, ev_col, 0));         // block column j factored
     const int rest = n - j - jb;
     if (rest <= 0) {
-      bfa::laswp_c(A, lda, 0, j, j, jb, d_ipiv, st);
+      laswp_local(A, lda, 0, j, j, jb, d_ipiv, st);
       break;
     }
-    bfa::laswp_c(A, lda, j + jb, n, j, jb, d_ipiv, st);
+    laswp_local(A, lda, j + jb, n, j, jb, d_ipiv, st);
     c2 *A12 = A + (size_t)(j + jb) * lda + j;
     const LinvPlanes &lp = P.lp[(j / LB) & 1];
     const int kp = (jb + HK - 1) / HK * HK;
@@ -1121,7 +1176,7 @@ int sched_lookahead(c2 *A, int n, int64_t lda, int *d_ipiv, int *flags, const Lo
     if (rc) return rc;
     HIPCHECK(hipEventRecord(ev_col, sp));
     // the left columns' interchanges are off the look-ahead chain
-    bfa::laswp_c(A, lda, 0, j, j, jb, d_ipiv, st);
+    laswp_local(A, lda, 0, j, j, jb, d_ipiv, st);
     gemm_planes(rest, rest - nb2, jb, A12 + (size_t)nb2 * lda + jb, lda, P.bulk, nb2, st, 0);
   }
   HIPCHECK(hipEventRecord(ev_col, sp));
@@ -1133,12 +1188,12 @@ int sched_lookahead(c2 *A, int n, int64_t lda, int *d_ipiv, int *flags, const Lo
 // the chain from one diagonal block's factorisation to the next's; here it
 // carries only 256-row pieces.  With blocks j, j+1, j+2 (rows/columns
 // starting at j, j1, j2), per step:
-//   side2:  L11^-1 of block j and its row planes;
+//   side2:  L11^-1 of block j and its row planes; block j's interchanges
+//           in block column j+1 and U12 there, A(j, j1) = L11^-1 A(j, j1);
 //           L21 below block j+1: A(j2.., j) = A(j2.., j) U11^-1.
-//   side:   U11^-1 and its column planes; block j's interchanges in block
-//           column j+1 and U12 there, A(j, j1) = L11^-1 A(j, j1); L21 of
-//           block j+1's rows, A(j1, j) = A(j1, j) U11^-1; the diagonal block
-//           A(j1, j1) -= A(j1, j) A(j, j1); its factorisation.
+//   side:   U11^-1 and its column planes; L21 of block j+1's rows,
+//           A(j1, j) = A(j1, j) U11^-1 (beside side2's U12); the diagonal
+//           block A(j1, j1) -= A(j1, j) A(j, j1); its factorisation.
 //   caller: block j's interchanges right of block j+1 and U12 there; the
 //           splits of L21 (all rows below block j) and U12 (all columns
 //           right of it); block column j+1 below its diagonal block; block
@@ -1174,7 +1229,7 @@ int sched_chain(c2 *A, int n, int64_t lda, int *ipiv, int *flags, const LocalPtr
     const int j1 = j + jb, jb1 = n - j1 < LB ? n - j1 : LB;
     if (jb1 <= 0) {                                      // the last block
       HIPCHECK(hipStreamWaitEvent(st, ev_diag, 0));
-      bfa::laswp_c(A, lda, 0, j, j, jb, ipiv, st);
+      laswp_local(A, lda, 0, j, j, jb, ipiv, st);
       break;
     }
     const int j2 = j1 + jb1, jb2 = n - j2 <= 0 ? 0 : n - j2 < LB ? n - j2 : LB;
@@ -1184,41 +1239,41 @@ int sched_chain(c2 *A, int n, int64_t lda, int *ipiv, int *flags, const LocalPtr
     c2 *Aj_1 = A + (size_t)j1 * lda + j;                 // block (j, j+1)
     c2 *Aj1_j = A + (size_t)j * lda + j1;                // block (j+1, j)
     c2 *Aj1_1 = A + (size_t)j1 * lda + j1;               // block (j+1, j+1)
-    // side2: L11^-1 and its planes
+    // side2: L11^-1 and its planes, then U12 of block column j+1 (after the
+    // caller's update of that column by step j-1): block j's interchanges
+    // there and A(j, j1) = L11^-1 A(j, j1)
     HIPCHECK(hipStreamWaitEvent(sp2, ev_diag, 0));
     hipLaunchKernelGGL(k_eye, dim3((jb + 255) / 256, jb), dim3(256), 0, sp2, P.Linv, jb);
     bfa::trsm_lower_c(jb, jb, Ajj, lda, P.Linv, jb, sp2);
     hipLaunchKernelGGL(k_split_rows<8>, sgrid, dim3(256), 0, sp2, (const c2 *)P.Linv, (int64_t)jb, jb, jb, kp,
                        (int64_t)LB, lp.P, lp.rf);
     HIPCHECK(hipEventRecord(ev_linv, sp2));
+    if (j > 0) HIPCHECK(hipStreamWaitEvent(sp2, ev_nn, 0));
+    laswp_local(A, lda, j1, j2, j, jb, ipiv, sp2);
+    hipLaunchKernelGGL(k_split_cols<8>, sgrid, dim3(256), 0, sp2, (const c2 *)Aj_1, lda, jb, jb1, kp, (int64_t)LB,
+                       P.small.PB, P.small.cf);
+    gemm_ops(jb, jb1, jb, Aj_1, lda, lp.P, lp.rf, LB, P.small.PB, P.small.cf, LB, 0, sp2, 1);
+    HIPCHECK(hipEventRecord(ev_u1, sp2));
     // side: U11^-1 and its planes
     hipLaunchKernelGGL(k_eye, dim3((jb + 255) / 256, jb), dim3(256), 0, sp, P.Uinv, jb);
     bfa::trsm_upper_c(jb, jb, Ajj, lda, P.Uinv, jb, sp);
     hipLaunchKernelGGL(k_split_cols<8>, sgrid, dim3(256), 0, sp, (const c2 *)P.Uinv, (int64_t)jb, jb, jb, kp,
                        (int64_t)LB, up.P, up.rf);
     HIPCHECK(hipEventRecord(ev_uinv, sp));
-    // side: U12 of block column j+1 (after the caller's update of that
-    // column by step j-1)
-    HIPCHECK(hipStreamWaitEvent(sp, ev_linv, 0));
-    if (j > 0) HIPCHECK(hipStreamWaitEvent(sp, ev_nn, 0));
-    bfa::laswp_c(A, lda, j1, j2, j, jb, ipiv, sp);
-    hipLaunchKernelGGL(k_split_cols<8>, sgrid, dim3(256), 0, sp, (const c2 *)Aj_1, lda, jb, jb1, kp, (int64_t)LB,
-                       P.small.PB, P.small.cf);
-    gemm_ops(jb, jb1, jb, Aj_1, lda, lp.P, lp.rf, LB, P.small.PB, P.small.cf, LB, 0, sp, 1);
-    HIPCHECK(hipEventRecord(ev_u1, sp));
     // side2: L21 below block j+1 (after the caller's update of block column
     // j below block j by step j-1)
     const int below2 = n - j2;
     if (j > 0) HIPCHECK(hipStreamWaitEvent(sp2, ev_gc, 0));
     if (below2 > 0) {
       c2 *Aj2_j = A + (size_t)j * lda + j2;
-      hipLaunchKernelGGL(k_split_rows<8>, pgrid, dim3(256), 0, sp2, (const c2 *)Aj2_j, lda, below2, jb, kp, P.pan.mp,
-                         P.pan.PA, P.pan.rf);
+      hipLaunchKernelGGL(k_split_rows<8>, pgrid, dim3(256), 0, sp2, (const c2 *)Aj2_j, lda, below2, jb, kp,
+                         P.pan.mp, P.pan.PA, P.pan.rf);
       HIPCHECK(hipStreamWaitEvent(sp2, ev_uinv, 0));
       gemm_ops(below2, jb, jb, Aj2_j, lda, P.pan.PA, P.pan.rf, P.pan.mp, up.P, up.rf, LB, 0, sp2, 1);
     }
     HIPCHECK(hipEventRecord(ev_lr, sp2));
-    // side: L21 of block j+1's rows, block j+1's diagonal block, its factorisation
+    // side: L21 of block j+1's rows, block j+1's diagonal block once U12 of
+    // block column j+1 is there, its factorisation
     if (j > 0) HIPCHECK(hipStreamWaitEvent(sp, ev_gc, 0));
     hipLaunchKernelGGL(k_split_rows<8>, sgrid, dim3(256), 0, sp, (const c2 *)Aj1_j, lda, jb1, jb, kp, (int64_t)LB,
                        P.small.PA, P.small.rf);
@@ -1226,6 +1281,7 @@ int sched_chain(c2 *A, int n, int64_t lda, int *ipiv, int *flags, const LocalPtr
     HIPCHECK(hipEventRecord(ev_l1, sp));
     hipLaunchKernelGGL(k_split_rows<8>, sgrid, dim3(256), 0, sp, (const c2 *)Aj1_j, lda, jb1, jb, kp, (int64_t)LB,
                        P.small.PA, P.small.rf);
+    HIPCHECK(hipStreamWaitEvent(sp, ev_u1, 0));
     hipLaunchKernelGGL(k_split_cols<8>, sgrid, dim3(256), 0, sp, (const c2 *)Aj_1, lda, jb, jb1, kp, (int64_t)LB,
                        P.small.PB, P.small.cf);
     gemm_ops(jb1, jb1, jb, Aj1_1, lda, P.small.PA, P.small.rf, LB, P.small.PB, P.small.cf, LB, 0, sp, 0);
@@ -1236,7 +1292,7 @@ int sched_chain(c2 *A, int n, int64_t lda, int *ipiv, int *flags, const LocalPtr
     const int rest1 = n - j2;
     if (rest1 > 0) {
       c2 *Aj_2 = A + (size_t)j2 * lda + j;
-      bfa::laswp_c(A, lda, j2, n, j, jb, ipiv, st);
+      laswp_local(A, lda, j2, n, j, jb, ipiv, st);
       hipLaunchKernelGGL(k_split_cols<8>, bgrid, dim3(256), 0, st, (const c2 *)Aj_2, lda, jb, rest1, kp, P.bulk.mp,
                          P.bulk.PB, P.bulk.cf);
       gemm_ops(jb, rest1, jb, Aj_2, lda, lp.P, lp.rf, LB, P.bulk.PB, P.bulk.cf, P.bulk.mp, 0, st, 1);
@@ -1268,7 +1324,7 @@ int sched_chain(c2 *A, int n, int64_t lda, int *ipiv, int *flags, const LocalPtr
       gemm_ops(below, nn, jb, A + (size_t)j2 * lda + j1, lda, P.bulk.PA, P.bulk.rf, P.bulk.mp, P.bulk.PB,
                P.bulk.cf, P.bulk.mp, jb1, st, 0);
     HIPCHECK(hipEventRecord(ev_nn, st));
-    if (j > 0) bfa::laswp_c(A, lda, 0, j, j, jb, ipiv, st);
+    if (j > 0) laswp_local(A, lda, 0, j, j, jb, ipiv, st);
     if (first) {                                         // rows of block j+1 right of block j+3
       const int e = j3 + jb3;
       gemm_ops(jb1, n - e, jb, A + (size_t)e * lda + j1, lda, P.bulk.PA, P.bulk.rf, P.bulk.mp, P.bulk.PB,
