@@ -10,6 +10,12 @@ Per step: warm assembly of the complex128 VIE matrix (static table resident,
 built once before timing = the reference's FIBBI cache), LU factorisation,
 two-RHS solve.  Inputs are resident in HBM before the timed region.
 
+Each form (fp64, mixed, fast, graph) is timed in a process of its own, one
+after the other: the library's look-ahead streams of one form would
+otherwise share hardware queues with another's (measured: the fast form's
+factorisation took 132 instead of 104 ms when its streams were created
+after the partitioned LU's CU-masked ones).
+
 Prints one JSON line (metric: seconds per frequency, lower is better).
 """
 import argparse
@@ -34,7 +40,7 @@ PEAK_HBM_GBS = 8000.0
 # read off a copy of known size in the same run): rocBLAS cgemm's launches on
 # the LU's update CUs (mixed form), and the average over all k_gemm_hs
 # launches of the fast form's factorisation
-MEASURED_TRAFFIC = {("rocblas_cgemm", 19872): 1.788e9, ("k_gemm_hs", 19872): 0.771e9}
+MEASURED_TRAFFIC = {("rocblas_cgemm", 19872): 1.788e9, ("k_gemm_hs", 19872): 0.271e9}
 
 
 def load_bfa():
@@ -220,48 +226,9 @@ def bench_batch(args, bfa, torch, log):
     print(json.dumps(out))
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=12, help="subcubes per axis (12 = the N=19,872 workload)")
-    ap.add_argument("--td-order", type=int, default=10)
-    ap.add_argument("--forms", default="fp64,mixed,fast,graph",
-                    help="comma list of fp64, mixed (complex64 partial-pivoting LU + FP64 refinement), fast "
-                         "(block-local pivoting, split-FP16 trailing updates + FP64 refinement), graph (mixed "
-                         "in grouped kernels replayed from a hipGraph)")
-    ap.add_argument("--workload", default="single", choices=["single", "batch"],
-                    help="single: one N=19,872 sphere; batch: 48 uneven spheres x 8 frequencies")
-    ap.add_argument("--batch-count", type=int, default=48)
-    ap.add_argument("--refine-tol", type=float, default=1e-10)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="first run a plain 1 GiB device-to-device copy (1 GiB read, 1 GiB written), against which "
-                         "counter runs scale FETCH_SIZE / WRITE_SIZE")
-    ap.add_argument("--quiet", action="store_true")
-    args = ap.parse_args()
-
-    def log(*a):
-        if not args.quiet:
-            print(*a, file=sys.stderr, flush=True)
-
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU sharding of this "
-              f"path is a design note, not a measured configuration.", file=sys.stderr)
-        sys.exit(2)
-
-    import torch
-    bfa = load_bfa()
-    if args.copy_calibration:
-        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-        torch.cuda.empty_cache()
-    if args.workload == "batch":
-        return bench_batch(args, bfa, torch, log)
+def run_forms_here(args, forms, bfa, torch, log):
+    """Times each form in this process: warm-up steps, then args.steps timed
+    steps bracketed by device synchronisations."""
     dev = torch.device("cuda:0")
     omega = 1.0
     eps = (4 + 0.05j, 5 + 0.05j, 6 + 0.05j)
@@ -286,7 +253,6 @@ def main():
     dp = torch.empty(N, dtype=torch.int32, device=dev)
 
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
     dM32 = torch.empty((N, N), dtype=torch.complex64, device=dev) if ("mixed" in forms or "fast" in forms) else None
     refine = {"iters": 0, "resid": 0.0}
 
@@ -357,6 +323,123 @@ def main():
                          "P": P.copy(), "refine": dict(refine) if form != "fp64" else None}
         log(f"{form}: {t_total / args.steps:.4f} s/frequency  phases {phase}  refine {refine if form != 'fp64' else ''}")
 
+    return results, g, V, T, N, t_cold
+
+
+def save_form_result(path, form, r, N, nt, t_cold):
+    meta = {k: v for k, v in r.items() if k not in ("P", "J")}
+    meta.update({"N": N, "nt": nt, "t_cold": t_cold, "form": form})
+    arrays = {"P": np.asarray(r["P"])}
+    if "J" in r:
+        arrays["J"] = np.asarray(r["J"])
+    np.savez(path, meta=json.dumps(meta, default=lambda o: o.item() if hasattr(o, "item") else str(o)), **arrays)
+
+
+def run_forms_in_children(args, forms, log):
+    """One process per form (this script with --one-form), run one after the
+    other; each writes its timings, powers and solution to a file read back
+    here.  Every form's library streams and their hardware queues then start
+    from the same state."""
+    import subprocess
+    import tempfile
+    results, N, nt, t_cold = {}, None, None, None
+    with tempfile.TemporaryDirectory() as tmp:
+        for form in forms:
+            path = os.path.join(tmp, f"{form}.npz")
+            cmd = [sys.executable, os.path.abspath(__file__), "--one-form", form, "--result-file", path,
+                   "--steps", str(args.steps), "--warmup", str(args.warmup), "--n", str(args.n),
+                   "--td-order", str(args.td_order), "--refine-tol", repr(args.refine_tol), "--no-cpu-baseline"]
+            if args.quiet:
+                cmd.append("--quiet")
+            if args.copy_calibration:
+                cmd.append("--copy-calibration")
+            rc = subprocess.run(cmd).returncode
+            if rc != 0:
+                raise RuntimeError(f"form {form}: child process exited with {rc}")
+            z = np.load(path)
+            meta = json.loads(str(z["meta"]))
+            r = {k: v for k, v in meta.items() if k not in ("N", "nt", "t_cold", "form")}
+            for k in ("gemm", "gemm_big", "gemm_slice", "gemm_split"):
+                if k in r:
+                    r[k] = tuple(r[k])
+            r["P"] = z["P"]
+            if "J" in z.files:
+                r["J"] = z["J"]
+            results[form] = r
+            N, nt = meta["N"], meta["nt"]
+            t_cold = meta["t_cold"] if t_cold is None else t_cold
+    return results, N, nt, t_cold
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=12, help="subcubes per axis (12 = the N=19,872 workload)")
+    ap.add_argument("--td-order", type=int, default=10)
+    ap.add_argument("--forms", default="fp64,mixed,fast,graph",
+                    help="comma list of fp64, mixed (complex64 partial-pivoting LU + FP64 refinement), fast "
+                         "(block-local pivoting, split-FP16 trailing updates + FP64 refinement), graph (mixed "
+                         "in grouped kernels replayed from a hipGraph)")
+    ap.add_argument("--workload", default="single", choices=["single", "batch"],
+                    help="single: one N=19,872 sphere; batch: 48 uneven spheres x 8 frequencies")
+    ap.add_argument("--batch-count", type=int, default=48)
+    ap.add_argument("--refine-tol", type=float, default=1e-10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 1 GiB device-to-device copy (1 GiB read, 1 GiB written), against which "
+                         "counter runs scale FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--same-process", action="store_true",
+                    help="run every form in this process (by default each form of a multi-form run gets a process "
+                         "of its own, so that no form inherits another's streams and hardware-queue assignment)")
+    ap.add_argument("--one-form", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--result-file", default=None, help=argparse.SUPPRESS)
+    args = ap.parse_args()
+
+    def log(*a):
+        if not args.quiet:
+            print(*a, file=sys.stderr, flush=True)
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU sharding of this "
+              f"path is a design note, not a measured configuration.", file=sys.stderr)
+        sys.exit(2)
+
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    if args.workload == "single" and args.one_form is None and len(forms) > 1 and not args.same_process:
+        results, N, nt, t_cold = run_forms_in_children(args, forms, log)
+        if not args.no_cpu_baseline:
+            import torch
+            bfa = load_bfa()
+            V, T = bfa.kuhn_sphere(args.n, 1.0, 0.05, 0x5EED)
+            g = bfa.Geometry(V, T)
+            g.upload()
+            g.build_static(args.td_order)
+            torch.cuda.synchronize()
+    else:
+        import torch
+        bfa = load_bfa()
+        if args.copy_calibration:
+            src = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0")
+            dst = torch.empty_like(src)
+            dst.copy_(src)
+            torch.cuda.synchronize()
+            del src, dst
+            torch.cuda.empty_cache()
+        if args.workload == "batch":
+            return bench_batch(args, bfa, torch, log)
+        if args.one_form is not None:
+            forms = [args.one_form]
+        results, g, V, T, N, t_cold = run_forms_here(args, forms, bfa, torch, log)
+        nt = g.nt
+        if args.one_form is not None:
+            save_form_result(args.result_file, args.one_form, results[args.one_form], N, nt, t_cold)
+            return
+    omega = 1.0
+    eps = (4 + 0.05j, 5 + 0.05j, 6 + 0.05j)
+    spec = [((1, 0, 0), (0, 0, 1)), ((0, 1, 0), (0, 0, 1))]
     lu_flops = 8.0 / 3.0 * N ** 3
     ref = results.get("fp64")
     errs = {}
@@ -399,7 +482,7 @@ def main():
         "fp64_s_per_freq": ref["s_per_freq"] if ref else None,
         "speedup_vs_fp64": ref["s_per_freq"] / r["s_per_freq"] if ref else None,
         "data": "synthetic",
-        "config": {"workload": "kuhn_sphere_n12_diag_eps_2rhs", "N": N, "tets": g.nt, "form": best,
+        "config": {"workload": "kuhn_sphere_n12_diag_eps_2rhs", "N": N, "tets": nt, "form": best,
                    "omega": omega, "eps": "diag(4,5,6)+0.05i", "nrhs": len(spec), "td_order": args.td_order},
         "power_rel_err_vs_fp64": err,
         "speedup_mixed_over_fp64": (results["fp64"]["s_per_freq"] / results["mixed"]["s_per_freq"]

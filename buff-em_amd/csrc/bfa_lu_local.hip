@@ -966,7 +966,10 @@ hipEvent_t sp_event(hipStream_t st)
 // super-tile edge G (4, or smaller while fewer than 16 super-tiles would
 // leave XCDs idle) and the padded grid of super_tile().  Measured at
 // 19,616 x 19,360 x 256: G = 1, 2 or 4 2.86 ms, G = 8 3.03 ms, G = 16
-// 3.16 ms (an 8 x 8 super-tile's operand slabs fill an XCD's 4 MB L2).
+// 3.16 ms (an 8 x 8 super-tile's operand slabs fill an XCD's 4 MB L2); at
+// k = 512, alone: G = 1, 2, 4, 8 4.43, 4.28, 4.37, 4.72 ms, but G = 2
+// fetched 10.1 GB against 7.8 GB for G = 4 and was no faster inside the
+// factorisation, where other kernels share the bandwidth.
 void super_grid(int tm, int tn, int *G, size_t *nblk)
 {
   int g = 4;

@@ -188,6 +188,21 @@ def test_surrogate_gain(bfa, n5, torch_cuda):
     # recomputed by cubature, so the table is at least as good as the rule
     nf7, _ = S.table(w, mode=S.SURROGATE, tol=1e-7)
     print(f"[surrogate] cubature fallbacks at tol 1e-7: {nf7}/{S.npairs}", flush=True)
+    # wall time of each way to fill the table of all pairs (second call of
+    # each, after a warm one), on the device
+    def timed(**kw):
+        S.table(w, **kw)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r = S.table(w, **kw)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, r
+    tt = {"4-point rule": timed(mode=S.RULE)[0], "16-point rule": timed(mode=S.CUBATURE, npts=16, level=0)[0],
+          "cubature 33 x refined once (labels)": timed(mode=S.CUBATURE, npts=33, level=1)[0],
+          "surrogate, no fallback": timed(mode=S.SURROGATE, tol=np.inf)[0],
+          "surrogate, fallback at tol 1e-7": timed(mode=S.SURROGATE, tol=1e-7)[0]}
+    print("[surrogate] table time for " + str(S.npairs) + " pairs: " +
+          ", ".join(f"{k} {v * 1e3:.1f} ms" for k, v in tt.items()), flush=True)
     # solved powers: reference rule vs surrogate vs cubature for every pair
     mat = bfa.material("iso", eps=4 + 0.1j)
     pw = bfa.planewaves([((1, 0, 0), (0, 0, 1))])
