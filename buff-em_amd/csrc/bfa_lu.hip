@@ -349,10 +349,18 @@ static void prof_begin(hipStream_t st, hipEvent_t &a)
 // factorisation takes longer (317 against 297 ms).  BFA_GEMM=own or =blas
 // selects one kind for both precisions.  Streams being captured into a graph
 // always use the hand-written kernels.
-static rocblas_handle g_blas[64] = {};
+// One rocBLAS handle per (device, stream): the CU-partitioned LU runs
+// rocBLAS GEMMs on two streams at the same time, and a handle's workspace
+// and state belong to one stream.  The table keeps the first 32 pairs; a
+// further stream shares the last entry's handle, re-pointed at it.
+struct BlasEntry {
+  int dev = -1;
+  hipStream_t st = nullptr;
+  rocblas_handle h = nullptr;
+};
+static BlasEntry g_blas[32];
 static rocblas_handle blas_handle(hipStream_t st, bool want)
 {
-  rocblas_handle *h = g_blas;
   static int mode = -1;     // 0: own kernels, 1: where the caller wants rocBLAS, 2: always
   if (mode < 0) {
     const char *e = getenv("BFA_GEMM");
@@ -362,13 +370,27 @@ static rocblas_handle blas_handle(hipStream_t st, bool want)
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return nullptr;
   int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
-  if (!h[dev] && rocblas_create_handle(&h[dev]) != rocblas_status_success) {
-    h[dev] = nullptr;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  const int ne = (int)(sizeof g_blas / sizeof g_blas[0]);
+  int i = 0;
+  for (; i < ne && g_blas[i].h; i++)
+    if (g_blas[i].dev == dev && g_blas[i].st == st) return g_blas[i].h;
+  if (i == ne) {            // table full: share the last entry's handle
+    BlasEntry &e = g_blas[ne - 1];
+    if (e.dev != dev || rocblas_set_stream(e.h, st) != rocblas_status_success) return nullptr;
+    e.st = st;
+    return e.h;
+  }
+  rocblas_handle h = nullptr;
+  if (rocblas_create_handle(&h) != rocblas_status_success) return nullptr;
+  if (rocblas_set_stream(h, st) != rocblas_status_success) {
+    (void)rocblas_destroy_handle(h);
     return nullptr;
   }
-  if (rocblas_set_stream(h[dev], st) != rocblas_status_success) return nullptr;
-  return h[dev];
+  g_blas[i].dev = dev;
+  g_blas[i].st = st;
+  g_blas[i].h = h;
+  return h;
 }
 
 static bool blas_gemm_sub(rocblas_handle h, int m, int n, int k, const cplx<double> *A, int64_t lda,
@@ -2013,6 +2035,14 @@ namespace bfa {
 // handles.  A later LU call creates them again.
 void lu_release()
 {
+  // the handles first: each is bound to a stream released below
+  for (auto &e : g_blas) {
+    if (e.h && hipSetDevice(e.dev) == hipSuccess) {
+      (void)hipDeviceSynchronize();
+      (void)rocblas_destroy_handle(e.h);
+    }
+    e = BlasEntry();
+  }
   for (int d = 0; d < 16; d++) {
     LUWork &w = g_lu[d];
     if (!w.mem && !w.side && !w.s_pan) continue;
@@ -2026,11 +2056,6 @@ void lu_release()
     for (auto &e : w.evm) if (e) (void)hipEventDestroy(e);
     w = LUWork();
   }
-  for (int d = 0; d < 64; d++)
-    if (g_blas[d]) {
-      (void)rocblas_destroy_handle(g_blas[d]);
-      g_blas[d] = nullptr;
-    }
   for (hipEvent_t e : g_prof.pool) (void)hipEventDestroy(e);
   g_prof = GemmProfile();
 }
