@@ -69,9 +69,10 @@ static void dev_free_all(bfa::DevBuffers &d)
 {
   void **ptrs[] = {&d.V, &d.tetVI, &d.tetFI, &d.tetVol, &d.q4, &d.q16, &d.q33, &d.bfV, &d.bfT,
                    &d.bfPF, &d.bfQ, &d.bfRad, &d.rowptr, &d.cols, &d.pairRow, &d.ftTab, &d.tdPairs,
-                   &d.tdVerts, &d.tdMom, &d.pairTD, &d.table};
+                   &d.tdVerts, &d.tdMom, &d.pairTD, &d.table, &d.tpTab};
   for (void **p : ptrs)
     if (*p) { hipFree(*p); *p = nullptr; }
+  d.tpTabBytes = 0;
 }
 
 extern "C" void bfa_geometry_destroy(bfa_geometry *g)
@@ -658,8 +659,8 @@ __device__ __forceinline__ double rsqrt_nr(double x)
 // 4-point product-rule moments of one tetrahedron pair (signed weights):
 // row tetrahedron us of the tile (points in sh), column tetrahedron's points
 // ys and centroid cT in registers; out[i * stride] for i < 16
-template <bool REALK>
-__device__ __forceinline__ void tp_moments(const FarTpShared &sh, int us, const double *cS, const double *ys,
+template <bool REALK, typename SH>
+__device__ __forceinline__ void tp_moments(const SH &sh, int us, const double *cS, const double *ys,
                                            const double *cT, double kre, double kim, double *out, int stride)
 {
   double m[16];
@@ -696,7 +697,8 @@ __device__ __forceinline__ void tp_moments(const FarTpShared &sh, int us, const 
 }
 
 // points and centroid of distinct tetrahedron u of side `which` into registers
-__device__ __forceinline__ void tp_load(const FarTpShared &sh, int which, int u, double *x, double *c)
+template <typename SH>
+__device__ __forceinline__ void tp_load(const SH &sh, int which, int u, double *x, double *c)
 {
 #pragma unroll
   for (int e = 0; e < 16; e++) x[e] = sh.pts[which][e][u];
@@ -800,6 +802,176 @@ __global__ __launch_bounds__(FT_THREADS) void k_far_tp(DevGeom g, double kre, do
   int ti, tj;
   tile_coords(blockIdx.x, ntiles, ti, tj);
   far_tp_tile<REALK>(g, kre, kim, M, ld, ti, tj, sh);
+}
+
+// Two-pass far field.  A 32 x 32 BF tile evaluates ~31 x 31 tetrahedron
+// pairs, and a tetrahedron lies in the support of BFs that fall in ~1.9
+// different tiles per side, so the single pass evaluates each pair 3.45
+// times on average.  Pass 1 evaluates every tetrahedron pair S <= T once
+// (32 x 32 tetrahedron tiles) and stores its 16 moments, 128 B, at
+// S nt - S (S - 1) / 2 + (T - S); pass 2 is the single pass's combination
+// with the moments gathered from that table (for S > T the record of (T, S)
+// with Ix and Iy exchanged: the kernel g is symmetric).  Pairs of one
+// tetrahedron with itself are never needed (their BF pairs touch) and are
+// stored as zeros.
+__device__ __forceinline__ size_t tp_index(int S, int T, int nt)
+{
+  return (size_t)S * nt - (size_t)S * (S - 1) / 2 + (size_t)(T - S);
+}
+
+// points and centroids of a 32 x 32 tetrahedron tile (pass 1 keeps no
+// moments in LDS: 19 KB, so that several workgroups share a CU)
+struct TpTileShared {
+  double pts[2][16][2 * FT2];
+  double cen[2][3][2 * FT2];
+};
+#define TPT_THREADS 256
+
+template <bool REALK>
+__global__ __launch_bounds__(TPT_THREADS) void k_tp_table(DevGeom g, double kre, double kim, double *tab, int ntb)
+{
+  __shared__ TpTileShared sh;
+  int bs, bt;
+  tile_coords(blockIdx.x, ntb, bs, bt);
+  const int tid = threadIdx.x, nt = g.nt;
+  // points of the tile's 32 row and 32 column tetrahedra
+  for (int idx = tid; idx < 2 * 32 * 16; idx += TPT_THREADS) {
+    const int which = idx / (32 * 16), u = (idx / 16) % 32, e = idx % 16;
+    const int t = (which ? bt : bs) * 32 + u;
+    sh.pts[which][e][u] = t < nt ? g.q4[(size_t)t * 16 + e] : 0.0;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < 2 * 32; idx += TPT_THREADS) {
+    const int which = idx / 32, u = idx % 32;
+    for (int d = 0; d < 3; d++)
+      sh.cen[which][d][u] = 0.25 * (sh.pts[which][d][u] + sh.pts[which][4 + d][u] + sh.pts[which][8 + d][u] +
+                                    sh.pts[which][12 + d][u]);
+  }
+  __syncthreads();
+  // lanes along the column tetrahedron: consecutive records
+  for (int pidx = tid; pidx < 32 * 32; pidx += TPT_THREADS) {
+    const int us = pidx / 32, ut = pidx % 32;
+    const int S = bs * 32 + us, T = bt * 32 + ut;
+    if (S >= nt || T >= nt || S > T) continue;
+    double m[16];
+    if (S == T) {
+#pragma unroll
+      for (int i = 0; i < 16; i++) m[i] = 0.0;
+    } else {
+      double xs[16], cS[3], ys[16], cT[3];
+      tp_load(sh, 0, us, xs, cS);
+      tp_load(sh, 1, ut, ys, cT);
+      tp_moments<REALK>(sh, us, cS, ys, cT, kre, kim, m, 1);
+    }
+    double2 *o = (double2 *)(tab + tp_index(S, T, nt) * 16);
+#pragma unroll
+    for (int i = 0; i < 8; i++) o[i] = make_double2(m[2 * i], m[2 * i + 1]);
+  }
+}
+
+// the 16 moments of row tetrahedron S and column tetrahedron T from the table
+__device__ __forceinline__ void tp_fetch(const double *__restrict__ tab, int S, int T, int nt, double *m)
+{
+  const bool sw = S > T;
+  const double2 *r = (const double2 *)(tab + (sw ? tp_index(T, S, nt) : tp_index(S, T, nt)) * 16);
+  double v[16];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const double2 x = r[i];
+    v[2 * i] = x.x;
+    v[2 * i + 1] = x.y;
+  }
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    // I0 (0, 1) and Ixy (14, 15) stay; Ix (2..7) and Iy (8..13) exchange
+    const int src = (i >= 2 && i < 14 && sw) ? (i < 8 ? i + 6 : i - 6) : i;
+    m[i] = v[src];
+  }
+}
+
+// Pass 2: the single pass's tile with the moments read from the table
+// instead of evaluated.  Measured at N = 19,872 against variants that were
+// slower: eight lanes per record (16.2 ms), results staged in LDS for
+// coalesced stores of both triangles (13.3 ms), the BF data staged in LDS
+// (15.5 ms), 26 KB of LDS and the records read per BF pair (15.8 ms) -- the
+// tile is bound by the latency of its dependent global reads, not by HBM.
+__global__ __launch_bounds__(FT_THREADS) void k_far_gather(DevGeom g, double kre, double kim,
+                                                           const double *__restrict__ tpt, cd *M, int64_t ld,
+                                                           int ntiles)
+{
+  __shared__ FarTpShared sh;
+  int ti, tj;
+  tile_coords(blockIdx.x, ntiles, ti, tj);
+  const int tid = threadIdx.x, nth = blockDim.x, N = g.nbf, nt = g.nt;
+  const int *tab[2] = {g.ftTab + (size_t)ti * FTAB, g.ftTab + (size_t)tj * FTAB};
+  const int nr = tab[0][4 * FT2], nc = tab[1][4 * FT2];
+  for (int idx = tid; idx < 2 * 2 * FT2; idx += nth) {
+    const int which = idx / (2 * FT2), k = idx % (2 * FT2);
+    sh.slot[which][k] = tab[which][2 * FT2 + k];
+  }
+  for (int idx = tid; idx < 2 * 2 * FT2 * 16; idx += nth) {
+    const int which = idx / (2 * FT2 * 16), rem = idx % (2 * FT2 * 16);
+    const int u = rem / 16, e = rem % 16;
+    if (u < (which ? nc : nr)) sh.pts[which][e][u] = g.q4[(size_t)tab[which][u] * 16 + e];
+  }
+  __syncthreads();
+  for (int idx = tid; idx < 2 * 2 * FT2; idx += nth) {
+    const int which = idx / (2 * FT2), u = idx % (2 * FT2);
+    if (u < (which ? nc : nr))
+      for (int d = 0; d < 3; d++)
+        sh.cen[which][d][u] = 0.25 * (sh.pts[which][d][u] + sh.pts[which][4 + d][u] + sh.pts[which][8 + d][u] +
+                                      sh.pts[which][12 + d][u]);
+  }
+  const bool direct = nr * nc > FTT_MAX;
+  if (!direct) {
+    for (int pidx = tid; pidx < nr * nc; pidx += nth) {
+      double m[16];
+      tp_fetch(tpt, tab[0][pidx / nc], tab[1][pidx % nc], nt, m);
+#pragma unroll
+      for (int i = 0; i < 16; i++) sh.mom[i][pidx] = m[i];
+    }
+  }
+  __syncthreads();
+  const double k2re = kre * kre - kim * kim, k2im = 2.0 * kre * kim;
+  const double ik2d = k2re * k2re + k2im * k2im;
+  const double ik2re = k2re / ik2d, ik2im = -k2im / ik2d;
+  for (int pidx = tid; pidx < FT2 * FT2; pidx += nth) {
+    const int lb = pidx % FT2, la = pidx / FT2;
+    const int a = ti * FT2 + la, b = tj * FT2 + lb;
+    if (a >= N || b >= N || b < a) continue;
+    if (bf_ncv(g.bfV + 5 * a, g.bfV + 5 * b) > 0) continue;
+    double accr = 0.0, acci = 0.0;
+    for (int sa = 0; sa < 2; sa++) {
+      const int us = sh.slot[0][2 * la + sa];
+      const double pfa = g.bfPF[2 * a + sa];
+      const double *Qa = g.bfQ + 6 * a + 3 * sa;
+      const double al0 = Qa[0] - sh.cen[0][0][us], al1 = Qa[1] - sh.cen[0][1][us], al2 = Qa[2] - sh.cen[0][2][us];
+      for (int sb = 0; sb < 2; sb++) {
+        const int ut = sh.slot[1][2 * lb + sb];
+        const double pfb = g.bfPF[2 * b + sb];
+        const double *Qb = g.bfQ + 6 * b + 3 * sb;
+        const double be0 = Qb[0] - sh.cen[1][0][ut], be1 = Qb[1] - sh.cen[1][1][ut], be2 = Qb[2] - sh.cen[1][2][ut];
+        double m[16];
+        if (direct) {
+          tp_fetch(tpt, tab[0][us], tab[1][ut], nt, m);
+        } else {
+#pragma unroll
+          for (int i = 0; i < 16; i++) m[i] = sh.mom[i][us * nc + ut];
+        }
+        const double ab = al0 * be0 + al1 * be1 + al2 * be2;
+        double dr = m[14] - (be0 * m[2] + be1 * m[4] + be2 * m[6]) - (al0 * m[8] + al1 * m[10] + al2 * m[12]) + ab * m[0];
+        double di = m[15] - (be0 * m[3] + be1 * m[5] + be2 * m[7]) - (al0 * m[9] + al1 * m[11] + al2 * m[13]) + ab * m[1];
+        dr -= 9.0 * (m[0] * ik2re - m[1] * ik2im);
+        di -= 9.0 * (m[0] * ik2im + m[1] * ik2re);
+        const double f = pfa * pfb;
+        accr += f * dr;
+        acci += f * di;
+      }
+    }
+    const cd v = {accr / FOURPI, acci / FOURPI};
+    M[(size_t)b * ld + a] = v;
+    M[(size_t)a * ld + b] = v;
+  }
 }
 
 // FP32 variant of the far-field tile (the precision form's far field):
@@ -1356,6 +1528,34 @@ __global__ __launch_bounds__(256) void k_near(DevGeom g, double kre, double kim,
   if (j < g.nnz) near_entry(g, kre, kim, table, mat, M, ld, j);
 }
 
+// The two-pass far field's moment table (nt (nt + 1) / 2 records of 128 B,
+// 6.9 GB at 10,368 tetrahedra), allocated on first use and kept with the
+// geometry; null -- the single-pass kernel -- when it would take more than a
+// quarter of the free device memory, when the allocation fails, or with
+// BFA_FAR_TABLE=0.
+static double *far_table(bfa_geometry *g)
+{
+  static int use = -1;
+  if (use < 0) {
+    const char *e = getenv("BFA_FAR_TABLE");
+    use = !(e && strcmp(e, "0") == 0);
+  }
+  if (!use) return nullptr;
+  const size_t need = (size_t)g->nt * (g->nt + 1) / 2 * 16 * sizeof(double);
+  if (g->d.tpTab && g->d.tpTabBytes >= need) return (double *)g->d.tpTab;
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess || need > fr / 4) return nullptr;
+  void *p = nullptr;
+  if (hipMalloc(&p, need) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  if (g->d.tpTab) (void)hipFree(g->d.tpTab);
+  g->d.tpTab = p;
+  g->d.tpTabBytes = need;
+  return (double *)p;
+}
+
 static int assemble_vie(bfa_geometry *g, double omega_re, double omega_im, const bfa_material *mat, void *d_M,
                         int64_t ld, bool far32, void *stream)
 {
@@ -1378,7 +1578,19 @@ static int assemble_vie(bfa_geometry *g, double omega_re, double omega_im, const
   } else {
     const int nt2 = (N + FT2 - 1) / FT2;
     const int64_t nblk2 = (int64_t)nt2 * (nt2 + 1) / 2;
-    if (omega_im == 0.0)
+    double *tpt = far_table(g);
+    if (tpt) {
+      const int ntb = (g->nt + 31) / 32;
+      const int64_t nblk1 = (int64_t)ntb * (ntb + 1) / 2;
+      if (omega_im == 0.0)
+        hipLaunchKernelGGL(k_tp_table<true>, dim3((unsigned)nblk1), dim3(TPT_THREADS), 0, st, g->dg, omega_re, 0.0,
+                           tpt, ntb);
+      else
+        hipLaunchKernelGGL(k_tp_table<false>, dim3((unsigned)nblk1), dim3(TPT_THREADS), 0, st, g->dg, omega_re,
+                           omega_im, tpt, ntb);
+      hipLaunchKernelGGL(k_far_gather, dim3((unsigned)nblk2), dim3(FT_THREADS), 0, st, g->dg, omega_re, omega_im,
+                         (const double *)tpt, M, ld, nt2);
+    } else if (omega_im == 0.0)
       hipLaunchKernelGGL(k_far_tp<true>, dim3((unsigned)nblk2), dim3(FT_THREADS), 0, st, g->dg, omega_re, 0.0, M, ld, nt2);
     else
       hipLaunchKernelGGL(k_far_tp<false>, dim3((unsigned)nblk2), dim3(FT_THREADS), 0, st, g->dg, omega_re, omega_im, M, ld,

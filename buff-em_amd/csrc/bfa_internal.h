@@ -54,6 +54,8 @@ struct DevBuffers {
   void *tdMom = nullptr;       // [ntd][3][8] moments
   void *pairTD = nullptr;      // [nnz][4] int: TD index*2+swap, or -1 for 16-pt
   void *table = nullptr;       // [nnz][6] FIBBI records
+  void *tpTab = nullptr;       // far field: 16 moments per tetrahedron pair S <= T (per frequency)
+  size_t tpTabBytes = 0;
 };
 
 }  // namespace bfa

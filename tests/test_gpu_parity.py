@@ -197,3 +197,50 @@ def test_cpp_driver_matches_python_path(bfa, orc, torch_cuda):
         P = o.emt_power(w, pw, J)[0]
         got = np.array([float(l[2]), float(l[3])])
         assert np.all(np.abs(got - P) / np.abs(P) < 1e-8), (got, P)
+
+
+_SINGLE_PASS_CHILD = r"""
+import os, sys
+import numpy as np
+import torch
+sys.path.insert(0, os.path.join(sys.argv[1], "tests"))
+from conftest import load_bfa, kuhn
+bfa = load_bfa()
+V, T = kuhn(bfa, 6)
+g = bfa.Geometry(V, T)
+g.upload()
+g.build_static(10)
+N = g.N
+dM = torch.zeros((N, N), dtype=torch.complex128, device="cuda")
+g.assemble_vie(1.7, bfa.material("diag", eps=(4 + 0.05j, 5 + 0.05j, 6 + 0.05j)), dM)
+np.save(sys.argv[2], dM.cpu().numpy())
+"""
+
+
+def test_far_field_table_equals_single_pass(bfa, torch_cuda, tmp_path):
+    """The two-pass far field (every tetrahedron pair's moments once into a
+    table, then the BF tiles gathering them) against the single-pass
+    tetrahedron-pair kernel, which a child process selects with
+    BFA_FAR_TABLE=0 (the switch is read once per process): the same sums in
+    another order, n = 6 sphere (N = 2,376), complex128 to 1e-14 relative."""
+    import os
+    import subprocess
+    import sys
+    torch = torch_cuda
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = str(tmp_path / "single.npy")
+    r = subprocess.run([sys.executable, "-c", _SINGLE_PASS_CHILD, root, out], capture_output=True, text=True,
+                       env=dict(os.environ, BFA_FAR_TABLE="0"), timeout=240)
+    print(r.stdout, r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    V, T = kuhn(bfa, 6)
+    g = bfa.Geometry(V, T)
+    g.upload()
+    g.build_static(10)
+    N = g.N
+    dM = torch.zeros((N, N), dtype=torch.complex128, device="cuda")
+    g.assemble_vie(1.7, bfa.material("diag", eps=(4 + 0.05j, 5 + 0.05j, 6 + 0.05j)), dM)
+    A, B = dM.cpu().numpy(), np.load(out)
+    err = np.linalg.norm(A - B) / np.linalg.norm(B)
+    print(f"[far table] N={N}: two-pass vs single-pass {err:.2e}", flush=True)
+    assert err < 1e-14
