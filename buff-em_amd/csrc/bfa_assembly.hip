@@ -890,16 +890,22 @@ __device__ __forceinline__ void tp_fetch(const double *__restrict__ tab, int S, 
 }
 
 // Pass 2: the single pass's tile with the moments read from the table
-// instead of evaluated.  Measured at N = 19,872 against variants that were
-// slower: eight lanes per record (16.2 ms), results staged in LDS for
-// coalesced stores of both triangles (13.3 ms), the BF data staged in LDS
-// (15.5 ms), 26 KB of LDS and the records read per BF pair (15.8 ms) -- the
-// tile is bound by the latency of its dependent global reads, not by HBM.
+// instead of evaluated, and the tile's 64 BFs (prefactors, Q minus the
+// centroid of each side's tetrahedron, vertices) read once into LDS by field
+// (11.8 ms at N = 19,872; 12.4 ms reading them per BF pair).  Slower
+// variants measured: eight lanes per record (16.2 ms), results staged in
+// LDS for coalesced stores of both triangles (13.3 ms), the BF data in LDS
+// by BF (15.5 ms: bank conflicts), 26 KB of LDS with the records read per
+// BF pair (15.8 ms) -- the tile is bound by the latency of its dependent
+// global reads, not by HBM (without the table reads it still takes
+// 10.7 ms).
 __global__ __launch_bounds__(FT_THREADS) void k_far_gather(DevGeom g, double kre, double kim,
                                                            const double *__restrict__ tpt, cd *M, int64_t ld,
                                                            int ntiles)
 {
   __shared__ FarTpShared sh;
+  __shared__ double bfd[2][8][FT2];
+  __shared__ int bfv[2][5][FT2];
   int ti, tj;
   tile_coords(blockIdx.x, ntiles, ti, tj);
   const int tid = threadIdx.x, nth = blockDim.x, N = g.nbf, nt = g.nt;
@@ -922,6 +928,18 @@ __global__ __launch_bounds__(FT_THREADS) void k_far_gather(DevGeom g, double kre
         sh.cen[which][d][u] = 0.25 * (sh.pts[which][d][u] + sh.pts[which][4 + d][u] + sh.pts[which][8 + d][u] +
                                       sh.pts[which][12 + d][u]);
   }
+  __syncthreads();
+  for (int idx = tid; idx < 2 * FT2; idx += nth) {
+    const int which = idx / FT2, k = idx % FT2;
+    const int a = (which ? tj : ti) * FT2 + k;
+    if (a >= N) continue;
+    for (int v = 0; v < 5; v++) bfv[which][v][k] = g.bfV[5 * a + v];
+    for (int sd = 0; sd < 2; sd++) {
+      const int u = sh.slot[which][2 * k + sd];
+      bfd[which][4 * sd][k] = g.bfPF[2 * a + sd];
+      for (int d = 0; d < 3; d++) bfd[which][4 * sd + 1 + d][k] = g.bfQ[6 * a + 3 * sd + d] - sh.cen[which][d][u];
+    }
+  }
   const bool direct = nr * nc > FTT_MAX;
   if (!direct) {
     for (int pidx = tid; pidx < nr * nc; pidx += nth) {
@@ -939,18 +957,21 @@ __global__ __launch_bounds__(FT_THREADS) void k_far_gather(DevGeom g, double kre
     const int lb = pidx % FT2, la = pidx / FT2;
     const int a = ti * FT2 + la, b = tj * FT2 + lb;
     if (a >= N || b >= N || b < a) continue;
-    if (bf_ncv(g.bfV + 5 * a, g.bfV + 5 * b) > 0) continue;
+    int va[5], vb[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) { va[v] = bfv[0][v][la]; vb[v] = bfv[1][v][lb]; }
+    if (bf_ncv(va, vb) > 0) continue;
     double accr = 0.0, acci = 0.0;
+#pragma unroll
     for (int sa = 0; sa < 2; sa++) {
       const int us = sh.slot[0][2 * la + sa];
-      const double pfa = g.bfPF[2 * a + sa];
-      const double *Qa = g.bfQ + 6 * a + 3 * sa;
-      const double al0 = Qa[0] - sh.cen[0][0][us], al1 = Qa[1] - sh.cen[0][1][us], al2 = Qa[2] - sh.cen[0][2][us];
+      const double pfa = bfd[0][4 * sa][la], al0 = bfd[0][4 * sa + 1][la], al1 = bfd[0][4 * sa + 2][la],
+                   al2 = bfd[0][4 * sa + 3][la];
+#pragma unroll
       for (int sb = 0; sb < 2; sb++) {
         const int ut = sh.slot[1][2 * lb + sb];
-        const double pfb = g.bfPF[2 * b + sb];
-        const double *Qb = g.bfQ + 6 * b + 3 * sb;
-        const double be0 = Qb[0] - sh.cen[1][0][ut], be1 = Qb[1] - sh.cen[1][1][ut], be2 = Qb[2] - sh.cen[1][2][ut];
+        const double pfb = bfd[1][4 * sb][lb], be0 = bfd[1][4 * sb + 1][lb], be1 = bfd[1][4 * sb + 2][lb],
+                     be2 = bfd[1][4 * sb + 3][lb];
         double m[16];
         if (direct) {
           tp_fetch(tpt, tab[0][us], tab[1][ut], nt, m);
