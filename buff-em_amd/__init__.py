@@ -437,7 +437,8 @@ class Geometry:
         return h.value, m.value
 
     def assemble_vie(self, omega, mat, d_M, ld=None, stream=None, far32=False):
-        """VIE matrix at omega; far32 evaluates the far-field cubature in FP32."""
+        """VIE matrix at omega; far32 evaluates the far-field kernel with its
+        remainder beyond second order in FP32 (the precision form)."""
         omega = complex(omega)
         fn = lib().bfa_assemble_vie_far32 if far32 else lib().bfa_assemble_vie
         _check(fn(self.h, omega.real, omega.imag, ctypes.byref(mat), _ptr(d_M), int(ld or self.N),

@@ -581,11 +581,10 @@ extern "C" int bfa_static_download(const bfa_geometry *g, double *table)
 
 // ===========================================================================
 // Warm step: dense far field over the upper triangle (touching pairs are
-// skipped: the sparse near-field pass writes them).  FP64: tetrahedron-pair
-// tiles (k_far_tp); FP32 (precision form): 64 x 64 BF tiles, lane = row a,
-// each wave sweeping 16 columns b whose points are broadcast from LDS.
+// skipped: the sparse near-field pass writes them), from tetrahedron-pair
+// moments; the precision form evaluates the kernel of those moments with
+// its rounding-sensitive part in FP32 (tp_moments_split).
 // ===========================================================================
-#define TILE 64
 
 __device__ __forceinline__ int bf_ncv(const int *va, const int *vb)
 {
@@ -696,6 +695,179 @@ __device__ __forceinline__ void tp_moments(const SH &sh, int us, const double *c
   for (int i = 0; i < 16; i++) out[i * stride] = m[i];
 }
 
+// The far-field kernel with its point evaluations in FP32 (the precision
+// form).  Plain FP32 evaluation of g at each point pair misses 1e-8 in the
+// powers by ~40x: the scalar-potential term of a BF pair is a second
+// difference of g over its four tetrahedron pairs, (h/D)^2 of the terms it
+// sums, so the terms' 6e-8 relative rounding reaches the result amplified.
+// Split instead, about the centroid separation d = cS - cT (length D,
+// direction e), with w = (x - cS) - (y - cT), p = e.w, s = |w|^2 - p^2:
+//   g(|d + w|) = g(D) [T2 + E3],
+//   T2 = 1 + c1 (p + s / 2D) + c2 p^2 / 2,  c1 = ik - 1/D,  c2 = c1^2 + 1/D^2,
+// the second-order Taylor part (FP64: one complex exponential per
+// tetrahedron pair, a polynomial per point pair), and E3, the remainder of
+// third order in |w| / D, evaluated in FP32 from formulas in which no two
+// large terms cancel, so that its rounding error stays relative to itself:
+//   r = sqrt((D + p)^2 + |w - p e|^2),  e1 = |w - p e|^2 / (D + p + r),
+//   Delta = r - D = p + e1,  u = Delta / D,  x = ik Delta,
+//   X3 = e^x - 1 - x - x^2/2 (its series for |x| <= 1),  U3 = u^3 / (1 + u),
+//   R3 = X3 / (1 + u) - (1 + x + x^2/2) U3 + x u^2 + x^2 u (u - 1) / 2
+//        [= e^x / (1 + u) - 1 - c1 Delta - c2 Delta^2 / 2],
+//   E3 = R3 + c1 (Delta - p - s/2D) + c2 (Delta^2 - p^2) / 2,
+//   Delta - p - s/2D = -s (p + Delta) / (2 D (D + p + r)),
+//   (Delta^2 - p^2) / 2 = e1 (p + Delta) / 2.
+// The sum is the same 4-point rule as tp_moments; with E3 rounded to FP32
+// the far-field elements move by ~1e-9 relative (emulated on the 1,350-BF
+// sphere: powers within 1.3e-9 of the FP64 rule).  The series holds for
+// k |Delta| <= 1; beyond it X3 comes from e^x directly.
+struct SplitK {
+  double g0r, g0i;                   // g(D) = e^{ikD} / D
+  double c1r, c1i, c2r, c2i;         // c1, c2 (FP64, for T2)
+  double e0, e1, e2, D, hiD;         // e, D, 1 / 2D
+  float f_e0, f_e1, f_e2, f_D, f_iD, f_kr, f_ki, f_c1r, f_c1i, f_c2r, f_c2i;
+};
+
+template <bool REALK>
+__device__ __forceinline__ SplitK split_setup(const double *cS, const double *cT, double kre, double kim)
+{
+  SplitK z;
+  const double d0 = cS[0] - cT[0], d1 = cS[1] - cT[1], d2 = cS[2] - cT[2];
+  const double D2 = d0 * d0 + d1 * d1 + d2 * d2, iD = rsqrt_nr(D2), D = D2 * iD;
+  z.e0 = d0 * iD; z.e1 = d1 * iD; z.e2 = d2 * iD; z.D = D; z.hiD = 0.5 * iD;
+  double sn, co;
+  sincos(kre * D, &sn, &co);
+  double amp = iD;
+  if (!REALK) amp *= exp(-kim * D);
+  z.g0r = amp * co; z.g0i = amp * sn;
+  // c1 = ik - 1/D = (-kim - 1/D) + i kre;  c2 = c1^2 + 1/D^2
+  z.c1r = -kim - iD; z.c1i = kre;
+  z.c2r = z.c1r * z.c1r - z.c1i * z.c1i + iD * iD;
+  z.c2i = 2.0 * z.c1r * z.c1i;
+  z.f_e0 = (float)z.e0; z.f_e1 = (float)z.e1; z.f_e2 = (float)z.e2;
+  z.f_D = (float)D; z.f_iD = (float)iD; z.f_kr = (float)kre; z.f_ki = (float)kim;
+  z.f_c1r = (float)z.c1r; z.f_c1i = (float)z.c1i; z.f_c2r = (float)z.c2r; z.f_c2i = (float)z.c2i;
+  return z;
+}
+
+// E3 (above) for one point pair, w in FP32
+template <bool REALK>
+__device__ __forceinline__ void split_e3(const SplitK &z, float w0, float w1, float w2, float &er, float &ei)
+{
+  const float p = z.f_e0 * w0 + z.f_e1 * w1 + z.f_e2 * w2;
+  const float q0 = w0 - p * z.f_e0, q1 = w1 - p * z.f_e1, q2 = w2 - p * z.f_e2;
+  const float s = q0 * q0 + q1 * q1 + q2 * q2;
+  const float dp = z.f_D + p;
+  const float r = sqrtf(dp * dp + s);
+  const float den = dp + r, iden = 1.0f / den;
+  const float e1 = s * iden, Dl = p + e1, u = Dl * z.f_iD;
+  const float i1u = 1.0f / (1.0f + u);
+  const float U3 = u * u * u * i1u;
+  const float pD = p + Dl;
+  const float m = -s * pD * iden * 0.5f * z.f_iD;   // Delta - p - s/2D
+  const float h = 0.5f * e1 * pD;                  // (Delta^2 - p^2) / 2
+  float r3r, r3i;
+  if (REALK) {
+    // x = i t
+    const float t = z.f_kr * Dl, t2 = t * t;
+    float xr, xi;
+    if (fabsf(t) <= 1.0f) {
+      xr = t2 * t2 * (1.0f / 24 + t2 * (-1.0f / 720 + t2 * (1.0f / 40320 + t2 * (-1.0f / 3628800 + t2 * (1.0f / 479001600)))));
+      xi = t * t2 * (-1.0f / 6 + t2 * (1.0f / 120 + t2 * (-1.0f / 5040 + t2 * (1.0f / 362880 + t2 * (-1.0f / 39916800)))));
+    } else {
+      float sn, co;
+      sincosf(t, &sn, &co);
+      xr = (co - 1.0f) + 0.5f * t2;
+      xi = sn - t;
+    }
+    // (1 + x + x^2/2) = (1 - t^2/2) + i t;  x u^2 + x^2 u (u - 1)/2 = i t u^2 - t^2 u (u - 1)/2
+    r3r = xr * i1u - (1.0f - 0.5f * t2) * U3 - 0.5f * t2 * u * (u - 1.0f);
+    r3i = xi * i1u - t * U3 + t * u * u;
+  } else {
+    const float xr = -z.f_ki * Dl, xi = z.f_kr * Dl;
+    const float x2r = xr * xr - xi * xi, x2i = 2.0f * xr * xi;
+    float Xr, Xi;
+    if (xr * xr + xi * xi <= 1.0f) {
+      // x^3 (1/3! + x (1/4! + x (...))), Horner from 1/12!
+      const float c[10] = {1.0f / 6, 1.0f / 24, 1.0f / 120, 1.0f / 720, 1.0f / 5040, 1.0f / 40320,
+                           1.0f / 362880, 1.0f / 3628800, 1.0f / 39916800, 1.0f / 479001600};
+      float ar = c[9], ai = 0.0f;
+#pragma unroll
+      for (int n = 8; n >= 0; n--) {
+        const float tr = ar * xr - ai * xi + c[n];
+        ai = ar * xi + ai * xr;
+        ar = tr;
+      }
+      const float x3r = x2r * xr - x2i * xi, x3i = x2r * xi + x2i * xr;
+      Xr = ar * x3r - ai * x3i;
+      Xi = ar * x3i + ai * x3r;
+    } else {
+      float sn, co;
+      sincosf(xi, &sn, &co);
+      const float ex = expf(xr);
+      Xr = ex * co - 1.0f - xr - 0.5f * x2r;
+      Xi = ex * sn - xi - 0.5f * x2i;
+    }
+    const float ar = 1.0f + xr + 0.5f * x2r, ai = xi + 0.5f * x2i;
+    const float uu = u * u, hu = 0.5f * u * (u - 1.0f);
+    r3r = Xr * i1u - ar * U3 + xr * uu + x2r * hu;
+    r3i = Xi * i1u - ai * U3 + xi * uu + x2i * hu;
+  }
+  er = r3r + z.f_c1r * m + z.f_c2r * h;
+  ei = r3i + z.f_c1i * m + z.f_c2i * h;
+}
+
+// tp_moments with g from the split above
+template <bool REALK, typename SH>
+__device__ __forceinline__ void tp_moments_split(const SH &sh, int us, const double *cS, const double *ys,
+                                                 const double *cT, double kre, double kim, double *out, int stride)
+{
+  const SplitK z = split_setup<REALK>(cS, cT, kre, kim);
+  double m[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) m[i] = 0.0;
+#pragma unroll 1
+  for (int p = 0; p < 4; p++) {
+    double x[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) x[e] = sh.pts[0][4 * p + e][us];
+    const double xi0 = x[0] - cS[0], xi1 = x[1] - cS[1], xi2 = x[2] - cS[2];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const double *y = ys + 4 * q;
+      const double e0 = y[0] - cT[0], e1 = y[1] - cT[1], e2 = y[2] - cT[2];
+      const double w0 = xi0 - e0, w1 = xi1 - e1, w2 = xi2 - e2;
+      const double pp = z.e0 * w0 + z.e1 * w1 + z.e2 * w2;
+      const double ww = w0 * w0 + w1 * w1 + w2 * w2;
+      // T2 - 1 = c1 (p + s/2D) + c2 p^2/2
+      const double l = pp + (ww - pp * pp) * z.hiD, hp = 0.5 * pp * pp;
+      float fr, fi;
+      split_e3<REALK>(z, (float)w0, (float)w1, (float)w2, fr, fi);
+      const double vr = 1.0 + (z.c1r * l + z.c2r * hp) + (double)fr;
+      const double vi = (z.c1i * l + z.c2i * hp) + (double)fi;
+      const double amp = x[3] * y[3];
+      const double gr = amp * (z.g0r * vr - z.g0i * vi), gi = amp * (z.g0r * vi + z.g0i * vr);
+      const double xy = xi0 * e0 + xi1 * e1 + xi2 * e2;
+      m[0] += gr; m[1] += gi;
+      m[2] += gr * xi0; m[3] += gi * xi0; m[4] += gr * xi1; m[5] += gi * xi1; m[6] += gr * xi2; m[7] += gi * xi2;
+      m[8] += gr * e0; m[9] += gi * e0; m[10] += gr * e1; m[11] += gi * e1; m[12] += gr * e2; m[13] += gi * e2;
+      m[14] += gr * xy; m[15] += gi * xy;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 16; i++) out[i * stride] = m[i];
+}
+
+// one of the two moment routines
+template <bool REALK, bool SPLIT, typename SH>
+__device__ __forceinline__ void tp_moments_any(const SH &sh, int us, const double *cS, const double *ys,
+                                               const double *cT, double kre, double kim, double *out, int stride)
+{
+  if (SPLIT)
+    tp_moments_split<REALK>(sh, us, cS, ys, cT, kre, kim, out, stride);
+  else
+    tp_moments<REALK>(sh, us, cS, ys, cT, kre, kim, out, stride);
+}
+
 // points and centroid of distinct tetrahedron u of side `which` into registers
 template <typename SH>
 __device__ __forceinline__ void tp_load(const SH &sh, int which, int u, double *x, double *c)
@@ -706,7 +878,7 @@ __device__ __forceinline__ void tp_load(const SH &sh, int which, int u, double *
   for (int d = 0; d < 3; d++) c[d] = sh.cen[which][d][u];
 }
 
-template <bool REALK>
+template <bool REALK, bool SPLIT = false>
 __device__ void far_tp_tile(const DevGeom &g, double kre, double kim, cd *M, int64_t ld, int ti, int tj,
                             FarTpShared &sh)
 {
@@ -741,7 +913,7 @@ __device__ void far_tp_tile(const DevGeom &g, double kre, double kim, cd *M, int
       double xs[16], cS[3], ys[16], cT[3];
       tp_load(sh, 0, us, xs, cS);
       tp_load(sh, 1, ut, ys, cT);
-      tp_moments<REALK>(sh, us, cS, ys, cT, kre, kim, &sh.mom[0][pidx], FTT_MAX);
+      tp_moments_any<REALK, SPLIT>(sh, us, cS, ys, cT, kre, kim, &sh.mom[0][pidx], FTT_MAX);
     }
   }
   __syncthreads();
@@ -771,7 +943,7 @@ __device__ void far_tp_tile(const DevGeom &g, double kre, double kim, cd *M, int
           double xs[16], cS[3], ys[16], cT[3];
           tp_load(sh, 0, us, xs, cS);
           tp_load(sh, 1, ut, ys, cT);
-          tp_moments<REALK>(sh, us, cS, ys, cT, kre, kim, m, 1);
+          tp_moments_any<REALK, SPLIT>(sh, us, cS, ys, cT, kre, kim, m, 1);
         } else {
 #pragma unroll
           for (int i = 0; i < 16; i++) m[i] = sh.mom[i][us * nc + ut];
@@ -794,14 +966,14 @@ __device__ void far_tp_tile(const DevGeom &g, double kre, double kim, cd *M, int
   }
 }
 
-template <bool REALK>
+template <bool REALK, bool SPLIT>
 __global__ __launch_bounds__(FT_THREADS) void k_far_tp(DevGeom g, double kre, double kim, cd *M, int64_t ld,
                                                        int ntiles)
 {
   __shared__ FarTpShared sh;
   int ti, tj;
   tile_coords(blockIdx.x, ntiles, ti, tj);
-  far_tp_tile<REALK>(g, kre, kim, M, ld, ti, tj, sh);
+  far_tp_tile<REALK, SPLIT>(g, kre, kim, M, ld, ti, tj, sh);
 }
 
 // Two-pass far field.  A 32 x 32 BF tile evaluates ~31 x 31 tetrahedron
@@ -827,7 +999,7 @@ struct TpTileShared {
 };
 #define TPT_THREADS 256
 
-template <bool REALK>
+template <bool REALK, bool SPLIT>
 __global__ __launch_bounds__(TPT_THREADS) void k_tp_table(DevGeom g, double kre, double kim, double *tab, int ntb)
 {
   __shared__ TpTileShared sh;
@@ -861,7 +1033,7 @@ __global__ __launch_bounds__(TPT_THREADS) void k_tp_table(DevGeom g, double kre,
       double xs[16], cS[3], ys[16], cT[3];
       tp_load(sh, 0, us, xs, cS);
       tp_load(sh, 1, ut, ys, cT);
-      tp_moments<REALK>(sh, us, cS, ys, cT, kre, kim, m, 1);
+      tp_moments_any<REALK, SPLIT>(sh, us, cS, ys, cT, kre, kim, m, 1);
     }
     double2 *o = (double2 *)(tab + tp_index(S, T, nt) * 16);
 #pragma unroll
@@ -990,81 +1162,6 @@ __global__ __launch_bounds__(FT_THREADS) void k_far_gather(DevGeom g, double kre
       }
     }
     const cd v = {accr / FOURPI, acci / FOURPI};
-    M[(size_t)b * ld + a] = v;
-    M[(size_t)a * ld + b] = v;
-  }
-}
-
-// FP32 variant of the far-field tile (the precision form's far field):
-// identical rule and pairing, cubature arithmetic and accumulation in FP32,
-// the sum stored as complex128.
-template <bool REALK>
-__global__ __launch_bounds__(256) void k_far32(DevGeom g, double kre_d, double kim_d, cd *M, int64_t ld, int ntiles)
-{
-  __shared__ float cpts[TILE][8][7];
-  __shared__ int cverts[TILE][5];
-  int ti, tj;
-  tile_coords(blockIdx.x, ntiles, ti, tj);
-  const int tid = threadIdx.x, N = g.nbf;
-  for (int idx = tid; idx < TILE * 8; idx += blockDim.x) {
-    const int bl = idx >> 3, pt = idx & 7, b = tj * TILE + bl;
-    if (b < N) {
-      double o[7];
-      bf_points4(g, b, pt, o);
-      for (int c = 0; c < 7; c++) cpts[bl][pt][c] = (float)o[c];
-    }
-  }
-  for (int idx = tid; idx < TILE * 5; idx += blockDim.x) {
-    const int bl = idx / 5, b = tj * TILE + bl;
-    cverts[bl][idx % 5] = b < N ? g.bfV[5 * b + idx % 5] : -1;
-  }
-  __syncthreads();
-  const int a = ti * TILE + (tid & 63);
-  if (a >= N) return;
-  float rp[8][7];
-#pragma unroll
-  for (int pt = 0; pt < 8; pt++) {
-    double o[7];
-    bf_points4(g, a, pt, o);
-    for (int c = 0; c < 7; c++) rp[pt][c] = (float)o[c];
-  }
-  int va[5];
-#pragma unroll
-  for (int i = 0; i < 5; i++) va[i] = g.bfV[5 * a + i];
-  const float kre = (float)kre_d, kim = (float)kim_d;
-  const double k2re_d = kre_d * kre_d - kim_d * kim_d, k2im_d = 2.0 * kre_d * kim_d;
-  const double ik2d = k2re_d * k2re_d + k2im_d * k2im_d;
-  const float ik2re = (float)(k2re_d / ik2d), ik2im = (float)(-k2im_d / ik2d);
-  const int b0 = (tid >> 6) * 16;
-  for (int jj = 0; jj < 16; jj++) {
-    const int bl = b0 + jj, b = tj * TILE + bl;
-    if (b >= N || b < a) continue;
-    if (bf_ncv(va, cverts[bl]) > 0) continue;
-    float accr = 0.f, acci = 0.f;
-#pragma unroll 2
-    for (int ip = 0; ip < 8; ip++) {
-#pragma unroll
-      for (int jp = 0; jp < 8; jp++) {
-        const float *c = cpts[bl][jp];
-        const float R0 = rp[ip][0] - c[0], R1 = rp[ip][1] - c[1], R2 = rp[ip][2] - c[2];
-        const float r = sqrtf(R0 * R0 + R1 * R1 + R2 * R2);
-        const float inv = 1.0f / r;
-        const float dot = rp[ip][3] * c[3] + rp[ip][4] * c[4] + rp[ip][5] * c[5];
-        const float dd = rp[ip][6] * c[6];
-        float sn, co;
-        sincosf(kre * r, &sn, &co);
-        if (REALK) {
-          const float pe = (dot - dd * ik2re) * inv;
-          accr += pe * co; acci += pe * sn;
-        } else {
-          const float amp = expf(-kim * r) * inv;
-          const float per = dot - dd * ik2re, pei = -dd * ik2im;
-          const float er = amp * co, ei = amp * sn;
-          accr += per * er - pei * ei; acci += per * ei + pei * er;
-        }
-      }
-    }
-    const cd v = {(double)accr / FOURPI, (double)acci / FOURPI};
     M[(size_t)b * ld + a] = v;
     M[(size_t)a * ld + b] = v;
   }
@@ -1587,35 +1684,32 @@ static int assemble_vie(bfa_geometry *g, double omega_re, double omega_im, const
   if (omega_re == 0.0 && omega_im == 0.0) return fail(BFA_ERR_ARG, "bfa_assemble_vie: omega = 0");
   hipStream_t st = (hipStream_t)stream;
   const int N = g->nbf;
-  const int ntiles = (N + TILE - 1) / TILE;
-  const int64_t nblk = (int64_t)ntiles * (ntiles + 1) / 2;
   cd *M = (cd *)d_M;
-  if (far32) {
-    if (omega_im == 0.0)
-      hipLaunchKernelGGL(k_far32<true>, dim3((unsigned)nblk), dim3(256), 0, st, g->dg, omega_re, 0.0, M, ld, ntiles);
-    else
-      hipLaunchKernelGGL(k_far32<false>, dim3((unsigned)nblk), dim3(256), 0, st, g->dg, omega_re, omega_im, M, ld,
-                         ntiles);
+  const int nt2 = (N + FT2 - 1) / FT2;
+  const int64_t nblk2 = (int64_t)nt2 * (nt2 + 1) / 2;
+  const bool rk = omega_im == 0.0;
+  double *tpt = far_table(g);
+  if (tpt) {
+    const int ntb = (g->nt + 31) / 32;
+    const dim3 gr1((unsigned)((int64_t)ntb * (ntb + 1) / 2)), bl1(TPT_THREADS);
+    if (far32) {
+      if (rk) hipLaunchKernelGGL((k_tp_table<true, true>), gr1, bl1, 0, st, g->dg, omega_re, 0.0, tpt, ntb);
+      else hipLaunchKernelGGL((k_tp_table<false, true>), gr1, bl1, 0, st, g->dg, omega_re, omega_im, tpt, ntb);
+    } else {
+      if (rk) hipLaunchKernelGGL((k_tp_table<true, false>), gr1, bl1, 0, st, g->dg, omega_re, 0.0, tpt, ntb);
+      else hipLaunchKernelGGL((k_tp_table<false, false>), gr1, bl1, 0, st, g->dg, omega_re, omega_im, tpt, ntb);
+    }
+    hipLaunchKernelGGL(k_far_gather, dim3((unsigned)nblk2), dim3(FT_THREADS), 0, st, g->dg, omega_re, omega_im,
+                       (const double *)tpt, M, ld, nt2);
   } else {
-    const int nt2 = (N + FT2 - 1) / FT2;
-    const int64_t nblk2 = (int64_t)nt2 * (nt2 + 1) / 2;
-    double *tpt = far_table(g);
-    if (tpt) {
-      const int ntb = (g->nt + 31) / 32;
-      const int64_t nblk1 = (int64_t)ntb * (ntb + 1) / 2;
-      if (omega_im == 0.0)
-        hipLaunchKernelGGL(k_tp_table<true>, dim3((unsigned)nblk1), dim3(TPT_THREADS), 0, st, g->dg, omega_re, 0.0,
-                           tpt, ntb);
-      else
-        hipLaunchKernelGGL(k_tp_table<false>, dim3((unsigned)nblk1), dim3(TPT_THREADS), 0, st, g->dg, omega_re,
-                           omega_im, tpt, ntb);
-      hipLaunchKernelGGL(k_far_gather, dim3((unsigned)nblk2), dim3(FT_THREADS), 0, st, g->dg, omega_re, omega_im,
-                         (const double *)tpt, M, ld, nt2);
-    } else if (omega_im == 0.0)
-      hipLaunchKernelGGL(k_far_tp<true>, dim3((unsigned)nblk2), dim3(FT_THREADS), 0, st, g->dg, omega_re, 0.0, M, ld, nt2);
-    else
-      hipLaunchKernelGGL(k_far_tp<false>, dim3((unsigned)nblk2), dim3(FT_THREADS), 0, st, g->dg, omega_re, omega_im, M, ld,
-                         nt2);
+    const dim3 gr2((unsigned)nblk2), bl2(FT_THREADS);
+    if (far32) {
+      if (rk) hipLaunchKernelGGL((k_far_tp<true, true>), gr2, bl2, 0, st, g->dg, omega_re, 0.0, M, ld, nt2);
+      else hipLaunchKernelGGL((k_far_tp<false, true>), gr2, bl2, 0, st, g->dg, omega_re, omega_im, M, ld, nt2);
+    } else {
+      if (rk) hipLaunchKernelGGL((k_far_tp<true, false>), gr2, bl2, 0, st, g->dg, omega_re, 0.0, M, ld, nt2);
+      else hipLaunchKernelGGL((k_far_tp<false, false>), gr2, bl2, 0, st, g->dg, omega_re, omega_im, M, ld, nt2);
+    }
   }
   const int64_t nnz = g->dg.nnz;
   hipLaunchKernelGGL(k_near, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st, g->dg, omega_re, omega_im,

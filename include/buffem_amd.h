@@ -180,6 +180,12 @@ int  bfa_dressed_rytov(bfa_geometry *g, double omega, const bfa_material *eps,
  * sum over a <= b of Re(Z0 DR[b,a]) Im Q_ab.  Host output.                 */
 int  bfa_emt_power_dr(bfa_geometry *g, double omega, const void *d_DR, int64_t ld, double *out,
                       void *stream);
+/* bfa_assemble_vie with the far-field kernel evaluated as the precision
+ * form's split (AssembleVIEMatrix, VIEMatrix.cc:431-460, with the fixed rule
+ * of GMatrixElements.cc:270-288): about each tetrahedron pair's centroid
+ * separation, the second-order Taylor part of e^{ikr}/r in FP64 and the
+ * remainder in FP32; the matrix moves by ~1e-9 relative from the FP64 rule
+ * (powers within ~1e-9).                                                    */
 int  bfa_assemble_vie_far32(bfa_geometry *g, double omega_re, double omega_im,
                             const bfa_material *mat, void *d_M, int64_t ld, void *stream);
 int  bfa_assemble_rhs(bfa_geometry *g, double omega, const bfa_planewave *pw,

@@ -199,21 +199,27 @@ def test_cpp_driver_matches_python_path(bfa, orc, torch_cuda):
         assert np.all(np.abs(got - P) / np.abs(P) < 1e-8), (got, P)
 
 
+_FAR_CASES = ((1.7, False), (1.7, True), (1.7 + 0.2j, False), (1.7 + 0.2j, True))
+
 _SINGLE_PASS_CHILD = r"""
 import os, sys
 import numpy as np
 import torch
 sys.path.insert(0, os.path.join(sys.argv[1], "tests"))
 from conftest import load_bfa, kuhn
+from test_gpu_parity import _FAR_CASES
 bfa = load_bfa()
 V, T = kuhn(bfa, 6)
 g = bfa.Geometry(V, T)
 g.upload()
 g.build_static(10)
 N = g.N
-dM = torch.zeros((N, N), dtype=torch.complex128, device="cuda")
-g.assemble_vie(1.7, bfa.material("diag", eps=(4 + 0.05j, 5 + 0.05j, 6 + 0.05j)), dM)
-np.save(sys.argv[2], dM.cpu().numpy())
+out = {}
+for i, (omega, far32) in enumerate(_FAR_CASES):
+    dM = torch.zeros((N, N), dtype=torch.complex128, device="cuda")
+    g.assemble_vie(omega, bfa.material("diag", eps=(4 + 0.05j, 5 + 0.05j, 6 + 0.05j)), dM, far32=far32)
+    out[f"m{i}"] = dM.cpu().numpy()
+np.savez(sys.argv[2], **out)
 """
 
 
@@ -222,25 +228,36 @@ def test_far_field_table_equals_single_pass(bfa, torch_cuda, tmp_path):
     table, then the BF tiles gathering them) against the single-pass
     tetrahedron-pair kernel, which a child process selects with
     BFA_FAR_TABLE=0 (the switch is read once per process): the same sums in
-    another order, n = 6 sphere (N = 2,376), complex128 to 1e-14 relative."""
+    another order, n = 6 sphere (N = 2,376), complex128 to 1e-14 relative,
+    for the FP64 kernel and the precision form's split kernel, at a real and
+    a complex frequency.  The split kernel's matrix is also held to the FP64
+    one: its FP32 remainder moves the whole matrix by ~1e-9 relative."""
     import os
     import subprocess
     import sys
     torch = torch_cuda
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-    out = str(tmp_path / "single.npy")
+    out = str(tmp_path / "single.npz")
     r = subprocess.run([sys.executable, "-c", _SINGLE_PASS_CHILD, root, out], capture_output=True, text=True,
                        env=dict(os.environ, BFA_FAR_TABLE="0"), timeout=240)
     print(r.stdout, r.stderr[-2000:], flush=True)
     assert r.returncode == 0
+    single = np.load(out)
     V, T = kuhn(bfa, 6)
     g = bfa.Geometry(V, T)
     g.upload()
     g.build_static(10)
     N = g.N
-    dM = torch.zeros((N, N), dtype=torch.complex128, device="cuda")
-    g.assemble_vie(1.7, bfa.material("diag", eps=(4 + 0.05j, 5 + 0.05j, 6 + 0.05j)), dM)
-    A, B = dM.cpu().numpy(), np.load(out)
-    err = np.linalg.norm(A - B) / np.linalg.norm(B)
-    print(f"[far table] N={N}: two-pass vs single-pass {err:.2e}", flush=True)
-    assert err < 1e-14
+    two = []
+    for i, (omega, far32) in enumerate(_FAR_CASES):
+        dM = torch.zeros((N, N), dtype=torch.complex128, device="cuda")
+        g.assemble_vie(omega, bfa.material("diag", eps=(4 + 0.05j, 5 + 0.05j, 6 + 0.05j)), dM, far32=far32)
+        A, B = dM.cpu().numpy(), single[f"m{i}"]
+        err = np.linalg.norm(A - B) / np.linalg.norm(B)
+        print(f"[far table] N={N} omega={omega} far32={far32}: two-pass vs single-pass {err:.2e}", flush=True)
+        assert err < 1e-14
+        two.append(A)
+    for i in (0, 2):
+        e = np.linalg.norm(two[i + 1] - two[i]) / np.linalg.norm(two[i])
+        print(f"[far table] omega={_FAR_CASES[i][0]}: split (FP32 remainder) vs FP64 {e:.2e}", flush=True)
+        assert e < 1e-8
