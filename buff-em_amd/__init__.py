@@ -126,6 +126,7 @@ _SIGS = {
     "bfa_surr_train": (I, [P, I, I, D, P]),
     "bfa_surr_table": (I, [P, D, I, D, I, I, P, P]),
     "bfa_surr_table_download": (I, [P, P]),
+    "bfa_surr_table_upload": (I, [P, P]),
     "bfa_assemble_vie_table": (I, [P, D, P, P, I64, P]),
     "bfa_surr_info": (I, [P, P, P, P, P]),
     "bfa_surr_params": (I, [P, P, P]),
@@ -656,6 +657,7 @@ class Surrogate:
     one geometry closer than `factor` diameters (4x4 complex blocks)."""
 
     RULE, CUBATURE, SURROGATE = 0, 1, 2
+    DEFAULT_TOL = 0.3
 
     def __init__(self, geom, factor=3.0, ensemble=4, hidden=256, seed=1):
         self.geom = geom
@@ -705,8 +707,12 @@ class Surrogate:
         _check(lib().bfa_surr_train(self._h, int(epochs), int(batch), float(lr), hist.ctypes.data), "bfa_surr_train")
         return hist
 
-    def table(self, omega, mode=2, tol=1e-7, npts=33, level=1):
-        """Build the block table at omega; returns (cubature fallbacks, max ensemble disagreement)."""
+    def table(self, omega, mode=2, tol=None, npts=33, level=1):
+        """Build the block table at omega; returns (cubature fallbacks, max ensemble disagreement).
+        tol: largest accepted disagreement of the members, in units of
+        ||R16 - R4|| (default DEFAULT_TOL)."""
+        if tol is None:
+            tol = self.DEFAULT_TOL
         nf, dm = I(), D()
         _check(lib().bfa_surr_table(self._h, float(omega), int(mode), float(tol), int(npts), int(level),
                                     ctypes.byref(nf), ctypes.byref(dm)), "bfa_surr_table")
@@ -716,6 +722,12 @@ class Surrogate:
         out = np.zeros((self.npairs, 32))
         _check(lib().bfa_surr_table_download(self._h, out.ctypes.data), "bfa_surr_table_download")
         return (out[:, 0::2] + 1j * out[:, 1::2]).reshape(-1, 4, 4)
+
+    def table_upload(self, blocks):
+        """blocks: complex (npairs, 4, 4), the layout of table_download"""
+        b = np.asarray(blocks, np.complex128).reshape(self.npairs, 16)
+        out = np.ascontiguousarray(np.stack([b.real, b.imag], axis=-1).reshape(self.npairs, 32))
+        _check(lib().bfa_surr_table_upload(self._h, out.ctypes.data), "bfa_surr_table_upload")
 
     def assemble(self, omega, mat, d_M, ld=None, stream=None):
         _check(lib().bfa_assemble_vie_table(self._h, float(omega), ctypes.byref(mat), _ptr(d_M),

@@ -369,8 +369,10 @@ __global__ void k_gather(const _Float16 *src16, int kp, const float *srcT, const
 }
 
 // Ensemble combination: block = R16 + s * mean_e(y_e) (s = ||R16 - R4||);
-// disagreement = max_e s ||y_e - mean|| / ||block||; flag if > tol or an
-// input lies outside the training range [lo, hi].
+// disagreement = max_e ||y_e - mean||, the members' spread about the
+// correction in units of s (the scale of the residual being learned, so
+// that the tolerance means the same for close and distant pairs); flag if
+// > tol or an input lies outside the training range [lo, hi].
 __global__ void k_combine(const float *Y, int E, int n, const float *ymu, const float *ysd, const double *r4,
                           const double *r16, const float *feat, const float *flo, const float *fhi, double tol,
                           double *blocks, float *disagree, int *flag)
@@ -385,13 +387,7 @@ __global__ void k_combine(const float *Y, int E, int n, const float *ymu, const 
     for (int e = 0; e < E; e++) s += (double)Y[((size_t)e * n + i) * NOUT + k] * ysd[k] + ymu[k];
     mean[k] = s / E;
   }
-  double nb = 0.0;
-  for (int k = 0; k < NOUT; k++) {
-    const double v = b[k] + sc * mean[k];
-    blocks[(size_t)i * NOUT + k] = v;
-    nb += v * v;
-  }
-  nb = sqrt(nb);
+  for (int k = 0; k < NOUT; k++) blocks[(size_t)i * NOUT + k] = b[k] + sc * mean[k];
   double worst = 0.0;
   for (int e = 0; e < E; e++) {
     double d2 = 0.0;
@@ -399,7 +395,7 @@ __global__ void k_combine(const float *Y, int E, int n, const float *ymu, const 
       const double d = (double)Y[((size_t)e * n + i) * NOUT + k] * ysd[k] + ymu[k] - mean[k];
       d2 += d * d;
     }
-    worst = fmax(worst, sc * sqrt(d2) / nb);
+    worst = fmax(worst, sqrt(d2));
   }
   bool ood = false;
   for (int k = 0; k < NFEAT; k++) {
@@ -1045,6 +1041,15 @@ extern "C" int bfa_surr_table_download(const bfa_surrogate *S, double *blocks)
 {
   if (!S || !blocks) return fail(BFA_ERR_ARG, "bfa_surr_table_download: bad argument");
   HIPCHECK(hipMemcpy(blocks, S->d_blocks, (size_t)S->npairs * NOUT * 8, hipMemcpyDeviceToHost));
+  return BFA_OK;
+}
+
+// a block table from the host (e.g. one computed earlier, or edited), in the
+// layout of bfa_surr_table_download
+extern "C" int bfa_surr_table_upload(bfa_surrogate *S, const double *blocks)
+{
+  if (!S || !blocks) return fail(BFA_ERR_ARG, "bfa_surr_table_upload: bad argument");
+  HIPCHECK(hipMemcpy(S->d_blocks, blocks, (size_t)S->npairs * NOUT * 8, hipMemcpyHostToDevice));
   return BFA_OK;
 }
 

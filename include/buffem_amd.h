@@ -358,10 +358,15 @@ int  bfa_surr_dataset(bfa_surrogate *s, int per_pair, double wmin, double wmax, 
 int  bfa_surr_train(bfa_surrogate *s, int epochs, int batch, double lr, double *loss_hist);
 /* Block table at omega: mode 0 4-point rule, 1 cubature (npts, level),
  * 2 surrogate with cubature (npts, level) where the ensemble disagreement
- * exceeds tol or an input lies outside the training range.               */
+ * exceeds tol or an input lies outside the training range.  Disagreement:
+ * the largest member's distance from the ensemble mean, in units of
+ * ||R16 - R4||, the size of the residual the network corrects.            */
 int  bfa_surr_table(bfa_surrogate *s, double omega, int mode, double tol, int npts, int level,
                     int *nfallback, double *max_disagree);
 int  bfa_surr_table_download(const bfa_surrogate *s, double *blocks /* npairs*32 */);
+/* The inverse: a table in that layout from the host (no reference
+ * counterpart: lets a caller reuse or edit a table).                        */
+int  bfa_surr_table_upload(bfa_surrogate *s, const double *blocks /* npairs*32 */);
 /* bfa_assemble_vie with the far field read from the block table.           */
 int  bfa_assemble_vie_table(bfa_surrogate *s, double omega, const bfa_material *mat, void *d_M,
                             int64_t ld, void *stream);

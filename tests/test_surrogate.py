@@ -150,44 +150,55 @@ def _power(bfa, torch, g, mat, w, pw, assemble):
 
 @pytest.mark.gpu
 def test_surrogate_gain(bfa, n5, torch_cuda):
-    """Trained surrogate versus the reference's 4-point rule, judged by finer
-    cubature at an unseen frequency."""
+    """Trained surrogate (16-point rule plus the network's correction) versus
+    the 16-point rule and the reference's 4-point rule, judged by finer
+    cubature (33-point rule on tetrahedra refined twice): block errors over
+    every pair at omega = 1, and absorbed and scattered powers for 32 plane
+    waves at omega = 0.5, 1, 2, 4 (inside the training range, none of them a
+    training frequency)."""
     torch = torch_cuda
     V, T, g, S = n5
     import time
+    from test_configs import directions
     t0 = time.time()
-    S.dataset(per_pair=3, wmin=0.1, wmax=10.0, label_npts=33, label_level=1)
+    S.dataset(per_pair=8, wmin=0.1, wmax=10.0, label_npts=33, label_level=1)   # ~2M samples
     t1 = time.time()
     hist = S.train(epochs=60, batch=4096, lr=2e-3)
     t2 = time.time()
     print(f"[surrogate] dataset {t1 - t0:.1f}s train {t2 - t1:.1f}s loss {hist[0].mean():.3e} -> {hist[-1].mean():.3e}",
           flush=True)
     assert hist[-1].mean() < hist[0].mean()
+    n = S.npairs
+    nrm = lambda X: np.linalg.norm(X.reshape(len(X), -1), axis=1)
     w = 1.0
-    rng = np.random.default_rng(8)
-    sub = rng.choice(S.npairs, 3000, replace=False)
-    truth = S.cubature(sub, w, 33, 2)
+    truth = S.cubature(np.arange(n), w, 33, 2)
     S.table(w, mode=S.RULE)
-    rule = S.table_download()[sub]
+    rule = S.table_download()
     S.table(w, mode=S.CUBATURE, npts=16, level=0)
-    r16 = S.table_download()[sub]
+    r16 = S.table_download()
     nf, dmax = S.table(w, mode=S.SURROGATE, tol=np.inf)
-    surr = S.table_download()[sub]
-    err = lambda X: np.linalg.norm((X - truth).reshape(len(sub), -1), axis=1) / \
-        np.linalg.norm(truth.reshape(len(sub), -1), axis=1)
-    e_rule, e_16, e_surr = err(rule), err(r16), err(surr)
-    print(f"[surrogate] block error vs cubature(33,L2) at omega=1 (median / p90): 4-point rule "
+    surr = S.table_download()
+    e_rule, e_16, e_surr = (nrm(X - truth) / nrm(truth) for X in (rule, r16, surr))
+    print(f"[surrogate] block error vs cubature(33,L2) at omega=1 over {n} pairs (median / p90): 4-point rule "
           f"{np.median(e_rule):.2e} / {np.quantile(e_rule, .9):.2e}; 16-point rule {np.median(e_16):.2e} / "
           f"{np.quantile(e_16, .9):.2e}; surrogate {np.median(e_surr):.2e} / {np.quantile(e_surr, .9):.2e}; "
-          f"max ensemble disagreement {dmax:.2e}", flush=True)
+          f"surrogate closer on {np.mean(e_surr < e_16):.2f} of the pairs", flush=True)
     # what the network adds over the rule it corrects, and over the reference's rule
-    assert np.median(e_surr) < 0.8 * np.median(e_16)
-    assert np.quantile(e_surr, 0.9) < 0.95 * np.quantile(e_16, 0.9)
-    assert np.median(e_surr) < 0.02 * np.median(e_rule)
-    # fallback at the default disagreement tolerance: every flagged pair is
-    # recomputed by cubature, so the table is at least as good as the rule
-    nf7, _ = S.table(w, mode=S.SURROGATE, tol=1e-7)
-    print(f"[surrogate] cubature fallbacks at tol 1e-7: {nf7}/{S.npairs}", flush=True)
+    assert np.median(e_surr) < 0.5 * np.median(e_16)
+    assert np.quantile(e_surr, 0.9) < 0.6 * np.quantile(e_16, 0.9)
+    assert np.median(e_surr) < 0.01 * np.median(e_rule)
+    # cubature fallbacks by ensemble disagreement (in units of ||R16 - R4||)
+    counts = {}
+    for tol in (1.0, 0.3, 0.1, 0.03):
+        counts[tol] = S.table(w, mode=S.SURROGATE, tol=tol)[0]
+        if tol == S.DEFAULT_TOL:
+            fb = S.table_download()
+            e_fb = nrm(fb - truth) / nrm(truth)
+    print("[surrogate] cubature fallbacks by disagreement tolerance: " +
+          ", ".join(f"{t}: {c}/{n}" for t, c in counts.items()) +
+          f"; block error with fallback at {S.DEFAULT_TOL} (median / p90) {np.median(e_fb):.2e} / "
+          f"{np.quantile(e_fb, .9):.2e}", flush=True)
+    assert np.quantile(e_fb, 0.9) <= np.quantile(e_surr, 0.9)
     # wall time of each way to fill the table of all pairs (second call of
     # each, after a warm one), on the device
     def timed(**kw):
@@ -200,20 +211,32 @@ def test_surrogate_gain(bfa, n5, torch_cuda):
     tt = {"4-point rule": timed(mode=S.RULE)[0], "16-point rule": timed(mode=S.CUBATURE, npts=16, level=0)[0],
           "cubature 33 x refined once (labels)": timed(mode=S.CUBATURE, npts=33, level=1)[0],
           "surrogate, no fallback": timed(mode=S.SURROGATE, tol=np.inf)[0],
-          "surrogate, fallback at tol 1e-7": timed(mode=S.SURROGATE, tol=1e-7)[0]}
-    print("[surrogate] table time for " + str(S.npairs) + " pairs: " +
+          f"surrogate, fallback at {S.DEFAULT_TOL}": timed(mode=S.SURROGATE, tol=S.DEFAULT_TOL)[0]}
+    print("[surrogate] table time for " + str(n) + " pairs: " +
           ", ".join(f"{k} {v * 1e3:.1f} ms" for k, v in tt.items()), flush=True)
-    # solved powers: reference rule vs surrogate vs cubature for every pair
+    # solved powers against those of the all-cubature far field
     mat = bfa.material("iso", eps=4 + 0.1j)
-    pw = bfa.planewaves([((1, 0, 0), (0, 0, 1))])
-    P_rule = _power(bfa, torch, g, mat, w, pw, lambda M: g.assemble_vie(w, mat, M))
-    S.table(w, mode=S.CUBATURE, npts=33, level=1)
-    P_cub = _power(bfa, torch, g, mat, w, pw, lambda M: S.assemble(w, mat, M))
-    S.table(w, mode=S.CUBATURE, npts=16, level=0)
-    P_16 = _power(bfa, torch, g, mat, w, pw, lambda M: S.assemble(w, mat, M))
-    S.table(w, mode=S.SURROGATE, tol=np.inf)
-    P_surr = _power(bfa, torch, g, mat, w, pw, lambda M: S.assemble(w, mat, M))
-    d_rule, d_16, d_surr = (np.abs(P - P_cub) / np.abs(P_cub) for P in (P_rule, P_16, P_surr))
-    print(f"[surrogate] power vs all-cubature far field: 4-point rule {d_rule.ravel()}, 16-point rule "
-          f"{d_16.ravel()}, surrogate {d_surr.ravel()}", flush=True)
-    assert d_surr.max() < d_rule.max()
+    pw = bfa.planewaves(directions(16, 0xC0F3))
+    errs = {k: [] for k in ("4-point rule", "16-point rule", "surrogate", "surrogate + fallback")}
+    for w in (0.5, 1.0, 2.0, 4.0):
+        def power(tab):
+            S.table_upload(tab)
+            return _power(bfa, torch, g, mat, w, pw, lambda M: S.assemble(w, mat, M))
+        P_t = power(S.cubature(np.arange(n), w, 33, 2))
+        tabs = {}
+        for k, kw in (("4-point rule", dict(mode=S.RULE)), ("16-point rule", dict(mode=S.CUBATURE, npts=16, level=0)),
+                      ("surrogate", dict(mode=S.SURROGATE, tol=np.inf)),
+                      ("surrogate + fallback", dict(mode=S.SURROGATE, tol=S.DEFAULT_TOL))):
+            S.table(w, **kw)
+            tabs[k] = S.table_download()
+        for k, tab in tabs.items():
+            errs[k].append((np.abs(power(tab) - P_t) / np.abs(P_t)).ravel())
+    errs = {k: np.concatenate(v) for k, v in errs.items()}
+    print("[surrogate] power vs all-cubature far field, 32 waves x 4 frequencies x (abs, scat), "
+          "median / p90 / max: " + "; ".join(f"{k} {np.median(v):.2e} / {np.quantile(v, .9):.2e} / {v.max():.2e}"
+                                              for k, v in errs.items()) +
+          f"; surrogate closer than the 16-point rule on {np.mean(errs['surrogate'] < errs['16-point rule']):.2f}",
+          flush=True)
+    assert np.quantile(errs["surrogate"], 0.9) < 0.7 * np.quantile(errs["16-point rule"], 0.9)
+    assert errs["surrogate"].max() < errs["16-point rule"].max()
+    assert np.quantile(errs["surrogate"], 0.9) < 0.1 * np.quantile(errs["4-point rule"], 0.9)
