@@ -16,12 +16,17 @@ otherwise share hardware queues with another's (measured: the fast form's
 factorisation took 132 instead of 104 ms when its streams were created
 after the partitioned LU's CU-masked ones).
 
+The default line also carries, under 'batch', the batching form's sweep
+(48 uneven spheres x 8 frequencies, --workload batch) run in a child
+process after the forms.
+
 Prints one JSON line (metric: seconds per frequency, lower is better).
 """
 import argparse
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -119,6 +124,9 @@ def bench_graph_form(args, bfa, g, mat, pw, omega, log):
 
 
 BATCH_OMEGAS = np.logspace(-1, 1, 8)
+
+
+DEFAULT_FORMS = "fp64,mixed,fast,graph"
 
 
 def batch_workload(bfa, count=48, td_order=10, seed=0xBA7C4):
@@ -340,7 +348,6 @@ def run_forms_in_children(args, forms, log):
     other; each writes its timings, powers and solution to a file read back
     here.  Every form's library streams and their hardware queues then start
     from the same state."""
-    import subprocess
     import tempfile
     results, N, nt, t_cold = {}, None, None, None
     with tempfile.TemporaryDirectory() as tmp:
@@ -378,13 +385,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=12, help="subcubes per axis (12 = the N=19,872 workload)")
     ap.add_argument("--td-order", type=int, default=10)
-    ap.add_argument("--forms", default="fp64,mixed,fast,graph",
+    ap.add_argument("--forms", default=DEFAULT_FORMS,
                     help="comma list of fp64, mixed (complex64 partial-pivoting LU + FP64 refinement), fast "
                          "(block-local pivoting, split-FP16 trailing updates + FP64 refinement), graph (mixed "
                          "in grouped kernels replayed from a hipGraph)")
     ap.add_argument("--workload", default="single", choices=["single", "batch"],
                     help="single: one N=19,872 sphere; batch: 48 uneven spheres x 8 frequencies")
     ap.add_argument("--batch-count", type=int, default=48)
+    ap.add_argument("--no-batch", action="store_true",
+                    help="leave out the batched sweep that the default line (all forms, n = 12) carries as its "
+                         "'batch' key")
     ap.add_argument("--refine-tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -490,7 +500,29 @@ def main():
         "forms": forms_out, "cold_static_s": t_cold,
         "roofline": roofline, "cpu_baseline": cpu,
     }
+    if not args.no_batch and args.n == 12 and args.forms == DEFAULT_FORMS:
+        out["batch"] = batch_in_child(args, log)
     print(json.dumps(out))
+
+
+def batch_in_child(args, log):
+    """The batching form's figures for the default line: `--workload batch`
+    (48 uneven spheres x 8 frequencies, one timed sweep after a warm one) in
+    a process of its own; its whole line is kept under 'line'."""
+    cmd = [sys.executable, os.path.abspath(__file__), "--workload", "batch", "--steps", "1", "--warmup", "1",
+           "--td-order", str(args.td_order)]
+    log("batch: " + " ".join(cmd[1:]))
+    try:
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=900)
+    except subprocess.TimeoutExpired:
+        return {"error": "timed out after 900 s"}
+    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"error": f"exit status {r.returncode}"}
+    b = json.loads(lines[-1])
+    return {"metric": b["metric"], "s_per_freq": b["value"], "fp64_loop_s_per_freq": b["fp64_loop_s_per_freq"],
+            "speedup_vs_fp64_loop": b["speedup_vs_fp64_loop"],
+            "power_rel_err_vs_fp64_max": b["power_rel_err_vs_fp64_max"], "line": b}
 
 
 def fast_roofline(rk, N, steps):
