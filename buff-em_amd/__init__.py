@@ -111,6 +111,7 @@ _SIGS = {
     "bfa_batch_info": (I, [P, P, P, P, P]),
     "bfa_batch_tune": (I, [P, I]),
     "bfa_batch_tuning": (I, [P, P, P]),
+    "bfa_batch_set_tiles": (I, [P, P]),
     "bfa_batch_run": (I, [P, P, I, I, D, I, P, P, P, P]),
     "bfa_batch_timing": (I, [P, P]),
     "bfa_batch_solution": (I, [P, I, P]),
@@ -575,9 +576,14 @@ class Batch:
     def tune(self, reps=3):
         _check(lib().bfa_batch_tune(self._h, int(reps)), "bfa_batch_tune")
         tile = np.zeros(16, np.int32)
-        ms = np.zeros((16, 2))
+        ms = np.zeros((16, 3))
         _check(lib().bfa_batch_tuning(self._h, tile.ctypes.data, ms.ctypes.data), "bfa_batch_tuning")
         return tile, ms
+
+    def set_tiles(self, tile):
+        """tile variant per size bucket (16 values, -1 = the fixed rule)"""
+        t = np.ascontiguousarray(np.broadcast_to(np.asarray(tile, np.int32), (16,)), np.int32)
+        _check(lib().bfa_batch_set_tiles(self._h, t.ctypes.data), "bfa_batch_set_tiles")
 
     def run(self, omegas, use_graph=True, tol=1e-10, maxit=6, with_power=True):
         """Returns dict with power [nf, count, npw, 2] (None without the EMT pass),

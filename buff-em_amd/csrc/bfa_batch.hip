@@ -286,6 +286,9 @@ struct bfa_batch {
   PWBatch pws{};
   // large buffers, one allocation per instance
   std::vector<void *> M64, LU32;
+  // split-FP16 planes of each instance's trailing update (allocated when a
+  // tuned bucket first uses them): A's planes, B's, row and column factors
+  std::vector<void *> HS;
   // everything else in one allocation; per-instance byte offsets
   void *small = nullptr;
   struct Off { size_t ipiv, perm, B, X, R, D, Y, Xs, inv, tick, P, norms, part, sdata, sj; };
@@ -373,6 +376,58 @@ int tile_for(const bfa_batch *B, int m, int n)
   return (m >= 256 && n >= 256) ? TILE_128 : TILE_64;
 }
 
+// the split-FP16 option of the trailing update: plane rows for an m-row
+// update, the per-instance plane buffer (for Np - NB rows at most) and the
+// three grouped launches' descriptors of one step
+int64_t hs_rows(int m) { return ((int64_t)m + 127) / 128 * 128; }
+size_t hs_bytes(int Np) { return Np > NB ? (size_t)hs_rows(Np - NB) * (4 * 2 * NB * 2 + 8) : 0; }
+
+int hs_ensure(bfa_batch *B)
+{
+  if (B->HS.size() != (size_t)B->count) B->HS.assign(B->count, nullptr);
+  for (int i = 0; i < B->count; i++) {
+    if (B->HS[i] || hs_bytes(B->Np[i]) == 0) continue;
+    HIPCHECK(hipMalloc(&B->HS[i], hs_bytes(B->Np[i])));
+  }
+  return BFA_OK;
+}
+
+struct HsStep {
+  std::vector<HsSplitProb> rows, cols;
+  std::vector<HsGemmProb> gemm;
+  int brows = 0, bcols = 0, bgemm = 0;   // workgroups of each launch
+  // instance i's update C -= A B with A = L21 (m x NB), B = U12 (NB x n)
+  void add(const bfa_batch *B, int i, const GemmProb &g)
+  {
+    const int64_t pr = hs_rows(B->Np[i] - NB), mp = hs_rows(g.m), np = hs_rows(g.n);
+    char *base = (char *)B->HS[i];
+    void *PA = base, *PB = base + (size_t)pr * 4 * NB * 2;
+    void *rf = base + (size_t)pr * 8 * NB * 2, *cf = (char *)rf + (size_t)pr * 4;
+    HsSplitProb r{};
+    r.X = g.A; r.ldx = g.lda; r.P = PA; r.fac = rf; r.pr = mp; r.nv = g.m; r.k = NB; r.kp = NB; r.blk0 = brows;
+    rows.push_back(r);
+    brows += (int)(mp / 32);
+    HsSplitProb c{};
+    c.X = g.B; c.ldx = g.ldb; c.P = PB; c.fac = cf; c.pr = np; c.nv = g.n; c.k = NB; c.kp = NB; c.blk0 = bcols;
+    cols.push_back(c);
+    bcols += (int)(np / 32);
+    HsGemmProb q{};
+    q.PA = PA; q.PB = PB; q.rf = rf; q.cf = cf; q.C = g.C; q.mp = mp; q.np = np; q.ldc = g.ldc;
+    q.m = g.m; q.n = g.n; q.kp = NB; q.tiles_m = (int)(mp / 128); q.tiles_n = (int)(np / 128);
+    int G = 1, nb = 0;
+    hs_grid(g.m, g.n, &G, &nb);
+    q.G = G; q.blk0 = bgemm;
+    gemm.push_back(q);
+    bgemm += nb;
+  }
+  void launch(const HsSplitProb *dr, const HsSplitProb *dc, const HsGemmProb *dg, hipStream_t st) const
+  {
+    launch_split_g(false, dr, (int)rows.size(), brows, st);
+    launch_split_g(true, dc, (int)cols.size(), bcols, st);
+    launch_gemm_hs_g(dg, (int)gemm.size(), bgemm, st);
+  }
+};
+
 // Builder of the grouped launch sequence.  Launch closures refer to their
 // descriptors by offset; the arena is uploaded once after planning.
 struct Planner {
@@ -391,7 +446,10 @@ struct Planner {
     std::vector<GemmProb> by[NUM_TILES];
     for (auto &p : probs) {
       if (p.m <= 0 || p.n <= 0 || p.k <= 0) continue;
-      by[tile_for(B, p.m, p.n)].push_back(p);
+      // the split-FP16 variant serves only the k = NB trailing updates (lu());
+      // the other products of a bucket tuned to it take the 128 x 128 tile
+      const int t = tile_for(B, p.m, p.n);
+      by[t == TILE_HS ? TILE_128 : t].push_back(p);
     }
     for (int t = 0; t < NUM_TILES; t++) {
       if (by[t].empty()) continue;
@@ -588,6 +646,7 @@ struct Planner {
       std::vector<c2 *> Bm;
       std::vector<int> nc;
       std::vector<GemmProb> gp;
+      HsStep hs;
       for (const Pan &p : ps) {
         const int i = p.inst;
         c2 *A = (c2 *)B->LU32[i];
@@ -605,12 +664,23 @@ struct Planner {
           g.A = p.A + NB; g.lda = p.lda; g.B = p.A + (size_t)NB * p.lda; g.ldb = p.lda;
           g.C = p.A + (size_t)NB * p.lda + NB; g.ldc = p.lda;
           g.m = rest; g.n = rest; g.k = NB;
-          gp.push_back(g);
+          if (tile_for(B, rest, rest) == TILE_HS) {
+            if ((rc = hs_ensure(B))) return rc;
+            hs.add(B, i, g);
+          } else {
+            gp.push_back(g);
+          }
         }
       }
       swap(sw, false);
       trsm_lower_blocked(ps, Bm, nc, NB);
       gemm(gp, false);
+      if (!hs.gemm.empty()) {
+        const size_t o_r = ar.put(hs.rows), o_c = ar.put(hs.cols), o_g = ar.put(hs.gemm);
+        op([Bp = B, hs, o_r, o_c, o_g](hipStream_t st) {
+          hs.launch(dp<HsSplitProb>(Bp, o_r), dp<HsSplitProb>(Bp, o_c), dp<HsGemmProb>(Bp, o_g), st);
+        });
+      }
     }
     return BFA_OK;
   }
@@ -674,6 +744,7 @@ bfa_batch::~bfa_batch()
   if (cap) (void)hipStreamDestroy(cap);
   for (void *p : M64) if (p) (void)hipFree(p);
   for (void *p : LU32) if (p) (void)hipFree(p);
+  for (void *p : HS) if (p) (void)hipFree(p);
   if (small) (void)hipFree(small);
   if (power) (void)hipFree(power);
 }
@@ -987,7 +1058,9 @@ extern "C" int bfa_batch_tune(bfa_batch *B, int reps)
   HIPCHECK(hipEventCreate(&e0));
   HIPCHECK(hipEventCreate(&e1));
   void *dbuf = nullptr;
-  HIPCHECK(hipMalloc(&dbuf, (size_t)B->count * sizeof(GemmProb) + 256));
+  const size_t hs_r = 0, hs_c = ((size_t)B->count * sizeof(HsSplitProb) + 255) / 256 * 256,
+               hs_g = 2 * hs_c;
+  HIPCHECK(hipMalloc(&dbuf, std::max((size_t)B->count * sizeof(GemmProb), hs_g + B->count * sizeof(HsGemmProb)) + 256));
   int rc = BFA_OK;
   for (int b = 0; b < NBUCKET && rc == BFA_OK; b++) {
     int jbest = -1, best = 0;
@@ -998,8 +1071,10 @@ extern "C" int bfa_batch_tune(bfa_batch *B, int reps)
     int vbest = -1;
     for (int t = 0; t < NUM_TILES; t++) {
       std::vector<GemmProb> v;
+      HsStep hs;
       const int ts = tile_rows(t);
       int blk = 0;
+      if (t == TILE_HS && (rc = hs_ensure(B))) break;
       for (int i = 0; i < B->count; i++) {
         const int j = jbest, rest = B->Np[i] - (j + 1) * NB;
         if (rest <= 0 || bucket_of(rest) != b) continue;
@@ -1009,18 +1084,39 @@ extern "C" int bfa_batch_tune(bfa_batch *B, int reps)
         GemmProb g{};
         g.A = Ajj + NB; g.lda = ld; g.B = Ajj + (size_t)NB * ld; g.ldb = ld; g.C = Ajj + (size_t)NB * ld + NB; g.ldc = ld;
         g.m = rest; g.n = rest; g.k = NB;
+        if (t == TILE_HS) {
+          hs.add(B, i, g);
+          continue;
+        }
         g.tiles_m = (rest + ts - 1) / ts;
         g.blk0 = blk;
         blk += g.tiles_m * ((rest + ts - 1) / ts);
         v.push_back(g);
       }
-      if (hipMemcpy(dbuf, v.data(), v.size() * sizeof(GemmProb), hipMemcpyHostToDevice) != hipSuccess) {
+      // the split variant is timed with its two plane launches
+      char *db = (char *)dbuf;
+      const bool up = t == TILE_HS
+                          ? hipMemcpy(db + hs_r, hs.rows.data(), hs.rows.size() * sizeof(HsSplitProb),
+                                      hipMemcpyHostToDevice) == hipSuccess &&
+                                hipMemcpy(db + hs_c, hs.cols.data(), hs.cols.size() * sizeof(HsSplitProb),
+                                          hipMemcpyHostToDevice) == hipSuccess &&
+                                hipMemcpy(db + hs_g, hs.gemm.data(), hs.gemm.size() * sizeof(HsGemmProb),
+                                          hipMemcpyHostToDevice) == hipSuccess
+                          : hipMemcpy(dbuf, v.data(), v.size() * sizeof(GemmProb), hipMemcpyHostToDevice) == hipSuccess;
+      if (!up) {
         rc = fail(BFA_ERR_HIP, "bfa_batch_tune: upload");
         break;
       }
-      launch_gemm_g(false, t, (const GemmProb *)dbuf, (int)v.size(), blk, st);
+      auto go = [&]() {
+        if (t == TILE_HS)
+          hs.launch((const HsSplitProb *)(db + hs_r), (const HsSplitProb *)(db + hs_c), (const HsGemmProb *)(db + hs_g),
+                    st);
+        else
+          launch_gemm_g(false, t, (const GemmProb *)dbuf, (int)v.size(), blk, st);
+      };
+      go();
       (void)hipEventRecord(e0, st);
-      for (int r = 0; r < reps; r++) launch_gemm_g(false, t, (const GemmProb *)dbuf, (int)v.size(), blk, st);
+      for (int r = 0; r < reps; r++) go();
       (void)hipEventRecord(e1, st);
       if (hipEventSynchronize(e1) != hipSuccess) { rc = fail(BFA_ERR_HIP, "bfa_batch_tune: timing run failed"); break; }
       float ms = 0.f;
@@ -1036,6 +1132,18 @@ extern "C" int bfa_batch_tune(bfa_batch *B, int reps)
   if (rc == BFA_OK && hipGetLastError() != hipSuccess) rc = fail(BFA_ERR_HIP, "bfa_batch_tune: kernel launch failed");
   B->reset_plan();
   return rc;
+}
+
+// the tile variant per bucket, set by the caller (a stored tuning, or one
+// variant everywhere); the next run plans again
+extern "C" int bfa_batch_set_tiles(bfa_batch *B, const int *tile)
+{
+  if (!B || !tile) return fail(BFA_ERR_ARG, "bfa_batch_set_tiles: null argument");
+  for (int b = 0; b < NBUCKET; b++)
+    if (tile[b] < -1 || tile[b] >= NUM_TILES) return fail(BFA_ERR_ARG, "bfa_batch_set_tiles: tile %d", tile[b]);
+  for (int b = 0; b < NBUCKET; b++) B->tile[b] = tile[b];
+  B->reset_plan();
+  return BFA_OK;
 }
 
 extern "C" int bfa_batch_tuning(const bfa_batch *B, int *tile, double *ms)

@@ -136,8 +136,33 @@ void launch_emt_rows_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, 
 // out[(fidx * ni + i) * npw + j][2] = {P_abs, P_scat}
 void launch_emt_final_g(const AsmInst *d, int ni, const FreqState *fs, int npw, double *out, hipStream_t st);
 
-// GEMM tile variants for the autotuner
-enum { TILE_64 = 0, TILE_128 = 1, NUM_TILES = 2 };
-inline int tile_rows(int t) { return t == TILE_128 ? 128 : 64; }
+// GEMM tile variants for the autotuner: complex64 on FP32 MFMA in 64 x 64
+// or 128 x 128 tiles, or split FP16 (hi + lo planes, 128 x 128 tiles on the
+// FP16 matrix cores; the trailing updates with k = 256 only)
+enum { TILE_64 = 0, TILE_128 = 1, TILE_HS = 2, NUM_TILES = 3 };
+inline int tile_rows(int t) { return t == TILE_64 ? 64 : 128; }
+
+// Split-FP16 trailing update of a batch.  Planes: X (rows of A or columns
+// of B, nv of them, k each, column-major with ldx) -> P[4][pr][kp] halves
+// scaled by 2^e per row / column, fac = 2^-e; blocks of 32 rows / columns
+// from blk0, covering pr (a multiple of 128 >= nv).
+struct HsSplitProb {
+  const void *X;
+  void *P, *fac;
+  int64_t ldx, pr;
+  int nv, k, kp, blk0;
+};
+// C(m x n) -= (planes of A)(planes of B) 2^-(e_row + e_col); the workgroups
+// of one problem (a multiple of 8, from blk0) in G x G super-tiles
+struct HsGemmProb {
+  const void *PA, *PB, *rf, *cf;
+  void *C;
+  int64_t mp, np, ldc;
+  int m, n, kp, tiles_m, tiles_n, G, blk0, pad;
+};
+void launch_split_g(bool cols, const HsSplitProb *d, int np, int nblk, hipStream_t st);
+void launch_gemm_hs_g(const HsGemmProb *d, int np, int nblk, hipStream_t st);
+// super-tile size and workgroup count of an m x n product in 128 x 128 tiles
+void hs_grid(int m, int n, int *G, int *nblk);
 
 }  // namespace bfa

@@ -1914,7 +1914,7 @@ unsigned long long sv_tag_c64() { return (unsigned long long)SvTag<float>::bits 
 void launch_gemm_g(bool f64, int tile, const GemmProb *d, int np, int nblk, hipStream_t st)
 {
   if (np <= 0 || nblk <= 0) return;
-  if (tile == TILE_128) {
+  if (tile != TILE_64) {   // the tile rows of tile_rows(): 64 for TILE_64, else 128
     if (f64) hipLaunchKernelGGL(kg_gemm128<double>, dim3(nblk), dim3(512), 0, st, d, np);
     else hipLaunchKernelGGL(kg_gemm128<float>, dim3(nblk), dim3(512), 0, st, d, np);
   } else {

@@ -37,6 +37,7 @@
 #include <vector>
 
 #include "bfa_internal.h"
+#include "bfa_group.h"
 #include "bfa_lu.h"
 
 using namespace bfa;
@@ -79,12 +80,11 @@ __device__ __forceinline__ void split_one(float x, _Float16 &hi, _Float16 &lo)
 // chunks of 32 k (k <= 32 KC); thread (row = t % 32, chunk = t / 32) holds
 // 32 consecutive k of its row.
 template <int KC>
-__global__ __launch_bounds__(32 * KC) void k_split_rows(const c2 *__restrict__ A, int64_t lda, int m, int k, int kp,
-                                                    int64_t mp, _Float16 *P, float *rfac)
+__device__ __forceinline__ void split_rows_body(const c2 *__restrict__ A, int64_t lda, int m, int k, int kp,
+                                                int64_t mp, _Float16 *P, float *rfac, int blk, float (&smax)[KC][32])
 {
-  __shared__ float smax[KC][32];
   const int row = threadIdx.x & 31, kc = threadIdx.x >> 5;
-  const int r = blockIdx.x * 32 + row;
+  const int r = blk * 32 + row;
   float re[32], im[32];
   float mx = 0.0f;
 #pragma unroll
@@ -124,16 +124,24 @@ __global__ __launch_bounds__(32 * KC) void k_split_rows(const c2 *__restrict__ A
   }
 }
 
+template <int KC>
+__global__ __launch_bounds__(32 * KC) void k_split_rows(const c2 *__restrict__ A, int64_t lda, int m, int k, int kp,
+                                                    int64_t mp, _Float16 *P, float *rfac)
+{
+  __shared__ float smax[KC][32];
+  split_rows_body<KC>(A, lda, m, k, kp, mp, P, rfac, blockIdx.x, smax);
+}
+
 // U12 (k x n, column-major, k contiguous) -> planes P[4][np][kp] scaled by
 // 2^e per column, cfac[c] = 2^-e.  Thread (column = t / KC, chunk = t % KC)
 // holds 32 consecutive k of its column (k <= 32 KC); the column maximum by
 // shuffles.
 template <int KC>
-__global__ __launch_bounds__(32 * KC) void k_split_cols(const c2 *__restrict__ B, int64_t ldb, int k, int n, int kp,
-                                                    int64_t np, _Float16 *P, float *cfac)
+__device__ __forceinline__ void split_cols_body(const c2 *__restrict__ B, int64_t ldb, int k, int n, int kp,
+                                                int64_t np, _Float16 *P, float *cfac, int blk)
 {
   const int kc = threadIdx.x % KC;
-  const int c = blockIdx.x * 32 + threadIdx.x / KC;
+  const int c = blk * 32 + threadIdx.x / KC;
   float re[32], im[32];
   float mx = 0.0f;
 #pragma unroll
@@ -170,6 +178,13 @@ __global__ __launch_bounds__(32 * KC) void k_split_cols(const c2 *__restrict__ B
     *(h8 *)(P + 2 * (size_t)np * kp + o) = ih;
     *(h8 *)(P + 3 * (size_t)np * kp + o) = il;
   }
+}
+
+template <int KC>
+__global__ __launch_bounds__(32 * KC) void k_split_cols(const c2 *__restrict__ B, int64_t ldb, int k, int n, int kp,
+                                                    int64_t np, _Float16 *P, float *cfac)
+{
+  split_cols_body<KC>(B, ldb, k, n, kp, np, P, cfac, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1065,7 +1080,64 @@ int block_column(c2 *A, int n, int64_t lda, int j, int jb, int *ipiv, int *flags
   return BFA_OK;
 }
 
+// Grouped forms over the problems of a batch (descriptors sorted by blk0,
+// each workgroup's problem found by binary search): the split planes and
+// the split-FP16 product of every instance's trailing update in one launch
+// each.
+template <typename Q> __device__ __forceinline__ int hs_find(const Q *__restrict__ d, int np, int bid)
+{
+  int lo = 0, hi = np - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d[mid].blk0 <= bid) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <bool COLS>
+__global__ __launch_bounds__(256) void kg_split(const HsSplitProb *__restrict__ d, int np)
+{
+  __shared__ float smax[8][32];
+  const HsSplitProb q = d[hs_find(d, np, blockIdx.x)];
+  const int b = blockIdx.x - q.blk0;
+  if (COLS)
+    split_cols_body<8>((const c2 *)q.X, q.ldx, q.k, q.nv, q.kp, q.pr, (_Float16 *)q.P, (float *)q.fac, b);
+  else
+    split_rows_body<8>((const c2 *)q.X, q.ldx, q.nv, q.k, q.kp, q.pr, (_Float16 *)q.P, (float *)q.fac, b, smax);
+}
+
+__global__ __launch_bounds__(512, 1) void kg_gemm_hs(const HsGemmProb *__restrict__ d, int np)
+{
+  __shared__ __align__(16) _Float16 As[2][4][HT][HK];
+  __shared__ __align__(16) _Float16 Bs[2][4][HT][HK];
+  const HsGemmProb q = d[hs_find(d, np, blockIdx.x)];
+  int ti, tj;
+  if (!super_tile(blockIdx.x - q.blk0, q.tiles_m, q.tiles_n, q.G, ti, tj)) return;
+  gemm_hs_tile<8>(q.m, q.n, q.kp, (const _Float16 *)q.PA, q.mp, (const _Float16 *)q.PB, q.np, (const float *)q.rf,
+                  (const float *)q.cf, (c2 *)q.C, q.ldc, ti * HT, tj * HT, 0, As, Bs);
+}
+
 }  // namespace
+
+void bfa::launch_split_g(bool cols, const HsSplitProb *d, int np, int nblk, hipStream_t st)
+{
+  if (np <= 0 || nblk <= 0) return;
+  if (cols) hipLaunchKernelGGL(kg_split<true>, dim3(nblk), dim3(256), 0, st, d, np);
+  else hipLaunchKernelGGL(kg_split<false>, dim3(nblk), dim3(256), 0, st, d, np);
+}
+
+void bfa::launch_gemm_hs_g(const HsGemmProb *d, int np, int nblk, hipStream_t st)
+{
+  if (np > 0 && nblk > 0) hipLaunchKernelGGL(kg_gemm_hs, dim3(nblk), dim3(512), 0, st, d, np);
+}
+
+void bfa::hs_grid(int m, int n, int *G, int *nblk)
+{
+  size_t nb = 0;
+  super_grid((m + HT - 1) / HT, (n + HT - 1) / HT, G, &nb);
+  *nblk = (int)nb;
+}
 
 void bfa::local_release()
 {

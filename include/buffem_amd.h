@@ -294,8 +294,12 @@ int  bfa_batch_info(const bfa_batch *b, int *count, int *npw, int *N, int *Np);
  * (reps timed launches each).  Without it a fixed rule picks the tile.    */
 int  bfa_batch_tune(bfa_batch *b, int reps);
 /* tile[16]: chosen variant per bucket (-1 = untuned; 0 = 64x64, 1 =
- * 128x128); ms[16*2]: measured time per bucket and variant.                */
+ * 128x128 complex64 on FP32 MFMA, 2 = split FP16 on the FP16 matrix cores,
+ * timed with its plane launches); ms[16*3]: time per bucket and variant.  */
 int  bfa_batch_tuning(const bfa_batch *b, int *tile, double *ms);
+/* tile[16] as bfa_batch_tuning reports it (-1 = the fixed rule): a stored
+ * tuning, or one variant for every bucket (no reference counterpart).      */
+int  bfa_batch_set_tiles(bfa_batch *b, const int *tile);
 /* Sweep nfreq (<= 256) real frequencies.  Host outputs (may be NULL):
  * power[((f*count + i)*npw + j)*2 + {0,1}] = {P_abs, P_scat};
  * iters[f*count + i], resid[f*count + i]: refinement rounds and final
