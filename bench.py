@@ -222,7 +222,8 @@ def bench_batch(args, bfa, torch, log):
         "metric": "batched sweep: assemble+factor+solve s per frequency (48 uneven Kuhn spheres, all instances)",
         "value": per_freq, "unit": "s/frequency", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": t_sweep * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "complex64 LU (f32 MFMA) + f64 refinement", "data": "synthetic",
+        "dtype": "complex64 LU (trailing updates on the tuned variant: split-f16 or f32 MFMA) + f64 refinement",
+        "data": "synthetic",
         "config": {"workload": "batch_48_kuhn_spheres_8_freqs", "instances": desc, "omegas": omegas.tolist(),
                    "nrhs": 1, "td_order": args.td_order},
         "fp64_loop_s_per_freq": t_fp64, "speedup_vs_fp64_loop": t_fp64 / per_freq,
@@ -482,7 +483,7 @@ def main():
                      "gemm_tflops": (v["gemm"][1] / (v["gemm"][0] * 1e-3) / 1e12) if v["gemm"][0] > 0 else None,
                      "refine": v["refine"], "power_rel_err_vs_fp64": errs.get(f)} for f, v in results.items()}
     dtypes = {"fp64": "complex128 (f64)", "mixed": "complex64 LU (f32 MFMA) + f64 refinement",
-              "graph": "complex64 LU (f32 MFMA) + f64 refinement",
+              "graph": "complex64 LU (trailing updates on the tuned variant: split-f16 or f32 MFMA) + f64 refinement",
               "fast": "complex64 LU (split-f16 MFMA updates, block-local pivots) + f64 refinement"}
     out = {
         "metric": "assemble+factor+solve s/frequency at N~20k (Kuhn sphere n=12); mixed vs FP64 GPU at rel err <=1e-8",
