@@ -235,7 +235,9 @@ __device__ __forceinline__ h8 hneg(h8 x)
 // fed by direct global-to-LDS loads (one wave per SIMD) 3.61 ms -- neither
 // LDS nor operand traffic limits it; the C read-modify-write (16 B per
 // entry per 24 x 256 FP16 products, near the 312 flop/B ridge) and the
-// unoverlapped epilogue of one workgroup per CU do.
+// unoverlapped epilogue of one workgroup per CU do.  Issuing half of the C
+// loads during the last k stage (229 VGPRs; all of them spill at 256)
+// left the factorisation's time unchanged.
 template <int NW>
 __device__ __forceinline__ void gemm_hs_tile(int m, int n, int kp, const _Float16 *__restrict__ PA, int64_t mp,
                                              const _Float16 *__restrict__ PB, int64_t np,
