@@ -1295,7 +1295,10 @@ __device__ cd expr_run(const ExprHeader *h, int slot, const cd *vars)
 
 // Permittivity tensor at x (SVTensor::Evaluate, SVTensor.cc:306-373); kinds
 // 0-3 are the closed forms documented with bfa_material, kind 4 a compiled
-// SVTensor evaluated at w = omega * FreqUnit.
+// SVTensor evaluated at w = omega * FreqUnit.  EXPR = false leaves kind 4
+// out of the code (its interpreter's registers and stack would otherwise
+// set the calling kernel's occupancy for every material).
+template <bool EXPR = true>
 __device__ void eps_tensor(const bfa_material &m, const double *x, double omega, cd Q[3][3])
 {
   for (int r = 0; r < 3; r++)
@@ -1312,7 +1315,7 @@ __device__ void eps_tensor(const bfa_material &m, const double *x, double omega,
       const bool up = x[2] > 0.0;
       Q[0][0] = Q[1][1] = Q[2][2] = up ? cd{m.p[0], m.p[1]} : cd{m.p[2], m.p[3]};
     } return;
-    case 4: {
+    case 4: if constexpr (EXPR) {
       const ExprHeader *h = (const ExprHeader *)m.expr;
       cd v[EV_COUNT];
       const double rho = sqrt(x[0] * x[0] + x[1] * x[1]);
@@ -1403,6 +1406,7 @@ extern "C" int bfa_material_eval(const bfa_material *mat, double omega, int npts
 }
 
 // -(1/omega^2) int_t F_a . chi^-1 . F_b  over TetCR33 (GetOverlapIntegrand)
+template <bool EXPR = true>
 __device__ cd overlap_tet(const DevGeom &g, int t, const double *QA, double pfA, const double *QB,
                           double pfB, double omega, const bfa_material &mat)
 {
@@ -1411,7 +1415,7 @@ __device__ cd overlap_tet(const DevGeom &g, int t, const double *QA, double pfA,
   for (int i = 0; i < 33; i++) {
     const double x[3] = {q[4 * i], q[4 * i + 1], q[4 * i + 2]};
     cd Q[3][3], W[3][3];
-    eps_tensor(mat, x, omega, Q);
+    eps_tensor<EXPR>(mat, x, omega, Q);
     inv_chi(Q, W);
     double FA[3], FB[3];
     for (int d = 0; d < 3; d++) { FA[d] = pfA * (x[d] - QA[d]); FB[d] = pfB * (x[d] - QB[d]); }
@@ -1589,6 +1593,7 @@ extern "C" int bfa_overlap_blocks(bfa_geometry *g, double omega, const bfa_mater
   return rc;
 }
 
+template <bool EXPR = true>
 __device__ __forceinline__ void near_entry(const DevGeom &g, double kre, double kim, const double *table,
                                            const bfa_material &mat, cd *M, int64_t ld, int64_t j)
 {
@@ -1634,18 +1639,34 @@ __device__ __forceinline__ void near_entry(const DevGeom &g, double kre, double 
       const int ta = g.bfT[2 * a + sa], tb = g.bfT[2 * b + sb];
       if (ta != tb) continue;
       if (a == b && sa != sb) continue;
-      acc = cadd(acc, overlap_tet(g, ta, g.bfQ + 6 * a + 3 * sa, g.bfPF[2 * a + sa], g.bfQ + 6 * b + 3 * sb,
+      acc = cadd(acc, overlap_tet<EXPR>(g, ta, g.bfQ + 6 * a + 3 * sa, g.bfPF[2 * a + sa], g.bfQ + 6 * b + 3 * sb,
                                   g.bfPF[2 * b + sb], omega, mat));
     }
   M[(size_t)b * ld + a] = acc;
   M[(size_t)a * ld + b] = acc;
 }
 
+// EXPR: the material may be a compiled SVTensor (kind 4); the closed-form
+// kinds run the variant without the expression interpreter
+template <bool EXPR>
 __global__ __launch_bounds__(256) void k_near(DevGeom g, double kre, double kim, const double *table,
                                               bfa_material mat, cd *M, int64_t ld)
 {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < g.nnz) near_entry(g, kre, kim, table, mat, M, ld, j);
+  if (j < g.nnz) near_entry<EXPR>(g, kre, kim, table, mat, M, ld, j);
+}
+
+static void launch_near(const bfa_geometry *g, double omega_re, double omega_im, const bfa_material *mat, cd *M,
+                        int64_t ld, hipStream_t st)
+{
+  const int64_t nnz = g->dg.nnz;
+  const dim3 grid((unsigned)((nnz + 255) / 256)), block(256);
+  if (mat->kind == 4)
+    hipLaunchKernelGGL(k_near<true>, grid, block, 0, st, g->dg, omega_re, omega_im, (const double *)g->d.table, *mat,
+                       M, ld);
+  else
+    hipLaunchKernelGGL(k_near<false>, grid, block, 0, st, g->dg, omega_re, omega_im, (const double *)g->d.table,
+                       *mat, M, ld);
 }
 
 // The two-pass far field's moment table (nt (nt + 1) / 2 records of 128 B,
@@ -1713,9 +1734,7 @@ static int assemble_vie(bfa_geometry *g, double omega_re, double omega_im, const
       else hipLaunchKernelGGL((k_far_tp<false, false>), gr2, bl2, 0, st, g->dg, omega_re, omega_im, M, ld, nt2);
     }
   }
-  const int64_t nnz = g->dg.nnz;
-  hipLaunchKernelGGL(k_near, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, st, g->dg, omega_re, omega_im,
-                     (const double *)g->d.table, *mat, M, ld);
+  launch_near(g, omega_re, omega_im, mat, M, ld, st);
   HIPCHECK(hipGetLastError());
   return BFA_OK;
 }
@@ -1747,9 +1766,7 @@ int bfa::assemble_near_only(bfa_geometry *g, double omega_re, double omega_im, c
 {
   if (!g || !mat || !d_M) return fail(BFA_ERR_ARG, "assemble_near_only: null argument");
   if (!g->uploaded || !g->static_built) return fail(BFA_ERR_STATE, "assemble_near_only: static table not built");
-  const int64_t nnz = g->dg.nnz;
-  hipLaunchKernelGGL(k_near, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, (hipStream_t)stream, g->dg, omega_re,
-                     omega_im, (const double *)g->d.table, *mat, (cd *)d_M, ld);
+  launch_near(g, omega_re, omega_im, mat, (cd *)d_M, ld, (hipStream_t)stream);
   HIPCHECK(hipGetLastError());
   return BFA_OK;
 }
