@@ -2270,11 +2270,13 @@ __global__ __launch_bounds__(FT_THREADS) void kg_far(const AsmInst *__restrict__
   far_tp_tile<REALK>(q.g, fs->omega_re, REALK ? 0.0 : fs->omega_im, (cd *)q.M, q.ldm, ti, tj, sh);
 }
 
+// EXPR: some instance's material is a compiled SVTensor (as k_near)
+template <bool EXPR>
 __global__ __launch_bounds__(256) void kg_near(const AsmInst *__restrict__ d, int ni, const FreqState *fs)
 {
   const AsmInst &q = d[find_inst(d, ni, blockIdx.x, GF_NEAR)];
   const int64_t j = (int64_t)(blockIdx.x - q.near0) * blockDim.x + threadIdx.x;
-  if (j < q.g.nnz) near_entry(q.g, fs->omega_re, fs->omega_im, q.table, q.mat, (cd *)q.M, q.ldm, j);
+  if (j < q.g.nnz) near_entry<EXPR>(q.g, fs->omega_re, fs->omega_im, q.table, q.mat, (cd *)q.M, q.ldm, j);
 }
 
 __global__ __launch_bounds__(256) void kg_rhs(const AsmInst *__restrict__ d, int ni, const FreqState *fs,
@@ -2365,9 +2367,11 @@ void launch_far_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, bool 
   if (realk) hipLaunchKernelGGL(kg_far<true>, dim3(nblk), dim3(FT_THREADS), 0, st, d, ni, fs);
   else hipLaunchKernelGGL(kg_far<false>, dim3(nblk), dim3(FT_THREADS), 0, st, d, ni, fs);
 }
-void launch_near_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, hipStream_t st)
+void launch_near_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, bool expr, hipStream_t st)
 {
-  if (nblk > 0) hipLaunchKernelGGL(kg_near, dim3(nblk), dim3(256), 0, st, d, ni, fs);
+  if (nblk <= 0) return;
+  if (expr) hipLaunchKernelGGL(kg_near<true>, dim3(nblk), dim3(256), 0, st, d, ni, fs);
+  else hipLaunchKernelGGL(kg_near<false>, dim3(nblk), dim3(256), 0, st, d, ni, fs);
 }
 void launch_rhs_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, const PWBatch &pw, int npw, hipStream_t st)
 {

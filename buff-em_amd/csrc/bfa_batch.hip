@@ -842,7 +842,11 @@ int build_plan(bfa_batch *B, double tol, int maxit)
   // bfa_batch_run accepts positive real frequencies only (k_set_state stores
   // omega_im = 0), so the far field runs its real-k kernel
   P.op([B, o_asm, ni, nfar, fs](hipStream_t st) { launch_far_g(dp<AsmInst>(B, o_asm), ni, nfar, fs, true, st); });
-  P.op([B, o_asm, ni, nnear, fs](hipStream_t st) { launch_near_g(dp<AsmInst>(B, o_asm), ni, nnear, fs, st); });
+  bool expr = false;
+  for (const bfa_material &m : B->mats) expr = expr || m.kind == 4;
+  P.op([B, o_asm, ni, nnear, fs, expr](hipStream_t st) {
+    launch_near_g(dp<AsmInst>(B, o_asm), ni, nnear, fs, expr, st);
+  });
   P.op([B, o_asm, ni, nrow, fs, pws, npw](hipStream_t st) {
     launch_rhs_g(dp<AsmInst>(B, o_asm), ni, nrow, fs, pws, npw, st);
   });

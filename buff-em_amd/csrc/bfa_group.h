@@ -126,7 +126,8 @@ struct PWBatch { bfa_planewave pw[BFA_MAXPW_BATCH]; };
 // realk: every frequency the launch will see is real (the kernel then never
 // compiles the complex-k moments, which would spill registers)
 void launch_far_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, bool realk, hipStream_t st);
-void launch_near_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, hipStream_t st);
+// expr: some instance's material is a compiled SVTensor (material kind 4)
+void launch_near_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, bool expr, hipStream_t st);
 void launch_rhs_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, const PWBatch &pw, int npw,
                   hipStream_t st);
 void launch_emt_q_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, hipStream_t st);
