@@ -1070,9 +1070,11 @@ __device__ __forceinline__ void tp_fetch(const double *__restrict__ tab, int S, 
 // by BF (15.5 ms: bank conflicts), 26 KB of LDS with the records read per
 // BF pair (15.8 ms), the tiles dealt to the XCDs in contiguous runs that
 // walk two tile rows at a time, so that neighbouring tiles sharing records
-// run on one L2 (fetch 25 -> 16 GB per launch, time 12.2 -> 12.4 ms) -- the
-// tile is bound by the latency of its dependent global reads, not by HBM
-// (without the table reads it still takes 10.7 ms).
+// run on one L2 (fetch 25 -> 16 GB per launch, time 12.2 -> 12.4 ms), one
+// workgroup per CU looping over the tiles (so that stores drain while the
+// next tile loads; unchanged) -- the tile is bound by the latency of its
+// dependent global reads, not by HBM (without the table reads it still
+// takes 10.7 ms).
 __global__ __launch_bounds__(FT_THREADS) void k_far_gather(DevGeom g, double kre, double kim,
                                                            const double *__restrict__ tpt, cd *M, int64_t ld,
                                                            int ntiles)
