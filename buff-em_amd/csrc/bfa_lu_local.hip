@@ -135,7 +135,8 @@ __global__ __launch_bounds__(32 * KC) void k_split_rows(const c2 *__restrict__ A
 // U12 (k x n, column-major, k contiguous) -> planes P[4][np][kp] scaled by
 // 2^e per column, cfac[c] = 2^-e.  Thread (column = t / KC, chunk = t % KC)
 // holds 32 consecutive k of its column (k <= 32 KC); the column maximum by
-// shuffles.
+// shuffles.  (Reading the block's columns along k into LDS first, for
+// fully contiguous loads, left the factorisation's time unchanged.)
 template <int KC>
 __device__ __forceinline__ void split_cols_body(const c2 *__restrict__ B, int64_t ldb, int k, int n, int kp,
                                                 int64_t np, _Float16 *P, float *cfac, int blk)
