@@ -999,12 +999,11 @@ struct TpTileShared {
 };
 #define TPT_THREADS 256
 
+// one 32 x 32 tile (bs, bt) of tetrahedron pairs of pass 1
 template <bool REALK, bool SPLIT>
-__global__ __launch_bounds__(TPT_THREADS) void k_tp_table(DevGeom g, double kre, double kim, double *tab, int ntb)
+__device__ __forceinline__ void tp_table_tile(const DevGeom &g, double kre, double kim, double *tab, int bs, int bt,
+                                              TpTileShared &sh)
 {
-  __shared__ TpTileShared sh;
-  int bs, bt;
-  tile_coords(blockIdx.x, ntb, bs, bt);
   const int tid = threadIdx.x, nt = g.nt;
   // points of the tile's 32 row and 32 column tetrahedra
   for (int idx = tid; idx < 2 * 32 * 16; idx += TPT_THREADS) {
@@ -1041,6 +1040,15 @@ __global__ __launch_bounds__(TPT_THREADS) void k_tp_table(DevGeom g, double kre,
   }
 }
 
+template <bool REALK, bool SPLIT>
+__global__ __launch_bounds__(TPT_THREADS) void k_tp_table(DevGeom g, double kre, double kim, double *tab, int ntb)
+{
+  __shared__ TpTileShared sh;
+  int bs, bt;
+  tile_coords(blockIdx.x, ntb, bs, bt);
+  tp_table_tile<REALK, SPLIT>(g, kre, kim, tab, bs, bt, sh);
+}
+
 // the 16 moments of row tetrahedron S and column tetrahedron T from the table
 __device__ __forceinline__ void tp_fetch(const double *__restrict__ tab, int S, int T, int nt, double *m)
 {
@@ -1075,15 +1083,20 @@ __device__ __forceinline__ void tp_fetch(const double *__restrict__ tab, int S, 
 // next tile loads; unchanged) -- the tile is bound by the latency of its
 // dependent global reads, not by HBM (without the table reads it still
 // takes 10.7 ms).
-__global__ __launch_bounds__(FT_THREADS) void k_far_gather(DevGeom g, double kre, double kim,
-                                                           const double *__restrict__ tpt, cd *M, int64_t ld,
-                                                           int ntiles)
+// LDS of one pass-2 tile: the records, and the tile's BFs by field
+struct GatherShared {
+  FarTpShared t;
+  double bfd[2][8][FT2];
+  int bfv[2][5][FT2];
+};
+
+__device__ __forceinline__ void far_gather_tile(const DevGeom &g, double kre, double kim,
+                                                const double *__restrict__ tpt, cd *M, int64_t ld, int ti, int tj,
+                                                GatherShared &gs)
 {
-  __shared__ FarTpShared sh;
-  __shared__ double bfd[2][8][FT2];
-  __shared__ int bfv[2][5][FT2];
-  int ti, tj;
-  tile_coords(blockIdx.x, ntiles, ti, tj);
+  FarTpShared &sh = gs.t;
+  double (&bfd)[2][8][FT2] = gs.bfd;
+  int (&bfv)[2][5][FT2] = gs.bfv;
   const int tid = threadIdx.x, nth = blockDim.x, N = g.nbf, nt = g.nt;
   const int *tab[2] = {g.ftTab + (size_t)ti * FTAB, g.ftTab + (size_t)tj * FTAB};
   const int nr = tab[0][4 * FT2], nc = tab[1][4 * FT2];
@@ -1169,6 +1182,16 @@ __global__ __launch_bounds__(FT_THREADS) void k_far_gather(DevGeom g, double kre
     M[(size_t)b * ld + a] = v;
     M[(size_t)a * ld + b] = v;
   }
+}
+
+__global__ __launch_bounds__(FT_THREADS) void k_far_gather(DevGeom g, double kre, double kim,
+                                                           const double *__restrict__ tpt, cd *M, int64_t ld,
+                                                           int ntiles)
+{
+  __shared__ GatherShared gs;
+  int ti, tj;
+  tile_coords(blockIdx.x, ntiles, ti, tj);
+  far_gather_tile(g, kre, kim, tpt, M, ld, ti, tj, gs);
 }
 
 // ---------------------------------------------------------------------------
@@ -2243,10 +2266,10 @@ extern "C" int bfa_emt_power(bfa_geometry *g, double omega, const bfa_planewave 
 // frequency comes from device memory so that one captured graph serves a
 // whole sweep.
 // ===========================================================================
-enum { GF_FAR = 0, GF_NEAR, GF_ROW, GF_EMT };
+enum { GF_FAR = 0, GF_NEAR, GF_ROW, GF_EMT, GF_TAB };
 __device__ __forceinline__ int inst_first(const AsmInst &q, int which)
 {
-  return which == GF_FAR ? q.far0 : which == GF_NEAR ? q.near0 : which == GF_ROW ? q.row0 : q.emt0;
+  return which == GF_FAR ? q.far0 : which == GF_NEAR ? q.near0 : which == GF_ROW ? q.row0 : which == GF_EMT ? q.emt0 : q.tab0;
 }
 __device__ __forceinline__ int find_inst(const AsmInst *__restrict__ d, int n, int bid, int which)
 {
@@ -2270,6 +2293,26 @@ __global__ __launch_bounds__(FT_THREADS) void kg_far(const AsmInst *__restrict__
   int ti, tj;
   tile_coords(blockIdx.x - q.far0, q.ntiles, ti, tj);
   far_tp_tile<REALK>(q.g, fs->omega_re, REALK ? 0.0 : fs->omega_im, (cd *)q.M, q.ldm, ti, tj, sh);
+}
+
+// the two-pass far field over the instances (as k_tp_table, k_far_gather)
+template <bool REALK>
+__global__ __launch_bounds__(TPT_THREADS) void kg_tp_table(const AsmInst *__restrict__ d, int ni, const FreqState *fs)
+{
+  __shared__ TpTileShared sh;
+  const AsmInst &q = d[find_inst(d, ni, blockIdx.x, GF_TAB)];
+  int bs, bt;
+  tile_coords(blockIdx.x - q.tab0, q.ntb, bs, bt);
+  tp_table_tile<REALK, false>(q.g, fs->omega_re, REALK ? 0.0 : fs->omega_im, q.tpt, bs, bt, sh);
+}
+
+__global__ __launch_bounds__(FT_THREADS) void kg_far_gather(const AsmInst *__restrict__ d, int ni, const FreqState *fs)
+{
+  __shared__ GatherShared gs;
+  const AsmInst &q = d[find_inst(d, ni, blockIdx.x, GF_FAR)];
+  int ti, tj;
+  tile_coords(blockIdx.x - q.far0, q.ntiles, ti, tj);
+  far_gather_tile(q.g, fs->omega_re, fs->omega_im, q.tpt, (cd *)q.M, q.ldm, ti, tj, gs);
 }
 
 // EXPR: some instance's material is a compiled SVTensor (as k_near)
@@ -2363,6 +2406,20 @@ __global__ __launch_bounds__(64) void kg_emt_final(const AsmInst *__restrict__ d
 }
 
 namespace bfa {
+void launch_tp_table_g(const AsmInst *d, int ni, int ntab, const FreqState *fs, bool realk, hipStream_t st)
+{
+  if (ntab <= 0) return;
+  if (realk) hipLaunchKernelGGL(kg_tp_table<true>, dim3(ntab), dim3(TPT_THREADS), 0, st, d, ni, fs);
+  else hipLaunchKernelGGL(kg_tp_table<false>, dim3(ntab), dim3(TPT_THREADS), 0, st, d, ni, fs);
+}
+
+void launch_far_gather_g(const AsmInst *d, int ni, int nfar, const FreqState *fs, hipStream_t st)
+{
+  if (nfar > 0) hipLaunchKernelGGL(kg_far_gather, dim3(nfar), dim3(FT_THREADS), 0, st, d, ni, fs);
+}
+
+size_t tp_table_bytes(int nt) { return (size_t)nt * (nt + 1) / 2 * 16 * sizeof(double); }
+
 void launch_far_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, bool realk, hipStream_t st)
 {
   if (nblk <= 0) return;

@@ -289,6 +289,11 @@ struct bfa_batch {
   // split-FP16 planes of each instance's trailing update (allocated when a
   // tuned bucket first uses them): A's planes, B's, row and column factors
   std::vector<void *> HS;
+  // two-pass far field: each instance's tetrahedron-pair moment table
+  // (allocated at the first plan when all of them fit in a quarter of the
+  // free memory; otherwise the single-pass far field)
+  std::vector<void *> TP;
+  int tp_state = 0;              // 0 not tried, 1 tables, -1 single pass
   // everything else in one allocation; per-instance byte offsets
   void *small = nullptr;
   struct Off { size_t ipiv, perm, B, X, R, D, Y, Xs, inv, tick, P, norms, part, sdata, sj; };
@@ -745,6 +750,7 @@ bfa_batch::~bfa_batch()
   for (void *p : M64) if (p) (void)hipFree(p);
   for (void *p : LU32) if (p) (void)hipFree(p);
   for (void *p : HS) if (p) (void)hipFree(p);
+  for (void *p : TP) if (p) (void)hipFree(p);
   if (small) (void)hipFree(small);
   if (power) (void)hipFree(power);
 }
@@ -793,10 +799,35 @@ std::vector<SolveInst> solve_descs(const bfa_batch *B, int &nrow, int &nres, int
   return v;
 }
 
-std::vector<AsmInst> asm_descs(const bfa_batch *B, bool solution, int &nfar, int &nnear, int &nrow, int &nemt)
+// the per-instance moment tables of the two-pass far field, or none
+void tp_tables(bfa_batch *B)
+{
+  if (B->tp_state != 0) return;
+  B->tp_state = -1;
+  const char *e = getenv("BFA_FAR_TABLE");
+  if (e && strcmp(e, "0") == 0) return;
+  size_t need = 0, fr = 0, tot = 0;
+  for (int i = 0; i < B->count; i++) need += tp_table_bytes(B->geo[i]->nt);
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess || need > fr / 4) return;
+  B->TP.assign(B->count, nullptr);
+  for (int i = 0; i < B->count; i++)
+    if (hipMalloc(&B->TP[i], tp_table_bytes(B->geo[i]->nt)) != hipSuccess) {
+      (void)hipGetLastError();
+      for (void *&p : B->TP) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+      }
+      B->TP.clear();
+      return;
+    }
+  B->tp_state = 1;
+}
+
+std::vector<AsmInst> asm_descs(const bfa_batch *B, bool solution, int &nfar, int &nnear, int &nrow, int &nemt,
+                               int &ntab)
 {
   std::vector<AsmInst> v(B->count);
-  nfar = nnear = nrow = nemt = 0;
+  nfar = nnear = nrow = nemt = ntab = 0;
   for (int i = 0; i < B->count; i++) {
     AsmInst &q = v[i];
     const bfa_geometry *g = B->geo[i];
@@ -820,6 +851,10 @@ std::vector<AsmInst> asm_descs(const bfa_batch *B, bool solution, int &nfar, int
     nrow += q.nrowblk;
     q.emt0 = nemt;
     nemt += q.ntiles * (q.ntiles + 1) / 2;
+    q.tpt = B->tp_state == 1 ? (double *)B->TP[i] : nullptr;
+    q.ntb = (g->nt + 31) / 32;
+    q.tab0 = ntab;
+    ntab += q.ntb * (q.ntb + 1) / 2;
   }
   return v;
 }
@@ -830,9 +865,10 @@ int build_plan(bfa_batch *B, double tol, int maxit)
   Planner P(B);
   const int ni = B->count, npw = B->npw;
   // descriptor tables
-  int nfar, nnear, nrow, nemt, nfar2, nnear2, nrow2, nemt2, srow, sres, scol;
-  const size_t o_asm = P.ar.put(asm_descs(B, false, nfar, nnear, nrow, nemt));
-  const size_t o_emt = P.ar.put(asm_descs(B, true, nfar2, nnear2, nrow2, nemt2));
+  int nfar, nnear, nrow, nemt, ntab, nfar2, nnear2, nrow2, nemt2, ntab2, srow, sres, scol;
+  tp_tables(B);
+  const size_t o_asm = P.ar.put(asm_descs(B, false, nfar, nnear, nrow, nemt, ntab));
+  const size_t o_emt = P.ar.put(asm_descs(B, true, nfar2, nnear2, nrow2, nemt2, ntab2));
   const size_t o_sol = P.ar.put(solve_descs(B, srow, sres, scol));
   if (nemt < 0 || nfar < 0) return fail(BFA_ERR_ARG, "batch: problem too large for one grouped launch");
   const PWBatch pws = B->pws;
@@ -841,7 +877,12 @@ int build_plan(bfa_batch *B, double tol, int maxit)
   P.phase = 0;
   // bfa_batch_run accepts positive real frequencies only (k_set_state stores
   // omega_im = 0), so the far field runs its real-k kernel
-  P.op([B, o_asm, ni, nfar, fs](hipStream_t st) { launch_far_g(dp<AsmInst>(B, o_asm), ni, nfar, fs, true, st); });
+  if (B->tp_state == 1) {
+    P.op([B, o_asm, ni, ntab, fs](hipStream_t st) { launch_tp_table_g(dp<AsmInst>(B, o_asm), ni, ntab, fs, true, st); });
+    P.op([B, o_asm, ni, nfar, fs](hipStream_t st) { launch_far_gather_g(dp<AsmInst>(B, o_asm), ni, nfar, fs, st); });
+  } else {
+    P.op([B, o_asm, ni, nfar, fs](hipStream_t st) { launch_far_g(dp<AsmInst>(B, o_asm), ni, nfar, fs, true, st); });
+  }
   bool expr = false;
   for (const bfa_material &m : B->mats) expr = expr || m.kind == 4;
   P.op([B, o_asm, ni, nnear, fs, expr](hipStream_t st) {

@@ -111,6 +111,8 @@ struct AsmInst {
   int far0, ntiles;      // far-field tiles (upper triangle of FT2 x FT2 BF tiles) from far0
   int near0, row0;       // touching-pair blocks (256) / row blocks (256)
   int emt0, nrowblk;     // EMT Q blocks (256 entries of N x N) from emt0
+  double *tpt;           // two-pass far field: the instance's tetrahedron-pair moment table (null: single pass)
+  int tab0, ntb;         // its 32 x 32 tetrahedron tiles (upper triangle of ntb x ntb) from tab0
 };
 
 // Frequency of the current replay: written by a one-thread kernel before a
@@ -126,6 +128,12 @@ struct PWBatch { bfa_planewave pw[BFA_MAXPW_BATCH]; };
 // realk: every frequency the launch will see is real (the kernel then never
 // compiles the complex-k moments, which would spill registers)
 void launch_far_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, bool realk, hipStream_t st);
+// the two-pass far field of the instances (every tpt set): the moment tables
+// (ntab workgroups in all), then the BF tiles gathering them (nfar)
+void launch_tp_table_g(const AsmInst *d, int ni, int ntab, const FreqState *fs, bool realk, hipStream_t st);
+void launch_far_gather_g(const AsmInst *d, int ni, int nfar, const FreqState *fs, hipStream_t st);
+// bytes of one instance's moment table (nt tetrahedra)
+size_t tp_table_bytes(int nt);
 // expr: some instance's material is a compiled SVTensor (material kind 4)
 void launch_near_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, bool expr, hipStream_t st);
 void launch_rhs_g(const AsmInst *d, int ni, int nblk, const FreqState *fs, const PWBatch &pw, int npw,
