@@ -140,3 +140,44 @@ def test_batch_autotuner(bfa, torch_cuda, batch_setup):
         assert (r["info"] == 0).all() and (r["resid"] <= 1e-10).all()
         assert rel.max() < 1e-9
     b.set_tiles(-1)
+
+
+_SINGLE_PASS_BATCH = r"""
+import os, sys
+import numpy as np
+import torch
+sys.path.insert(0, os.path.join(sys.argv[1], "tests"))
+from conftest import load_bfa
+from test_batch import _instances, SPEC
+bfa = load_bfa()
+geoms, mats = _instances(bfa)
+b = bfa.Batch(geoms, mats, bfa.planewaves(SPEC))
+r = b.run([1.0], use_graph=True)
+np.savez(sys.argv[2], power=r["power"], **{f"m{i}": b.matrix(i) for i in range(len(geoms))})
+"""
+
+
+def test_batch_single_pass_far_field(bfa, torch_cuda, batch_setup, tmp_path):
+    """The batch's far field without the moment tables (BFA_FAR_TABLE=0, in a
+    child process: the switch is read once per process) against the two-pass
+    one: the same sums in another order, matrices to 1e-14 relative, powers
+    to 1e-12."""
+    import os
+    import subprocess
+    import sys
+    geoms, mats, pws, b = batch_setup
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = str(tmp_path / "single_pass.npz")
+    r = subprocess.run([sys.executable, "-c", _SINGLE_PASS_BATCH, root, out], capture_output=True, text=True,
+                       env=dict(os.environ, BFA_FAR_TABLE="0"), timeout=300)
+    print(r.stdout, r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    single = np.load(out)
+    two = b.run([1.0], use_graph=True)
+    for i in range(len(geoms)):
+        A, B = b.matrix(i), single[f"m{i}"]
+        err = np.linalg.norm(A - B) / np.linalg.norm(B)
+        print(f"[batch] instance {i}: two-pass vs single-pass far field {err:.2e}", flush=True)
+        assert err < 1e-14
+    rel = np.abs(two["power"] - single["power"]) / np.abs(single["power"])
+    assert rel.max() < 1e-12, rel.max()
