@@ -3,7 +3,10 @@
 // hipGraph per sweep step.
 //
 // Per frequency (all instances at once, every launch grouped):
-//   assembly   far-field tiles, touching pairs, RHS       (complex128)
+//   assembly   far field (every instance's tetrahedron-pair moments into
+//              its own table, then the BF tiles gathering them; one pass
+//              when the tables would not fit), touching pairs, RHS
+//              (complex128)
 //   convert    M64 -> complex64 copy padded to a multiple of 256 with an
 //              identity block (the padding never pivots: its rows are zero
 //              in the original columns)
@@ -17,8 +20,10 @@
 //              the graph is static
 //   power      EMT absorbed / scattered power of each instance
 // The frequency lives in device memory (FreqState), written by a one-thread
-// kernel before each replay.  Trailing-update GEMM tile shapes come from a
-// measured choice per size bucket (bfa_batch_tune).
+// kernel before each replay.  The trailing update of each size bucket is
+// one of three measured variants (bfa_batch_tune): complex64 GEMM tiles of
+// 64 x 64 or 128 x 128 on FP32 MFMA, or the split-FP16 product (hi + lo
+// planes per instance, then one grouped 128 x 128 FP16-MFMA launch).
 //
 // No reference counterpart: the reference runs one problem per process and
 // one frequency at a time (src/applications/buff-scatter/buff-scatter.cc:
