@@ -268,7 +268,9 @@ def run_forms_here(args, forms, bfa, torch, log):
     def step(form, phase, record):
         if record:
             ev[0].record()
-        g.assemble_vie(omega, mat, dM)
+        # the precision forms (mixed, fast) evaluate the far-field kernel with
+        # its remainder beyond second order in FP32 (--far-field fp64: as FP64)
+        g.assemble_vie(omega, mat, dM, far32=form != "fp64" and args.far_field == "fp32")
         g.assemble_rhs(omega, pw, dB)
         if record:
             ev[1].record()
@@ -356,7 +358,8 @@ def run_forms_in_children(args, forms, log):
             path = os.path.join(tmp, f"{form}.npz")
             cmd = [sys.executable, os.path.abspath(__file__), "--one-form", form, "--result-file", path,
                    "--steps", str(args.steps), "--warmup", str(args.warmup), "--n", str(args.n),
-                   "--td-order", str(args.td_order), "--refine-tol", repr(args.refine_tol), "--no-cpu-baseline"]
+                   "--td-order", str(args.td_order), "--refine-tol", repr(args.refine_tol), "--no-cpu-baseline",
+                   "--far-field", args.far_field]
             if args.quiet:
                 cmd.append("--quiet")
             if args.copy_calibration:
@@ -397,6 +400,9 @@ def main():
                     help="leave out the batched sweep that the default line (all forms, n = 12) carries as its "
                          "'batch' key")
     ap.add_argument("--refine-tol", type=float, default=1e-10)
+    ap.add_argument("--far-field", default="fp32", choices=["fp32", "fp64"],
+                    help="far-field kernel of the precision forms (mixed, fast): fp32 = the split kernel with its "
+                         "remainder in FP32 (the precision form), fp64 = the FP64 kernel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run a plain 1 GiB device-to-device copy (1 GiB read, 1 GiB written), against which "
@@ -494,6 +500,7 @@ def main():
         "speedup_vs_fp64": ref["s_per_freq"] / r["s_per_freq"] if ref else None,
         "data": "synthetic",
         "config": {"workload": "kuhn_sphere_n12_diag_eps_2rhs", "N": N, "tets": nt, "form": best,
+                   "far_field": args.far_field if best in ("mixed", "fast") else "fp64",
                    "omega": omega, "eps": "diag(4,5,6)+0.05i", "nrhs": len(spec), "td_order": args.td_order},
         "power_rel_err_vs_fp64": err,
         "speedup_mixed_over_fp64": (results["fp64"]["s_per_freq"] / results["mixed"]["s_per_freq"]
