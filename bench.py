@@ -8,7 +8,10 @@ unknowns), radius 1 um, interior nodes jittered 5 %, eps = diag(4,5,6)+0.05i,
 omega = 1 (3e14 rad/s), two plane waves (x- and y-polarised along +z).
 Per step: warm assembly of the complex128 VIE matrix (static table resident,
 built once before timing = the reference's FIBBI cache), LU factorisation,
-two-RHS solve.  Inputs are resident in HBM before the timed region.
+two-RHS solve.  Inputs are resident in HBM before the timed region.  The
+precision forms (mixed, fast) assemble with the far-field kernel's
+remainder beyond second order in FP32 (--far-field fp64 keeps it in FP64);
+their powers are held to 1e-8 of the fp64 form's.
 
 Each form (fp64, mixed, fast, graph) is timed in a process of its own, one
 after the other: the library's look-ahead streams of one form would
