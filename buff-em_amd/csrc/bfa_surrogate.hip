@@ -11,7 +11,7 @@
 // (face i of S, face j of T, SWG restrictions without the +/- sign) from:
 //   rule       the reference's 4-point product rule,
 //   cubature   a product TetCR rule on regularly subdivided tetrahedra (the
-//              accurate integral: training labels, fallback, judge),
+//              accurate integral: training labels, fallback, yardstick),
 //   surrogate  the 16-point product rule R16 plus the ensemble mean of an
 //              MLP that predicts R16's remaining error in units of the
 //              per-pair error estimate ||R16 - R4||, from the pair geometry

@@ -151,7 +151,7 @@ def _power(bfa, torch, g, mat, w, pw, assemble):
 @pytest.mark.gpu
 def test_surrogate_gain(bfa, n5, torch_cuda):
     """Trained surrogate (16-point rule plus the network's correction) versus
-    the 16-point rule and the reference's 4-point rule, judged by finer
+    the 16-point rule and the reference's 4-point rule, measured against finer
     cubature (33-point rule on tetrahedra refined twice): block errors over
     every pair at omega = 1, and absorbed and scattered powers for 32 plane
     waves at omega = 0.5, 1, 2, 4 (inside the training range, none of them a
