@@ -658,8 +658,35 @@ __device__ __forceinline__ double rsqrt_nr(double x)
 // 4-point product-rule moments of one tetrahedron pair (signed weights):
 // row tetrahedron us of the tile (points in sh), column tetrahedron's points
 // ys and centroid cT in registers; out[i * stride] for i < 16
-template <bool REALK, typename SH>
-__device__ __forceinline__ void tp_moments(const SH &sh, int us, const double *cS, const double *ys,
+// one point pair (x, y) of tp_moments: g(|x - y|) times the quadrature
+// weights, accumulated into the 16 moments
+template <bool REALK>
+__device__ __forceinline__ void tp_point_pair(const double *x, const double *y, double xi0, double xi1, double xi2,
+                                              const double *cT, double kre, double kim, double *m)
+{
+  const double R0 = x[0] - y[0], R1 = x[1] - y[1], R2 = x[2] - y[2];
+  const double r2 = R0 * R0 + R1 * R1 + R2 * R2, ir = rsqrt_nr(r2), r = r2 * ir;
+  double sn, co;
+  sincos(kre * r, &sn, &co);
+  double amp = x[3] * y[3] * ir;
+  if (!REALK) amp *= exp(-kim * r);
+  const double gr = amp * co, gi = amp * sn;
+  const double e0 = y[0] - cT[0], e1 = y[1] - cT[1], e2 = y[2] - cT[2];
+  const double xy = xi0 * e0 + xi1 * e1 + xi2 * e2;
+  m[0] += gr; m[1] += gi;
+  m[2] += gr * xi0; m[3] += gi * xi0; m[4] += gr * xi1; m[5] += gi * xi1; m[6] += gr * xi2; m[7] += gi * xi2;
+  m[8] += gr * e0; m[9] += gi * e0; m[10] += gr * e1; m[11] += gi * e1; m[12] += gr * e2; m[13] += gi * e2;
+  m[14] += gr * xy; m[15] += gi * xy;
+}
+
+// LDSY: the column tetrahedron's points are read from LDS (pts[1][.][ut])
+// two points at a time instead of from ys, which takes the table kernel
+// from 168 VGPRs to 122 (four waves per SIMD instead of three); the
+// single-pass kernel keeps ys (with LDSY it would spill).  The arithmetic
+// and its order are the same either way; the time did not change
+// measurably (the kernel is bound by its FP64 arithmetic).
+template <bool REALK, bool LDSY = false, typename SH>
+__device__ __forceinline__ void tp_moments(const SH &sh, int us, int ut, const double *cS, const double *ys,
                                            const double *cT, double kre, double kim, double *out, int stride)
 {
   double m[16];
@@ -673,22 +700,17 @@ __device__ __forceinline__ void tp_moments(const SH &sh, int us, const double *c
 #pragma unroll
     for (int e = 0; e < 4; e++) x[e] = sh.pts[0][4 * p + e][us];
     const double xi0 = x[0] - cS[0], xi1 = x[1] - cS[1], xi2 = x[2] - cS[2];
+    if constexpr (LDSY) {
+#pragma unroll 2
+      for (int q = 0; q < 4; q++) {
+        double y[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const double *y = ys + 4 * q;
-      const double R0 = x[0] - y[0], R1 = x[1] - y[1], R2 = x[2] - y[2];
-      const double r2 = R0 * R0 + R1 * R1 + R2 * R2, ir = rsqrt_nr(r2), r = r2 * ir;
-      double sn, co;
-      sincos(kre * r, &sn, &co);
-      double amp = x[3] * y[3] * ir;
-      if (!REALK) amp *= exp(-kim * r);
-      const double gr = amp * co, gi = amp * sn;
-      const double e0 = y[0] - cT[0], e1 = y[1] - cT[1], e2 = y[2] - cT[2];
-      const double xy = xi0 * e0 + xi1 * e1 + xi2 * e2;
-      m[0] += gr; m[1] += gi;
-      m[2] += gr * xi0; m[3] += gi * xi0; m[4] += gr * xi1; m[5] += gi * xi1; m[6] += gr * xi2; m[7] += gi * xi2;
-      m[8] += gr * e0; m[9] += gi * e0; m[10] += gr * e1; m[11] += gi * e1; m[12] += gr * e2; m[13] += gi * e2;
-      m[14] += gr * xy; m[15] += gi * xy;
+        for (int e = 0; e < 4; e++) y[e] = sh.pts[1][4 * q + e][ut];
+        tp_point_pair<REALK>(x, y, xi0, xi1, xi2, cT, kre, kim, m);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; q++) tp_point_pair<REALK>(x, ys + 4 * q, xi0, xi1, xi2, cT, kre, kim, m);
     }
   }
 #pragma unroll
@@ -819,7 +841,8 @@ __device__ __forceinline__ void split_e3(const SplitK &z, float w0, float w1, fl
 // tp_moments with g from the split above.  The column tetrahedron's points
 // are read from LDS (pts[1][.][ut]) one point at a time rather than from
 // ys: the real-k table kernel then holds 155 VGPRs (three waves per SIMD)
-// instead of 217 (two), with the same arithmetic in the same order.
+// instead of 217 (two), with the same arithmetic in the same order (the
+// kernel's time did not change measurably: it is bound by its arithmetic).
 template <bool REALK, typename SH>
 __device__ __forceinline__ void tp_moments_split(const SH &sh, int us, int ut, const double *cS, const double *ys,
                                                  const double *cT, double kre, double kim, double *out, int stride)
@@ -863,14 +886,14 @@ __device__ __forceinline__ void tp_moments_split(const SH &sh, int us, int ut, c
 }
 
 // one of the two moment routines
-template <bool REALK, bool SPLIT, typename SH>
+template <bool REALK, bool SPLIT, bool LDSY = false, typename SH>
 __device__ __forceinline__ void tp_moments_any(const SH &sh, int us, int ut, const double *cS, const double *ys,
                                                const double *cT, double kre, double kim, double *out, int stride)
 {
   if (SPLIT)
     tp_moments_split<REALK>(sh, us, ut, cS, ys, cT, kre, kim, out, stride);
   else
-    tp_moments<REALK>(sh, us, cS, ys, cT, kre, kim, out, stride);
+    tp_moments<REALK, LDSY>(sh, us, ut, cS, ys, cT, kre, kim, out, stride);
 }
 
 // points and centroid of distinct tetrahedron u of side `which` into registers
@@ -1037,7 +1060,7 @@ __device__ __forceinline__ void tp_table_tile(const DevGeom &g, double kre, doub
       double xs[16], cS[3], ys[16], cT[3];
       tp_load(sh, 0, us, xs, cS);
       tp_load(sh, 1, ut, ys, cT);
-      tp_moments_any<REALK, SPLIT>(sh, us, ut, cS, ys, cT, kre, kim, m, 1);
+      tp_moments_any<REALK, SPLIT, true>(sh, us, ut, cS, ys, cT, kre, kim, m, 1);
     }
     double2 *o = (double2 *)(tab + tp_index(S, T, nt) * 16);
 #pragma unroll
